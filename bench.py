@@ -1,0 +1,141 @@
+"""Benchmark: hpsi HBM throughput on one MI355X (Si64, 72^3 grid, 160 bands).
+
+A step is one Hamiltonian application H psi over all 160 bands of the Si64
+cell (2x2x2 diamond supercell, a = 20.52 bohr, atoms displaced by N(0, 0.05
+bohr), 13-point-per-axis FD kinetic, GTH Si projectors), with psi and H psi
+resident in HBM.  `value` is algorithmic GB/s:
+
+    B = 2 s N nb  (psi read, H psi written)  + s N (Vloc)  + (s_uv + 4) E
+
+with E the projector-table entries (uVk value + JJP index), divided by the
+measured time per call (HIP events on the library's stream).  The FP32 twin
+is reported beside it.  The CPU baseline is the FP64 oracle hpsi on the
+host's cores, on a bounded band sample of the same cell.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "s to SCF conv (ΔE<1e-6 Ha/atom) Si216, mixed+learned vs FP64; hpsi HBM GB/s"
+
+
+def hpsi_bytes(N: int, nb: int, E: int, s: int) -> float:
+    return 2.0 * s * N * nb + s * N + (s + 4) * E
+
+
+def gpu_leg(cell, vloc, steps: int, warmup: int, dtype):
+    from rsdft_develop_amd import gpu, hip, systems
+
+    ctx = gpu.Context(0).setup(cell, vloc)
+    tabs = ctx.projector_tables()
+    E = int(tabs["nentries"])
+    N, nb = cell.npoints, cell.nband
+    psi = hip.DeviceArray.from_numpy(systems.random_bands(N, nb, seed=2024), dtype=dtype)
+    hpsi = hip.DeviceArray((N, nb), dtype=dtype)
+    for _ in range(warmup):
+        ctx.hpsi_device(psi.ptr, hpsi.ptr, N, nb, dtype=dtype)
+    ctx.synchronize()
+    t0, t1 = hip.Event(), hip.Event()
+    wall0 = time.perf_counter()
+    t0.record(ctx.stream)
+    for _ in range(steps):
+        ctx.hpsi_device(psi.ptr, hpsi.ptr, N, nb, dtype=dtype)
+    t1.record(ctx.stream)
+    t1.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0.elapsed_ms(t1) / steps
+    s = 8 if dtype == np.float64 else 4
+    B = hpsi_bytes(N, nb, E, s)
+    psi.free()
+    hpsi.free()
+    ctx.close()
+    return dict(ms=ms, gbs=B / (ms * 1e-3) / 1e9, bytes=B, E=E, wall=wall)
+
+
+def cpu_leg(cell, vloc, target_s: float = 12.0):
+    """FP64 oracle hpsi on a band sample of the same cell, all host cores."""
+    from oracle import pyoracle
+    from rsdft_develop_amd import systems
+
+    cores = os.cpu_count() or 1
+    pyoracle.set_threads(cores)
+    ham = pyoracle.Ham(cell, vloc=vloc)
+    E = int(ham.tables()["mjj"].sum())
+    nbs = 8
+    psi = systems.random_bands(cell.npoints, nbs, seed=7)
+    ham.hpsi(psi)  # warm-up
+    t = time.perf_counter()
+    calls = 0
+    while True:
+        ham.hpsi(psi)
+        calls += 1
+        el = time.perf_counter() - t
+        if el >= target_s or calls >= 200:
+            break
+    per_call = el / calls
+    # scale the per-call time to the full 160-band workload (the oracle's
+    # cost is linear in bands; tables and Vloc are read once per call)
+    full = per_call * cell.nband / nbs
+    B = hpsi_bytes(cell.npoints, cell.nband, E, 8)
+    return dict(value=B / full / 1e9, unit="GB/s", cores=cores, kind="port",
+                sample=f"oracle hpsi FP64, {nbs} of {cell.nband} bands, {calls} calls in {el:.1f} s, "
+                       f"scaled linearly to {cell.nband} bands ({full * 1e3:.1f} ms/call)")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the path is not sharded in this build); got --gpus {args.gpus}",
+              file=sys.stderr)
+        return 2
+    from rsdft_develop_amd import systems
+
+    cell = systems.silicon_supercell(2, nband=160, displacement=0.05, seed=0)
+    vloc = systems.model_vloc(cell)
+    r64 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float64)
+    r32 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float32)
+    cpu = None if args.no_cpu else cpu_leg(cell, vloc)
+    out = {
+        "metric": METRIC,
+        "value": round(r64["gbs"], 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r64["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded N(0,1) bands, model Vloc, GTH Si tables)",
+        "config": {"workload": "hpsi Si64 2x2x2 a=20.52 bohr, 72^3 grid, 160 bands, Md=6",
+                   "grid": [72, 72, 72], "bands": 160, "projector_entries": r64["E"]},
+        "roofline": {"bound": "hbm", "achieved": round(r64["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r64["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "hpsi call (k_nl_gather + k_kinetic_nl)",
+                     "algorithmic_bytes": r64["bytes"]},
+        "f32": {"value": round(r32["gbs"], 1), "ms_per_step": round(r32["ms"], 4),
+                "frac": round(r32["gbs"] / HBM_PEAK_GBS, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

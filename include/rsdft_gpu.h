@@ -1,0 +1,135 @@
+/*
+ * rsdft_gpu.h -- C ABI of the MI355X (gfx950) Kohn-Sham inner loop.
+ *
+ * One context owns one GPU's copy of the operator state (grid, FD
+ * coefficients, local potential, projector tables) and the device-resident
+ * wavefunction store, the way RSDFT's module globals own them on the CPU.
+ * Every entry point returns 0 on success and a nonzero RSG_ERR_* code on
+ * failure; rsg_last_error() then names the cause.  Calls on one context are
+ * not re-entrant; work is queued on the context's own HIP stream and entry
+ * points that return host-visible results synchronise that stream.
+ *
+ * Layout conventions (identical to the reference):
+ *   grid point (i1,i2,i3), 0-based, j = i1 + i2*N1 + i3*N1*N2 (x fastest),
+ *     kinetic/d_kinetic_sol_module.f90:129;
+ *   band blocks are column-major psi[j + ib*ld], ld >= N, as tpsi(ML0,nb)
+ *     in hamiltonian_module.f90:118-122;
+ *   lattice aa[3*i + r] = component r of lattice vector i (aa(r,i));
+ *   projector tables JJP(maxMJJ,nzlma) / uVk(maxMJJ,nzlma) column-major with
+ *     0-based JJP, MJJ(nzlma), iuV(nzlma), ps_nloc2_variables.f90.
+ */
+#ifndef RSDFT_GPU_H
+#define RSDFT_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSG_OK 0
+#define RSG_ERR_ARG 1      /* bad argument or size */
+#define RSG_ERR_HIP 2      /* HIP runtime / library failure */
+#define RSG_ERR_STATE 3    /* call out of order (e.g. no grid yet) */
+#define RSG_ERR_NOTPD 4    /* overlap matrix not positive definite */
+#define RSG_ERR_SOLVER 5   /* dense eigensolver failed */
+#define RSG_ERR_UNSUPPORTED 6
+
+typedef struct rsg_ctx rsg_ctx;
+
+/* Context lifetime.  No reference counterpart (module initialisation in
+ * main.f90:266-388 plays this role). */
+int rsg_create(rsg_ctx **ctx, int device);
+int rsg_destroy(rsg_ctx *ctx);
+const char *rsg_last_error(const rsg_ctx *ctx);
+int rsg_synchronize(rsg_ctx *ctx);
+/* The context's hipStream_t, for callers that record events on it. */
+void *rsg_stream(rsg_ctx *ctx);
+
+/* Grid and FD kinetic coefficients.  Replaces Init_Rgrid
+ * (rgrid_sol_module.f90:26-39: Hgrid, dV) and init_kinetic
+ * (kinetic/kinetic_module.f90:28-180; get_coef_lapla_fd, fd_module.f90:18-54).
+ * Orthorhombic cells only (the Gamma-path cross terms of
+ * d_kinetic_sol_module.f90:351-529 vanish). */
+int rsg_set_grid(rsg_ctx *ctx, int n1, int n2, int n3, const double *aa,
+                 int md);
+/* dv, hgrid[3], coef_lap0 and coef_lap[axis*md + m-1] as the context uses
+ * them. */
+int rsg_get_grid(rsg_ctx *ctx, double *dv, double *hgrid, double *coef_lap0,
+                 double *coef_lap);
+
+/* Local potential Vloc(:,s) (host array of N doubles).  Replaces the module
+ * array of localpot_module.f90:14 set by update_localpot (:209-215). */
+int rsg_set_vloc(rsg_ctx *ctx, const double *vloc);
+
+/* Projector tables in the reference's layout (ps_nloc2_variables.f90:
+ * MJJ, JJP, d_uVk, iuV) plus an optional lma -> atom map (amap, may be
+ * NULL) that lets the context gather the projectors of one atom together. */
+int rsg_set_projectors(rsg_ctx *ctx, int nzlma, int maxmjj, const int *mjj,
+                       const int *jjp, const double *uvk, const int *iuv,
+                       const int *amap);
+/* Build the GTH tables for a set of atoms.  Replaces prep_ps_nloc2 for
+ * ippform 4 (ps_nloc2_module.f90:42-1167 with prep_ps_nloc_gth,
+ * ps_nloc_gth_module.f90:19-180, and Ylm, ylm_module.f90:11-73).
+ * frac[3*a+i] fractional positions, kind[a] species; per-species arrays are
+ * [kind*4 + iorb] (norb[kind] <= 4), h < 0 flips iuV. */
+int rsg_build_gth_projectors(rsg_ctx *ctx, int natom, const double *frac,
+                             const int *kind, int nkind, const int *norb,
+                             const int *lo, const int *no, const double *rps,
+                             const double *hnl);
+int rsg_get_projector_sizes(rsg_ctx *ctx, int *nzlma, int *maxmjj,
+                            int64_t *nentries);
+int rsg_get_projectors(rsg_ctx *ctx, int *mjj, int *jjp, double *uvk,
+                       int *iuv);
+
+/* hpsi on device buffers: hpsi = (T + Vloc) psi + V_NL psi, overwriting
+ * hpsi.  Replaces d_hamiltonian1 (hamiltonian_module.f90:118-144): the fused
+ * kinetic+local stencil (d_kinetic_sol_module.f90:244-292) and
+ * d_op_ps_nloc2_hp (d_ps_nloc2_op_module.f90:146-288).  psi and hpsi must
+ * not overlap.  Asynchronous on the context stream. */
+int rsg_hpsi_f64(rsg_ctx *ctx, const double *d_psi, double *d_hpsi,
+                 int64_t ld, int nb);
+/* The same operator in FP32 storage with FP32 accumulation (tables rounded
+ * to FP32).  No reference counterpart (mixed-precision addition). */
+int rsg_hpsi_f32(rsg_ctx *ctx, const float *d_psi, float *d_hpsi, int64_t ld,
+                 int nb);
+/* hpsi on host arrays (copied through device scratch); the entry point the
+ * Fortran `hamiltonian` shim binds.  Synchronous. */
+int rsg_hpsi_f64_host(rsg_ctx *ctx, const double *psi, double *hpsi,
+                      int64_t ld, int nb);
+
+/* Device-resident wavefunction store, the counterpart of
+ * wf_module::unk(ML_0:ML_1, MB, k, s) (wf_module.f90:131). */
+int rsg_alloc_bands(rsg_ctx *ctx, int mb);
+int rsg_upload_bands(rsg_ctx *ctx, const double *u, int64_t ld, int n0,
+                     int nb);
+int rsg_download_bands(rsg_ctx *ctx, double *u, int64_t ld, int n0, int nb);
+int rsg_bands_device_ptr(rsg_ctx *ctx, double **d_u, int64_t *ld);
+
+/* Orthonormalise bands n0..n1 (1-based, inclusive) of the store under
+ * <u,v> = dV u^T v.  Replaces gram_schmidt(n0,n1,k,s)
+ * (gram_schmidt/gram_schmidt_module.f90:35-79) with the Cholesky-QR form of
+ * d_gram_schmidt_lusl (gram_schmidt_lusl_module.f90:241-328): S = dV U^T U,
+ * S = R^T R, U <- U R^-1.  RSG_ERR_NOTPD if S is not positive definite. */
+int rsg_gram_schmidt(rsg_ctx *ctx, int n0, int n1);
+
+/* Rayleigh-Ritz on the whole store.  Replaces subspace_diag(k,s,unk,esp)
+ * (subspace_diag/subspace_diag_module.f90:25-65 -> subspace_diag_la,
+ * subspace_diag_la_module.f90:18-223).  esp (host, mb) ascending. */
+int rsg_subspace_diag(rsg_ctx *ctx, double *esp);
+
+/* Band-block CG update of the whole store.  Replaces
+ * conjugate_gradient(n1,n2,k,s,unk,esp,res) (conjugate_gradient/
+ * cg_module.f90:53-431).  pc_kind 4 = hprsdft 7-point preconditioner with
+ * mloop steps (cgpc_hprsdft_module.f90:133-361), 0 = none.  esp, res (host,
+ * mb). */
+int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop,
+                           double *esp, double *res);
+
+/* e_n = dV <psi_n|H psi_n>.  Replaces d_esp_calc (esp_calc_module.f90:17-68). */
+int rsg_esp_calc(rsg_ctx *ctx, double *esp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -223,6 +223,7 @@ int orc_ham_build_gth(orc_ham *h, int natom, const double *frac,
     for (int io = 0; io < norb[kind[a]]; ++io, ++q)
       if (npts[q] > 0) nz += 2 * lo[kind[a] * ORC_MAXORB + io] + 1;
 
+  /* capacity per lma while filling: the largest sphere */
   int maxmjj = 0;
   for (q = 0; q < nsph; ++q)
     if (npts[q] > maxmjj) maxmjj = npts[q];
@@ -290,6 +291,18 @@ int orc_ham_build_gth(orc_ham *h, int natom, const double *frac,
       }
     }
   }
+  /* JJP/uVk leading dimension is maxval(MJJ) after the |uV| <= 1e-14 drop
+   * (d_allocate_ps_nloc2, ps_nloc2_variables.f90:153-161) */
+  int mx = 0;
+  for (int l = 0; l < nz; ++l)
+    if (h->mjj[l] > mx) mx = h->mjj[l];
+  if (mx < maxmjj && nz > 0) {
+    for (int l = 1; l < nz; ++l) {
+      memmove(h->jjp + (size_t)l * mx, h->jjp + (size_t)l * maxmjj, sizeof(int) * mx);
+      memmove(h->uvk + (size_t)l * mx, h->uvk + (size_t)l * maxmjj, sizeof(double) * mx);
+    }
+  }
+  h->maxmjj = mx;
   free(mark);
   free(pairs);
   free(tmpv);

@@ -1,0 +1,306 @@
+// capi.cpp -- the extern "C" entry points declared in include/rsdft_gpu.h.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "context.hpp"
+#include "solver.hpp"
+
+using rsg::DevBuf;
+
+extern "C" {
+
+int rsg_create(rsg_ctx **out, int device) {
+  if (!out) return RSG_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RSG_ERR_HIP;
+  if (device < 0 || device >= ndev) return RSG_ERR_ARG;
+  rsg_ctx *ctx = new (std::nothrow) rsg_ctx();
+  if (!ctx) return RSG_ERR_HIP;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+    delete ctx;
+    return RSG_ERR_HIP;
+  }
+  if (rocblas_create_handle(&ctx->blas) != rocblas_status_success ||
+      rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RSG_ERR_HIP;
+  }
+  *out = ctx;
+  return RSG_OK;
+}
+
+int rsg_destroy(rsg_ctx *ctx) {
+  if (!ctx) return RSG_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  DevBuf *bufs[] = {&ctx->vloc64, &ctx->vloc32, &ctx->uvunk, &ctx->unk, &ctx->work,
+                    &ctx->hsub, &ctx->esp_dev, &ctx->info, &ctx->cgbuf, &ctx->scal,
+                    &ctx->hostio};
+  for (DevBuf *b : bufs) b->release();
+  for (auto *nl : {&ctx->nl64.grp_off, &ctx->nl64.grp_lma0, &ctx->nl64.grp_nl,
+                   &ctx->nl64.grp_pts, &ctx->nl64.grp_coef, &ctx->nl64.scale,
+                   &ctx->nl64.pt_off, &ctx->nl64.pt_lma, &ctx->nl64.pt_uv,
+                   &ctx->nl32.grp_off, &ctx->nl32.grp_lma0, &ctx->nl32.grp_nl,
+                   &ctx->nl32.grp_pts, &ctx->nl32.grp_coef, &ctx->nl32.scale,
+                   &ctx->nl32.pt_off, &ctx->nl32.pt_lma, &ctx->nl32.pt_uv})
+    nl->release();
+  rsg::solver_release(ctx);
+  if (ctx->blas) rocblas_destroy_handle(ctx->blas);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RSG_OK;
+}
+
+const char *rsg_last_error(const rsg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rsg_synchronize(rsg_ctx *ctx) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+void *rsg_stream(rsg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int rsg_set_grid(rsg_ctx *ctx, int n1, int n2, int n3, const double *aa, int md) {
+  if (!ctx || !aa) return RSG_ERR_ARG;
+  if (n1 < 1 || n2 < 1 || n3 < 1 || md < 1 || md > 8)
+    return ctx->fail(RSG_ERR_ARG, "set_grid: grid sizes must be >= 1 and md in 1..8");
+  const int64_t N = int64_t(n1) * n2 * n3;
+  if (N > (int64_t(1) << 31) - 1) return ctx->fail(RSG_ERR_ARG, "set_grid: grid above 2^31 points");
+  RSG_HIP(hipSetDevice(ctx->device));
+  double len[3];
+  for (int i = 0; i < 3; ++i)
+    len[i] = std::sqrt(aa[3 * i] * aa[3 * i] + aa[3 * i + 1] * aa[3 * i + 1] + aa[3 * i + 2] * aa[3 * i + 2]);
+  // orthorhombic only: the lattice vectors must be mutually orthogonal
+  for (int i = 0; i < 3; ++i)
+    for (int k = i + 1; k < 3; ++k) {
+      double d = aa[3 * i] * aa[3 * k] + aa[3 * i + 1] * aa[3 * k + 1] + aa[3 * i + 2] * aa[3 * k + 2];
+      if (std::fabs(d) > 1e-12 * len[i] * len[k])
+        return ctx->fail(RSG_ERR_UNSUPPORTED, "set_grid: non-orthogonal cells are not supported");
+    }
+  const double det = aa[0] * (aa[4] * aa[8] - aa[5] * aa[7]) - aa[1] * (aa[3] * aa[8] - aa[5] * aa[6]) +
+                     aa[2] * (aa[3] * aa[7] - aa[4] * aa[6]);
+  if (!(std::fabs(det) > 0)) return ctx->fail(RSG_ERR_ARG, "set_grid: singular lattice");
+  ctx->n[0] = n1;
+  ctx->n[1] = n2;
+  ctx->n[2] = n3;
+  ctx->N = N;
+  ctx->md = md;
+  std::memcpy(ctx->aa, aa, sizeof(ctx->aa));
+  const int ng[3] = {n1, n2, n3};
+  for (int i = 0; i < 3; ++i) ctx->hgrid[i] = len[i] / ng[i];
+  ctx->dv = std::fabs(det) / double(N);
+  for (int i = 0; i < 6; ++i) ctx->ggg[i] = i < 3 ? 1.0 : 0.0;  // orthorhombic metric
+  double lap[2 * 8 + 1];
+  rsg::fd_lap_closed_form(md, lap);
+  ctx->coef_lap.assign(size_t(3) * md, 0.0);
+  double csum = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double c = -0.5 * ctx->ggg[a] / (ctx->hgrid[a] * ctx->hgrid[a]);
+    csum += c;
+    for (int m = 1; m <= md; ++m) ctx->coef_lap[size_t(a) * md + m - 1] = lap[md + m] * c;
+  }
+  ctx->coef_lap0 = lap[md] * csum;
+  if (ctx->vloc64.ensure(size_t(N) * 8) != hipSuccess || ctx->vloc32.ensure(size_t(N) * 4) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "set_grid: allocation failed");
+  RSG_HIP(hipMemsetAsync(ctx->vloc64.p, 0, size_t(N) * 8, ctx->stream));
+  RSG_HIP(hipMemsetAsync(ctx->vloc32.p, 0, size_t(N) * 4, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->have_grid = true;
+  ctx->have_vloc = false;
+  ctx->have_nl = false;
+  ctx->tables = rsg::HostTables();
+  ctx->mb = 0;
+  return RSG_OK;
+}
+
+int rsg_get_grid(rsg_ctx *ctx, double *dv, double *hgrid, double *c0, double *cl) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "get_grid: grid not set");
+  if (dv) *dv = ctx->dv;
+  if (hgrid)
+    for (int i = 0; i < 3; ++i) hgrid[i] = ctx->hgrid[i];
+  if (c0) *c0 = ctx->coef_lap0;
+  if (cl) std::memcpy(cl, ctx->coef_lap.data(), ctx->coef_lap.size() * sizeof(double));
+  return RSG_OK;
+}
+
+int rsg_set_vloc(rsg_ctx *ctx, const double *vloc) {
+  if (!ctx || !vloc) return RSG_ERR_ARG;
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "set_vloc: grid not set");
+  RSG_HIP(hipSetDevice(ctx->device));
+  std::vector<float> v32(vloc, vloc + ctx->N);
+  RSG_HIP(hipMemcpyAsync(ctx->vloc64.p, vloc, size_t(ctx->N) * 8, hipMemcpyHostToDevice, ctx->stream));
+  RSG_HIP(hipMemcpyAsync(ctx->vloc32.p, v32.data(), size_t(ctx->N) * 4, hipMemcpyHostToDevice, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->have_vloc = true;
+  return RSG_OK;
+}
+
+int rsg_set_projectors(rsg_ctx *ctx, int nzlma, int maxmjj, const int *mjj, const int *jjp,
+                       const double *uvk, const int *iuv, const int *amap) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "set_projectors: grid not set");
+  if (nzlma < 0 || maxmjj < 0 || (nzlma > 0 && (!mjj || !jjp || !uvk || !iuv)))
+    return ctx->fail(RSG_ERR_ARG, "set_projectors: bad sizes or null arrays");
+  rsg::HostTables t;
+  t.nzlma = nzlma;
+  t.maxmjj = maxmjj;
+  t.mjj.assign(mjj, mjj + nzlma);
+  t.iuv.assign(iuv, iuv + nzlma);
+  t.jjp.assign(jjp, jjp + size_t(nzlma) * maxmjj);
+  t.uvk.assign(uvk, uvk + size_t(nzlma) * maxmjj);
+  if (amap) t.amap.assign(amap, amap + nzlma);
+  for (int l = 0; l < nzlma; ++l) {
+    if (t.mjj[l] < 0 || t.mjj[l] > maxmjj) return ctx->fail(RSG_ERR_ARG, "set_projectors: MJJ out of range");
+    for (int j = 0; j < t.mjj[l]; ++j) {
+      int p = t.jjp[size_t(l) * maxmjj + j];
+      if (p < 0 || p >= ctx->N) return ctx->fail(RSG_ERR_ARG, "set_projectors: JJP outside the grid");
+      if (j > 0 && p <= t.jjp[size_t(l) * maxmjj + j - 1])
+        return ctx->fail(RSG_ERR_ARG, "set_projectors: JJP not strictly ascending within an lma");
+    }
+  }
+  RSG_HIP(hipSetDevice(ctx->device));
+  ctx->tables = std::move(t);
+  ctx->have_nl = false;
+  return rsg::upload_nonlocal(ctx);
+}
+
+int rsg_build_gth_projectors(rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind,
+                             const int *norb, const int *lo, const int *no, const double *rps,
+                             const double *hnl) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "build_gth: grid not set");
+  if (natom < 0 || nkind < 1 || (natom > 0 && (!frac || !kind)) || !norb || !lo || !no || !rps || !hnl)
+    return ctx->fail(RSG_ERR_ARG, "build_gth: bad arguments");
+  RSG_HIP(hipSetDevice(ctx->device));
+  rsg::HostTables t;
+  int rc = rsg::build_gth_tables(ctx, natom, frac, kind, nkind, norb, lo, no, rps, hnl, t);
+  if (rc) return ctx->fail(rc, "build_gth: invalid species parameters");
+  ctx->tables = std::move(t);
+  ctx->have_nl = false;
+  return rsg::upload_nonlocal(ctx);
+}
+
+int rsg_get_projector_sizes(rsg_ctx *ctx, int *nzlma, int *maxmjj, int64_t *nent) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (nzlma) *nzlma = ctx->tables.nzlma;
+  if (maxmjj) *maxmjj = ctx->tables.maxmjj;
+  if (nent) *nent = ctx->tables.nentries();
+  return RSG_OK;
+}
+
+int rsg_get_projectors(rsg_ctx *ctx, int *mjj, int *jjp, double *uvk, int *iuv) {
+  if (!ctx) return RSG_ERR_ARG;
+  const rsg::HostTables &t = ctx->tables;
+  if (mjj) std::memcpy(mjj, t.mjj.data(), t.mjj.size() * sizeof(int));
+  if (jjp) std::memcpy(jjp, t.jjp.data(), t.jjp.size() * sizeof(int));
+  if (uvk) std::memcpy(uvk, t.uvk.data(), t.uvk.size() * sizeof(double));
+  if (iuv) std::memcpy(iuv, t.iuv.data(), t.iuv.size() * sizeof(int));
+  return RSG_OK;
+}
+
+int rsg_hpsi_f64(rsg_ctx *ctx, const double *psi, double *hpsi, int64_t ld, int nb) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::launch_hpsi<double>(ctx, psi, hpsi, ld, nb);
+}
+
+int rsg_hpsi_f32(rsg_ctx *ctx, const float *psi, float *hpsi, int64_t ld, int nb) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::launch_hpsi<float>(ctx, psi, hpsi, ld, nb);
+}
+
+int rsg_hpsi_f64_host(rsg_ctx *ctx, const double *psi, double *hpsi, int64_t ld, int nb) {
+  if (!ctx || !psi || !hpsi) return RSG_ERR_ARG;
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "hpsi: grid not set");
+  if (nb <= 0) return nb == 0 ? RSG_OK : ctx->fail(RSG_ERR_ARG, "hpsi: nb < 0");
+  if (ld < ctx->N) return ctx->fail(RSG_ERR_ARG, "hpsi: ld < N");
+  RSG_HIP(hipSetDevice(ctx->device));
+  const size_t blk = size_t(ctx->N) * nb * sizeof(double);
+  if (ctx->hostio.ensure(2 * blk) != hipSuccess) return ctx->fail(RSG_ERR_HIP, "hpsi: staging allocation failed");
+  double *dp = ctx->hostio.as<double>(), *dh = dp + size_t(ctx->N) * nb;
+  RSG_HIP(hipMemcpy2DAsync(dp, ctx->N * 8, psi, ld * 8, ctx->N * 8, nb, hipMemcpyHostToDevice, ctx->stream));
+  int rc = rsg::launch_hpsi<double>(ctx, dp, dh, ctx->N, nb);
+  if (rc) return rc;
+  RSG_HIP(hipMemcpy2DAsync(hpsi, ld * 8, dh, ctx->N * 8, ctx->N * 8, nb, hipMemcpyDeviceToHost, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+int rsg_alloc_bands(rsg_ctx *ctx, int mb) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "alloc_bands: grid not set");
+  if (mb < 1) return ctx->fail(RSG_ERR_ARG, "alloc_bands: mb < 1");
+  RSG_HIP(hipSetDevice(ctx->device));
+  const size_t bytes = size_t(ctx->N) * mb * sizeof(double);
+  if (ctx->unk.ensure(bytes) != hipSuccess || ctx->work.ensure(bytes) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "alloc_bands: out of device memory");
+  RSG_HIP(hipMemsetAsync(ctx->unk.p, 0, bytes, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->mb = mb;
+  return RSG_OK;
+}
+
+int rsg_upload_bands(rsg_ctx *ctx, const double *u, int64_t ld, int n0, int nb) {
+  if (!ctx || !u) return RSG_ERR_ARG;
+  if (ctx->mb == 0) return ctx->fail(RSG_ERR_STATE, "upload_bands: no band store");
+  if (n0 < 0 || nb < 0 || n0 + nb > ctx->mb || ld < ctx->N)
+    return ctx->fail(RSG_ERR_ARG, "upload_bands: band range or ld out of bounds");
+  RSG_HIP(hipSetDevice(ctx->device));
+  double *d = ctx->unk.as<double>() + size_t(n0) * ctx->N;
+  RSG_HIP(hipMemcpy2DAsync(d, ctx->N * 8, u, ld * 8, ctx->N * 8, nb, hipMemcpyHostToDevice, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+int rsg_download_bands(rsg_ctx *ctx, double *u, int64_t ld, int n0, int nb) {
+  if (!ctx || !u) return RSG_ERR_ARG;
+  if (ctx->mb == 0) return ctx->fail(RSG_ERR_STATE, "download_bands: no band store");
+  if (n0 < 0 || nb < 0 || n0 + nb > ctx->mb || ld < ctx->N)
+    return ctx->fail(RSG_ERR_ARG, "download_bands: band range or ld out of bounds");
+  RSG_HIP(hipSetDevice(ctx->device));
+  const double *d = ctx->unk.as<double>() + size_t(n0) * ctx->N;
+  RSG_HIP(hipMemcpy2DAsync(u, ld * 8, d, ctx->N * 8, ctx->N * 8, nb, hipMemcpyDeviceToHost, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+int rsg_bands_device_ptr(rsg_ctx *ctx, double **d_u, int64_t *ld) {
+  if (!ctx || !d_u) return RSG_ERR_ARG;
+  if (ctx->mb == 0) return ctx->fail(RSG_ERR_STATE, "bands_device_ptr: no band store");
+  *d_u = ctx->unk.as<double>();
+  if (ld) *ld = ctx->N;
+  return RSG_OK;
+}
+
+int rsg_gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::gram_schmidt(ctx, n0, n1);
+}
+
+int rsg_subspace_diag(rsg_ctx *ctx, double *esp) {
+  if (!ctx || !esp) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::subspace_diag(ctx, esp);
+}
+
+int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res) {
+  if (!ctx || !esp || !res) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::conjugate_gradient(ctx, ncg, pc_kind, mloop, esp, res);
+}
+
+int rsg_esp_calc(rsg_ctx *ctx, double *esp) {
+  if (!ctx || !esp) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::esp_calc(ctx, esp);
+}
+
+}  // extern "C"

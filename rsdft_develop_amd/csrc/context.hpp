@@ -1,0 +1,124 @@
+// context.hpp -- internal state of an rsg_ctx (one GPU, one rank).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "rsdft_gpu.h"
+
+namespace rsg {
+
+// Device buffer owned by the context (hipMalloc/hipFree).
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t need);
+  void release();
+  template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Nonlocal projector operator in the two layouts the kernels stream.
+//  gather: groups of <= 4 projectors that share one point list (all
+//          projectors of one atom), coef[pt*4 + l] (zero-padded);
+//  scatter: per-grid-point CSR of (lma, coefficient) entries.
+template <typename T> struct NonlocalDev {
+  int ngroup = 0;
+  DevBuf grp_off;   // int[ngroup+1] into the group point list
+  DevBuf grp_lma0;  // int[ngroup]   first lma of the group
+  DevBuf grp_nl;    // int[ngroup]   projectors in the group (<= 4)
+  DevBuf grp_pts;   // int[npts]     grid index per group point
+  DevBuf grp_coef;  // T[npts*4]
+  DevBuf scale;     // T[nzlma]      iuV * dV
+  DevBuf pt_off;    // int[N+1]
+  DevBuf pt_lma;    // int[E]
+  DevBuf pt_uv;     // T[E]
+};
+
+struct HostTables {
+  int nzlma = 0, maxmjj = 0;
+  std::vector<int> mjj, jjp, iuv, amap;  // jjp/uvk column-major (maxmjj,nzlma)
+  std::vector<double> uvk;
+  int64_t nentries() const {
+    int64_t e = 0;
+    for (int v : mjj) e += v;
+    return e;
+  }
+};
+
+}  // namespace rsg
+
+struct rsg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  std::string err;
+
+  // grid
+  bool have_grid = false;
+  int n[3] = {0, 0, 0};
+  int64_t N = 0;
+  int md = 0;
+  double aa[9];
+  double hgrid[3], dv = 0.0, ggg[6];
+  double coef_lap0 = 0.0;
+  std::vector<double> coef_lap;  // [axis*md + m-1]
+
+  rsg::DevBuf vloc64, vloc32;
+  bool have_vloc = false;
+
+  rsg::HostTables tables;
+  rsg::NonlocalDev<double> nl64;
+  rsg::NonlocalDev<float> nl32;
+  bool have_nl = false;
+  rsg::DevBuf uvunk;  // scratch: nzlma x nb partial projections
+
+  // wavefunction store and work arrays
+  int mb = 0;
+  rsg::DevBuf unk, work;           // N x mb each (work: rotation target)
+  rsg::DevBuf hsub, esp_dev, info; // mb x mb dense, mb, int
+  rsg::DevBuf cgbuf;               // CG work vectors
+  rsg::DevBuf scal;                // small per-band scalars
+  rsg::DevBuf hostio;              // staging for the host hpsi entry
+  rsg::DevBuf dotpart;             // multidot partial sums
+
+  int fail(int code, const std::string &msg) {
+    err = msg;
+    return code;
+  }
+};
+
+#define RSG_HIP(call)                                                      \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess)                                                  \
+      return ctx->fail(RSG_ERR_HIP, std::string(#call ": ") +              \
+                                        hipGetErrorString(e_));            \
+  } while (0)
+
+#define RSG_BLAS(call)                                                     \
+  do {                                                                     \
+    rocblas_status s_ = (call);                                            \
+    if (s_ != rocblas_status_success)                                      \
+      return ctx->fail(RSG_ERR_HIP, std::string(#call ": ") +              \
+                                        rocblas_status_to_string(s_));     \
+  } while (0)
+
+// tables.cpp
+namespace rsg {
+void fd_lap_closed_form(int md, double *lap);
+int build_gth_tables(const rsg_ctx *ctx, int natom, const double *frac,
+                     const int *kind, int nkind, const int *norb,
+                     const int *lo, const int *no, const double *rps,
+                     const double *hnl, HostTables &out);
+int upload_nonlocal(rsg_ctx *ctx);
+}  // namespace rsg
+
+// hpsi.hip
+namespace rsg {
+template <typename T>
+int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb);
+}

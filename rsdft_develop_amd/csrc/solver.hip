@@ -1,0 +1,544 @@
+// solver.hip -- Gram-Schmidt (Cholesky-QR), Rayleigh-Ritz, band CG with the
+// hprsdft preconditioner, and per-band eigenvalues on the device band store.
+//
+// Dense contractions with K = N (overlap, Hsub, rotation, R^-1 apply) go to
+// rocBLAS (MFMA FP64 on gfx950); the small dense factorisations to rocSOLVER.
+// Band-wise vector work, dot products and the per-band 2x2 Ritz step are the
+// kernels below.
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "solver.hpp"
+
+namespace rsg {
+namespace {
+
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------- reductions
+struct DotArgs {
+  const double *x[6];
+  const double *y[6];
+};
+
+template <int NP>
+__global__ __launch_bounds__(kThreads) void k_multidot(DotArgs a, int64_t N, int nb, int nseg,
+                                                       int64_t seglen, double *partial) {
+  __shared__ double red[kThreads / 64][NP];
+  const int s = blockIdx.x, b = blockIdx.y;
+  const int64_t i0 = int64_t(s) * seglen, i1 = min(N, i0 + seglen);
+  double acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = 0.0;
+  const int64_t off = int64_t(b) * N;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) acc[q] += a.x[q][off + i] * a.y[q][off + i];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    double v = acc[q];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wid][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NP) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < kThreads / 64; ++k) v += red[k][threadIdx.x];
+    partial[(int64_t(threadIdx.x) * nb + b) * nseg + s] = v;
+  }
+}
+
+__global__ void k_reduce_partials(const double *__restrict__ partial, int nseg, int count,
+                                  double alpha, double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  double v = 0.0;
+  for (int s = 0; s < nseg; ++s) v += partial[int64_t(t) * nseg + s];
+  out[t] = alpha * v;
+}
+
+// --------------------------------------------------------- band-wise updates
+// gk = -2 (hx - E x)          (cg_module.f90:207)
+__global__ void k_cg_grad(const double *__restrict__ hx, const double *__restrict__ x,
+                          const double *__restrict__ E, double *__restrict__ g, int64_t N) {
+  const int b = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t k = int64_t(b) * N + i;
+  g[k] = -2.0 * (hx[k] - E[b] * x[k]);
+}
+
+// y = x (first step) or y = x + (rb/gkgk) y   (cg_module.f90:266-279)
+__global__ void k_cg_dir(const double *__restrict__ pg, double *__restrict__ pk,
+                         const double *__restrict__ rb, const double *__restrict__ gkgk, int first,
+                         int64_t N) {
+  const int b = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t k = int64_t(b) * N + i;
+  pk[k] = first ? pg[k] : pg[k] + (rb[b] / gkgk[b]) * pk[k];
+}
+
+// 2x2 generalised Ritz step per band (cg_module.f90:305-334): DSYGV on
+// (<x,p|H|x,p>, <x,p|x,p>), root switch, sign fix.  d6 holds
+// (xx, px, pp, xHx, pHx, pHp) * dV as [q*nb + b].
+__global__ void k_cg_ritz(const double *__restrict__ d6, int nb, double *__restrict__ E,
+                          double *__restrict__ E1, double *__restrict__ uc) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double b11 = d6[0 * nb + b], b12 = d6[1 * nb + b], b22 = d6[2 * nb + b];
+  const double a11 = d6[3 * nb + b], a12 = d6[4 * nb + b], a22 = d6[5 * nb + b];
+  // B = U^T U, C = U^-T A U^-1, C = Y diag(w) Y^T, X = U^-1 Y
+  const double u11 = sqrt(b11), u12 = b12 / u11, u22 = sqrt(b22 - u12 * u12);
+  const double i11 = 1.0 / u11, i12 = -u12 / (u11 * u22), i22 = 1.0 / u22;
+  const double t11 = a11 * i11, t12 = a11 * i12 + a12 * i22, t22 = a12 * i12 + a22 * i22;
+  const double c11 = i11 * t11, c12 = i11 * t12, c22 = i12 * t12 + i22 * t22;
+  double y11, y12, y21, y22, l1, l2;
+  if (c12 == 0.0) {
+    y11 = 1; y21 = 0; y12 = 0; y22 = 1; l1 = c11; l2 = c22;
+  } else {
+    const double th = (c22 - c11) / (2.0 * c12);
+    const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+    const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+    l1 = c11 - t * c12;
+    l2 = c22 + t * c12;
+    y11 = c; y21 = -s; y12 = s; y22 = c;
+  }
+  if (l2 < l1) {
+    double tl = l1; l1 = l2; l2 = tl;
+    double p = y11, q = y21;
+    y11 = y12; y21 = y22; y12 = p; y22 = q;
+  }
+  double v1 = i11 * y11 + i12 * y21, v2 = i22 * y21;
+  const double w1 = i11 * y12 + i12 * y22, w2 = i22 * y22;
+  double W = l1;
+  const double e = E[b];
+  if (fabs(l1 - e) > 1e-1 && fabs(l2 - e) <= 1e-1) {
+    v1 = w1;
+    v2 = w2;
+    W = l2;
+  }
+  if (v1 < 0.0) {
+    v1 = -v1;
+    v2 = -v2;
+  }
+  uc[2 * b] = v1;
+  uc[2 * b + 1] = v2;
+  E1[b] = e;
+  E[b] = W;
+}
+
+// hx <- u1 hx + u2 hp ; g <- -2 (hx - W (u1 x + u2 p))   (cg_module.f90:336-352)
+__global__ void k_cg_update(double *__restrict__ hx, const double *__restrict__ hp,
+                            const double *__restrict__ x, const double *__restrict__ p,
+                            double *__restrict__ g, const double *__restrict__ uc,
+                            const double *__restrict__ E, int64_t N) {
+  const int b = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t k = int64_t(b) * N + i;
+  const double u1 = uc[2 * b], u2 = uc[2 * b + 1], W = E[b];
+  const double h = u1 * hx[k] + u2 * hp[k];
+  hx[k] = h;
+  g[k] = -2.0 * (h - W * (u1 * x[k] + u2 * p[k]));
+}
+
+// divergence guard (cg_module.f90:358-364): reject when rb/res > 1e8
+__global__ void k_cg_guard(const double *__restrict__ rb, const double *__restrict__ res,
+                           double *__restrict__ E, const double *__restrict__ E1,
+                           int *__restrict__ accept, int nb) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int ok = !(rb[b] / res[b] > 1.0e8);
+  accept[b] = ok;
+  if (!ok) E[b] = E1[b];
+}
+
+// x <- u1 x + u2 p for accepted bands (cg_module.f90:366)
+__global__ void k_cg_accept(double *__restrict__ x, const double *__restrict__ p,
+                            const double *__restrict__ uc, const int *__restrict__ accept,
+                            int64_t N) {
+  const int b = blockIdx.y;
+  if (!accept[b]) return;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t k = int64_t(b) * N + i;
+  x[k] = uc[2 * b] * x[k] + uc[2 * b + 1] * p[k];
+}
+
+__global__ void k_scale_copy(const double *__restrict__ a, double s, double *__restrict__ b, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = s * a[t];
+}
+
+__global__ void k_fill(double *__restrict__ a, double v, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+// ------------------------------------------------ hprsdft preconditioner
+// f = A g with A the 7-point -1/2 Laplacian (E0 = 0): precond_cg_mat2_omp,
+// cgpc_hprsdft_module.f90:288-361, periodic 1-deep halo (bcset_1).
+__global__ __launch_bounds__(kThreads) void k_pc_mat(const double *__restrict__ g,
+                                                     double *__restrict__ f, int n1, int n2,
+                                                     int n3, double c0, double c1, double c2,
+                                                     double c3) {
+  const int64_t n12 = int64_t(n1) * n2, N = n12 * n3;
+  const int b = blockIdx.y;
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const int i1 = int(j % n1), i2 = int((j / n1) % n2), i3 = int(j / n12);
+  const double *w = g + int64_t(b) * N;
+  const int xp = i1 + 1 == n1 ? 0 : i1 + 1, xm = i1 == 0 ? n1 - 1 : i1 - 1;
+  const int yp = i2 + 1 == n2 ? 0 : i2 + 1, ym = i2 == 0 ? n2 - 1 : i2 - 1;
+  const int zp = i3 + 1 == n3 ? 0 : i3 + 1, zm = i3 == 0 ? n3 - 1 : i3 - 1;
+  const int64_t row = int64_t(i2) * n1 + i3 * n12;
+  f[int64_t(b) * N + j] = c0 * w[j] + c1 * (w[xp + row] + w[xm + row]) +
+                          c2 * (w[i1 + int64_t(yp) * n1 + i3 * n12] + w[i1 + int64_t(ym) * n1 + i3 * n12]) +
+                          c3 * (w[i1 + int64_t(i2) * n1 + zp * n12] + w[i1 + int64_t(i2) * n1 + zm * n12]);
+}
+
+// r = g - f ; p = r
+__global__ void k_pc_init(const double *__restrict__ g, const double *__restrict__ f,
+                          double *__restrict__ r, double *__restrict__ p, int64_t total) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  const double v = g[k] - f[k];
+  r[k] = v;
+  p[k] = v;
+}
+
+// a = rr1/pAp, rr0 = rr1 (per band)
+__global__ void k_pc_alpha(const double *__restrict__ rr1, const double *__restrict__ pap,
+                           double *__restrict__ rr0, double *__restrict__ a, int nb) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  rr0[b] = rr1[b];
+  a[b] = rr1[b] / pap[b];
+}
+
+// x += a p ; optionally r -= a f
+__global__ void k_pc_step(double *__restrict__ x, const double *__restrict__ p,
+                          double *__restrict__ r, const double *__restrict__ f,
+                          const double *__restrict__ a, int update_r, int64_t N) {
+  const int b = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t k = int64_t(b) * N + i;
+  const double s = a[b];
+  x[k] += s * p[k];
+  if (update_r) r[k] -= s * f[k];
+}
+
+// p = r + (rr1/rr0) p
+__global__ void k_pc_dir(const double *__restrict__ r, double *__restrict__ p,
+                         const double *__restrict__ rr1, const double *__restrict__ rr0,
+                         int64_t N) {
+  const int b = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t k = int64_t(b) * N + i;
+  p[k] = r[k] + (rr1[b] / rr0[b]) * p[k];
+}
+
+inline dim3 grid_nb(int64_t N, int nb) { return dim3(unsigned((N + kThreads - 1) / kThreads), nb); }
+
+struct Scratch {
+  // per-band device scalars, each nb doubles
+  double *E, *E1, *rb, *gkgk, *res, *d6, *uc, *rr0, *rr1, *a, *pap;
+  int *accept;
+};
+
+int ensure_scalars(rsg_ctx *ctx, int nb, int nseg, Scratch &s) {
+  const size_t per = size_t(nb);
+  (void)nseg;
+  const size_t nd = per * (9 + 6 + 2);
+  if (ctx->scal.ensure(nd * sizeof(double) + per * sizeof(int) + 256) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "solver: scalar scratch allocation failed");
+  double *p = ctx->scal.as<double>();
+  s.E = p; p += per;
+  s.E1 = p; p += per;
+  s.rb = p; p += per;
+  s.gkgk = p; p += per;
+  s.res = p; p += per;
+  s.rr0 = p; p += per;
+  s.rr1 = p; p += per;
+  s.a = p; p += per;
+  s.pap = p; p += per;
+  s.d6 = p; p += 6 * per;
+  s.uc = p; p += 2 * per;
+  s.accept = reinterpret_cast<int *>(p);
+  return RSG_OK;
+}
+
+int seg_count(int64_t N, int nb) {
+  int nseg = 1;
+  while (nseg < 64 && int64_t(nseg) * nb < 1024 && N / (nseg * 2) >= 4096) nseg *= 2;
+  return nseg;
+}
+
+}  // namespace
+
+int multidot(rsg_ctx *ctx, int npair, const double *const *x, const double *const *y, int nb,
+             double alpha, double *out) {
+  if (npair < 1 || npair > 6) return ctx->fail(RSG_ERR_ARG, "multidot: 1..6 pairs");
+  if (nb == 0) return RSG_OK;
+  const int64_t N = ctx->N;
+  const int nseg = seg_count(N, nb);
+  const int64_t seglen = (N + nseg - 1) / nseg;
+  if (ctx->dotpart.ensure(size_t(npair) * nb * nseg * sizeof(double)) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "multidot: scratch allocation failed");
+  double *partial = ctx->dotpart.as<double>();
+  DotArgs a{};
+  for (int q = 0; q < npair; ++q) {
+    a.x[q] = x[q];
+    a.y[q] = y[q];
+  }
+  dim3 grid(nseg, nb);
+  switch (npair) {
+#define RSG_DOT_CASE(P)                                                                           \
+  case P:                                                                                         \
+    hipLaunchKernelGGL(k_multidot<P>, grid, dim3(kThreads), 0, ctx->stream, a, N, nb, nseg, seglen, \
+                       partial);                                                                  \
+    break;
+    RSG_DOT_CASE(1)
+    RSG_DOT_CASE(2)
+    RSG_DOT_CASE(3)
+    RSG_DOT_CASE(4)
+    RSG_DOT_CASE(5)
+    RSG_DOT_CASE(6)
+#undef RSG_DOT_CASE
+  }
+  const int count = npair * nb;
+  hipLaunchKernelGGL(k_reduce_partials, dim3((count + 255) / 256), dim3(256), 0, ctx->stream, partial,
+                     nseg, count, alpha, out);
+  return RSG_OK;
+}
+
+namespace {
+
+int check_store(rsg_ctx *ctx, const char *who) {
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, std::string(who) + ": grid not set");
+  if (ctx->mb <= 0) return ctx->fail(RSG_ERR_STATE, std::string(who) + ": no band store");
+  return RSG_OK;
+}
+
+int read_info(rsg_ctx *ctx, int *host) {
+  RSG_HIP(hipMemcpyAsync(host, ctx->info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+// hprsdft preconditioner on an nn-band block (cgpc_hprsdft_module.f90:133-270):
+// mloop CG steps on A x = g from x0 = pg (= g on entry).
+int precond_hprsdft(rsg_ctx *ctx, int mloop, const double *g, double *pg, int nn, double *r,
+                    double *p, double *f, Scratch &s) {
+  if (mloop <= 0) return RSG_OK;
+  const int64_t N = ctx->N;
+  const double c0 = ctx->ggg[0] / (ctx->hgrid[0] * ctx->hgrid[0]) +
+                    ctx->ggg[1] / (ctx->hgrid[1] * ctx->hgrid[1]) +
+                    ctx->ggg[2] / (ctx->hgrid[2] * ctx->hgrid[2]);
+  const double c1 = -0.5 / (ctx->hgrid[0] * ctx->hgrid[0]) * ctx->ggg[0];
+  const double c2 = -0.5 / (ctx->hgrid[1] * ctx->hgrid[1]) * ctx->ggg[1];
+  const double c3 = -0.5 / (ctx->hgrid[2] * ctx->hgrid[2]) * ctx->ggg[2];
+  const dim3 gN = grid_nb(N, nn), blk(kThreads);
+  const int64_t tot = N * nn;
+  hipLaunchKernelGGL(k_pc_mat, gN, blk, 0, ctx->stream, g, f, ctx->n[0], ctx->n[1], ctx->n[2], c0, c1, c2, c3);
+  hipLaunchKernelGGL(k_pc_init, dim3(unsigned((tot + kThreads - 1) / kThreads)), blk, 0, ctx->stream, g, f, r, p, tot);
+  {
+    const double *xs[1] = {r}, *ys[1] = {r};
+    int rc = multidot(ctx, 1, xs, ys, nn, 1.0, s.rr1);
+    if (rc) return rc;
+  }
+  std::vector<double> h(static_cast<size_t>(nn));
+  RSG_HIP(hipMemcpyAsync(h.data(), s.rr1, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  bool small = true;
+  for (double v : h) small = small && (v < 1e-24);
+  if (small) return RSG_OK;
+  const dim3 gb((nn + 63) / 64), b64(64);
+  for (int it = 1; it <= mloop; ++it) {
+    hipLaunchKernelGGL(k_pc_mat, gN, blk, 0, ctx->stream, p, f, ctx->n[0], ctx->n[1], ctx->n[2], c0, c1, c2, c3);
+    const double *xs[1] = {p}, *ys[1] = {f};
+    int rc = multidot(ctx, 1, xs, ys, nn, 1.0, s.pap);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pc_alpha, gb, b64, 0, ctx->stream, s.rr1, s.pap, s.rr0, s.a, nn);
+    const int last = it == mloop;
+    hipLaunchKernelGGL(k_pc_step, gN, blk, 0, ctx->stream, pg, p, r, f, s.a, last ? 0 : 1, N);
+    if (last) break;
+    const double *xr[1] = {r}, *yr[1] = {r};
+    rc = multidot(ctx, 1, xr, yr, nn, 1.0, s.rr1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pc_dir, gN, blk, 0, ctx->stream, r, p, s.rr1, s.rr0, N);
+  }
+  RSG_HIP(hipGetLastError());
+  return RSG_OK;
+}
+
+}  // namespace
+
+int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
+  int rc = check_store(ctx, "gram_schmidt");
+  if (rc) return rc;
+  if (n0 < 1 || n1 > ctx->mb || n0 > n1) return ctx->fail(RSG_ERR_ARG, "gram_schmidt: band range outside 1..mb");
+  const int nb = n1 - n0 + 1;
+  const int64_t N = ctx->N;
+  double *U = ctx->unk.as<double>() + size_t(n0 - 1) * N;
+  if (ctx->hsub.ensure(size_t(nb) * nb * 8) != hipSuccess || ctx->info.ensure(64) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "gram_schmidt: allocation failed");
+  double *S = ctx->hsub.as<double>();
+  const double dv = ctx->dv, zero = 0.0, one = 1.0;
+  // S = dV U^T U (upper), gram_schmidt_lusl_module.f90:268 (calc_dsyrk3)
+  RSG_BLAS(rocblas_dsyrk(ctx->blas, rocblas_fill_upper, rocblas_operation_transpose, nb, int(N), &dv, U,
+                         int(N), &zero, S, nb));
+  // S = R^T R (:286, PDPOTRF)
+  RSG_BLAS(rocsolver_dpotrf(ctx->blas, rocblas_fill_upper, nb, S, nb, ctx->info.as<rocblas_int>()));
+  int info = 0;
+  rc = read_info(ctx, &info);
+  if (rc) return rc;
+  if (info != 0)
+    return ctx->fail(RSG_ERR_NOTPD, "gram_schmidt: overlap not positive definite (leading minor " +
+                                        std::to_string(info) + ")");
+  // U <- U R^-1 (:290 PDTRTRI + :313 calc_dtrmm3, done as one triangular solve)
+  RSG_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                         rocblas_diagonal_non_unit, int(N), nb, &one, S, nb, U, int(N)));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+int subspace_diag(rsg_ctx *ctx, double *esp) {
+  int rc = check_store(ctx, "subspace_diag");
+  if (rc) return rc;
+  const int mb = ctx->mb;
+  const int64_t N = ctx->N;
+  if (ctx->hsub.ensure(size_t(mb) * mb * 8) != hipSuccess || ctx->esp_dev.ensure(size_t(mb) * 16) != hipSuccess ||
+      ctx->info.ensure(64) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "subspace_diag: allocation failed");
+  double *U = ctx->unk.as<double>(), *HU = ctx->work.as<double>(), *Hs = ctx->hsub.as<double>();
+  double *D = ctx->esp_dev.as<double>(), *Ework = D + mb;
+  // psit = H U for all bands in one band-blocked call (subspace_diag_la_module.f90:79)
+  rc = launch_hpsi<double>(ctx, U, HU, N, mb);
+  if (rc) return rc;
+  // Hsub = 1/2 dV (U^T HU + HU^T U) (:86, DSYR2K 'U','T')
+  const double zz = 0.5 * ctx->dv, zero = 0.0, one = 1.0;
+  RSG_BLAS(rocblas_dsyr2k(ctx->blas, rocblas_fill_upper, rocblas_operation_transpose, mb, int(N), &zz, U,
+                          int(N), HU, int(N), &zero, Hs, mb));
+  // eigen-decomposition, ascending (:155, DSYEV 'V','U')
+  RSG_BLAS(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, mb, Hs, mb, D, Ework,
+                            ctx->info.as<rocblas_int>()));
+  int info = 0;
+  rc = read_info(ctx, &info);
+  if (rc) return rc;
+  if (info != 0) return ctx->fail(RSG_ERR_SOLVER, "subspace_diag: dsyevd did not converge");
+  // rotation U <- U V (:190) into the work buffer, which becomes the store
+  RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, int(N), mb, mb, &one, U,
+                         int(N), Hs, mb, &zero, HU, int(N)));
+  std::swap(ctx->unk, ctx->work);
+  RSG_HIP(hipMemcpyAsync(esp, D, mb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+int esp_calc(rsg_ctx *ctx, double *esp) {
+  int rc = check_store(ctx, "esp_calc");
+  if (rc) return rc;
+  const int mb = ctx->mb;
+  const int64_t N = ctx->N;
+  if (ctx->esp_dev.ensure(size_t(mb) * 16) != hipSuccess) return ctx->fail(RSG_ERR_HIP, "esp_calc: allocation failed");
+  double *U = ctx->unk.as<double>(), *HU = ctx->work.as<double>();
+  rc = launch_hpsi<double>(ctx, U, HU, N, mb);
+  if (rc) return rc;
+  const double *xs[1] = {U}, *ys[1] = {HU};
+  rc = multidot(ctx, 1, xs, ys, mb, ctx->dv, ctx->esp_dev.as<double>());
+  if (rc) return rc;
+  RSG_HIP(hipMemcpyAsync(esp, ctx->esp_dev.p, mb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+// d_conjugate_gradient_1 (cg_module.f90:112-431) with the band block MB_d
+// as large as the CG work arrays allow.
+int conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res) {
+  int rc = check_store(ctx, "conjugate_gradient");
+  if (rc) return rc;
+  if (ncg < 0 || (pc_kind != 0 && pc_kind != 4)) return ctx->fail(RSG_ERR_ARG, "conjugate_gradient: ncg < 0 or pc_kind not 0/4");
+  const int mb = ctx->mb;
+  const int64_t N = ctx->N;
+  const double dv = ctx->dv;
+  // 8 work vectors per band: hxk hpk gk Pgk pk + preconditioner r p f
+  const size_t budget = size_t(64) << 30;
+  int mbd = int(std::min<size_t>(size_t(mb), std::max<size_t>(1, budget / (size_t(N) * 8 * 8))));
+  if (ctx->cgbuf.ensure(size_t(N) * mbd * 8 * 8) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "conjugate_gradient: work allocation failed");
+  Scratch s;
+  rc = ensure_scalars(ctx, mbd, 1, s);
+  if (rc) return rc;
+  double *base = ctx->cgbuf.as<double>();
+  const size_t blk = size_t(N) * mbd;
+  double *hxk = base, *hpk = base + blk, *gk = base + 2 * blk, *pgk = base + 3 * blk, *pk = base + 4 * blk;
+  double *pr = base + 5 * blk, *pp = base + 6 * blk, *pf = base + 7 * blk;
+  std::vector<double> hE(static_cast<size_t>(mbd)), hE1(static_cast<size_t>(mbd));
+  for (int ns = 0; ns < mb; ns += mbd) {
+    const int nn = std::min(mbd, mb - ns);
+    double *x = ctx->unk.as<double>() + size_t(ns) * N;
+    const dim3 gN = grid_nb(N, nn), blkd(kThreads), gb((nn + 63) / 64), b64(64);
+    hipLaunchKernelGGL(k_fill, gb, b64, 0, ctx->stream, s.E1, 1.0e10, nn);
+    rc = launch_hpsi<double>(ctx, x, hxk, N, nn);
+    if (rc) return rc;
+    {
+      const double *xs[1] = {x}, *ys[1] = {hxk};
+      if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.E))) return rc;
+    }
+    hipLaunchKernelGGL(k_cg_grad, gN, blkd, 0, ctx->stream, hxk, x, s.E, gk, N);
+    {
+      const double *xs[1] = {gk}, *ys[1] = {gk};
+      if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
+    }
+    for (int icg = 1; icg <= ncg + 1; ++icg) {
+      RSG_HIP(hipMemcpyAsync(pgk, gk, size_t(N) * nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+      hipLaunchKernelGGL(k_scale_copy, gb, b64, 0, ctx->stream, s.rb, 0.25, s.res, nn);
+      RSG_HIP(hipMemcpyAsync(hE.data(), s.E, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
+      RSG_HIP(hipMemcpyAsync(hE1.data(), s.E1, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
+      RSG_HIP(hipStreamSynchronize(ctx->stream));
+      bool conv = true;
+      for (int b = 0; b < nn; ++b) conv = conv && (std::fabs(hE[b] - hE1[b]) < 1e-15);
+      if (conv || icg == ncg + 1) break;
+      if (pc_kind == 4 && (rc = precond_hprsdft(ctx, mloop, gk, pgk, nn, pr, pp, pf, s))) return rc;
+      {
+        const double *xs[1] = {pgk}, *ys[1] = {gk};
+        if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
+      }
+      hipLaunchKernelGGL(k_cg_dir, gN, blkd, 0, ctx->stream, pgk, pk, s.rb, s.gkgk, icg == 1 ? 1 : 0, N);
+      RSG_HIP(hipMemcpyAsync(s.gkgk, s.rb, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+      if ((rc = launch_hpsi<double>(ctx, pk, hpk, N, nn))) return rc;
+      {
+        const double *xs[6] = {x, pk, pk, x, pk, pk}, *ys[6] = {x, x, pk, hxk, hxk, hpk};
+        if ((rc = multidot(ctx, 6, xs, ys, nn, dv, s.d6))) return rc;
+      }
+      hipLaunchKernelGGL(k_cg_ritz, gb, b64, 0, ctx->stream, s.d6, nn, s.E, s.E1, s.uc);
+      hipLaunchKernelGGL(k_cg_update, gN, blkd, 0, ctx->stream, hxk, hpk, x, pk, gk, s.uc, s.E, N);
+      {
+        const double *xs[1] = {gk}, *ys[1] = {gk};
+        if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
+      }
+      hipLaunchKernelGGL(k_cg_guard, gb, b64, 0, ctx->stream, s.rb, s.res, s.E, s.E1, s.accept, nn);
+      hipLaunchKernelGGL(k_cg_accept, gN, blkd, 0, ctx->stream, x, pk, s.uc, s.accept, N);
+      RSG_HIP(hipGetLastError());
+    }
+    RSG_HIP(hipMemcpyAsync(esp + ns, s.E, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
+    RSG_HIP(hipMemcpyAsync(res + ns, s.res, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
+    RSG_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return RSG_OK;
+}
+
+void solver_release(rsg_ctx *ctx) { ctx->dotpart.release(); }
+
+}  // namespace rsg

@@ -1,0 +1,346 @@
+// tables.cpp -- host-side operator setup: FD weights, GTH projector tables
+// and the two device layouts of the nonlocal operator.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "context.hpp"
+
+namespace rsg {
+
+hipError_t DevBuf::ensure(size_t need) {
+  if (need <= bytes && p) return hipSuccess;
+  release();
+  if (need == 0) need = 256;
+  hipError_t e = hipMalloc(&p, need);
+  if (e != hipSuccess) {
+    p = nullptr;
+    bytes = 0;
+    return e;
+  }
+  bytes = need;
+  return hipSuccess;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+// Central-difference second-derivative weights of half-width md in closed
+// form: w(+-m) = 2 (-1)^(m+1) (md!)^2 / (m^2 (md-m)! (md+m)!),
+// w(0) = -2 sum 1/m^2.  Same weights as get_coef_lapla_fd
+// (fd_module.f90:18-54), which builds them from Lagrange products.
+void fd_lap_closed_form(int md, double *lap) {
+  auto lfact = [](int k) { return std::lgamma(double(k) + 1.0); };
+  double w0 = 0.0;
+  for (int m = 1; m <= md; ++m) {
+    double mag = std::exp(2.0 * lfact(md) - lfact(md - m) - lfact(md + m));
+    double w = 2.0 * ((m % 2) ? 1.0 : -1.0) * mag / double(m * m);
+    lap[md + m] = lap[md - m] = w;
+    w0 -= 2.0 / double(m * m);
+  }
+  lap[md] = w0;
+}
+
+namespace {
+
+const double kPi = 3.141592653589793238;
+
+int nint_away(double x) { return int(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5)); }
+
+double powi(double r, int e) {
+  double v = 1.0;
+  for (int i = 0; i < e; ++i) v *= r;
+  return v;
+}
+
+// Gamma(L + 2n - 1/2) as sqrt(pi) * prod_{i=1}^{L+2n-1} (i - 1/2)
+double gamma_half(int L, int n) {
+  double g = std::sqrt(std::acos(-1.0));
+  for (int i = 1; i <= L + 2 * n - 1; ++i) g *= (i - 0.5);
+  return g;
+}
+
+// Projector cutoff: first r = i*0.01 (i <= 3000) where the unscaled GTH
+// radial value is below 1e-10 (ps_nloc_gth_module.f90:137-180).
+double gth_cutoff(int L, int n, double rl) {
+  const double dr = 0.01, ep = 1e-10;
+  double c1 = std::sqrt(2.0) / (std::pow(rl, L + 2 * n - 0.5) * std::sqrt(gamma_half(L, n)));
+  double c2 = 0.5 / (rl * rl);
+  for (int i = 1; i <= 3000; ++i) {
+    double r = i * dr;
+    if (std::fabs(c1 * powi(r, L + 2 * n - 2) * std::exp(-r * r * c2)) < ep) return r;
+  }
+  return 0.0;
+}
+
+// Associated Legendre P_L^M(x) by upward recursion in L.
+double assoc_legendre(int L, int M, double x) {
+  if (std::fabs(std::fabs(x) - 1.0) < 1e-13) x = x >= 0 ? 1.0 : -1.0;
+  double pmm = 1.0;
+  if (M > 0) {
+    double s = std::sqrt((1.0 - x) * (1.0 + x)), f = 1.0;
+    for (int i = 1; i <= M; ++i) {
+      pmm = -pmm * f * s;
+      f += 2.0;
+    }
+  }
+  if (L == M) return pmm;
+  double p1 = x * (2.0 * M + 1.0) * pmm;
+  if (L == M + 1) return p1;
+  double pl = 0.0;
+  for (int l = M + 2; l <= L; ++l) {
+    pl = (x * (2.0 * l - 1.0) * p1 - (l + M - 1.0) * pmm) / (l - M);
+    pmm = p1;
+    p1 = pl;
+  }
+  return pl;
+}
+
+// Real spherical harmonic with RSDFT's sign/ordering convention
+// (ylm_module.f90:11-73).
+double real_ylm(double x, double y, double z, int L, int m) {
+  const int M = std::abs(m);
+  const double pi4 = 4.0 * kPi;
+  if (L == 0) return 1.0 / std::sqrt(pi4);
+  if (x == 0.0 && y == 0.0 && z == 0.0) return 0.0;
+  double c = 1.0;
+  for (int k = L - M + 1; k <= L + M; ++k) c *= double(k);
+  const double clm = std::sqrt((2.0 * L + 1.0) / (c * pi4));
+  const double r = std::sqrt(x * x + y * y + z * z), ct = z / r;
+  double phi;
+  if (std::fabs(x) < 1e-10) {
+    phi = 0.5 * kPi * (y >= 0 ? 1.0 : -1.0);
+  } else {
+    phi = std::atan(y / x);
+    if (x < 0) phi += kPi;
+  }
+  const double p = assoc_legendre(L, M, ct);
+  if (m > 0) return std::sqrt(2.0) * clm * p * std::cos(phi * M);
+  if (m == 0) return clm * p;
+  if (M % 2 == 0) return -std::sqrt(2.0) * clm * p * std::sin(phi * M);
+  return std::sqrt(2.0) * clm * p * std::sin(phi * M);
+}
+
+struct SpherePoint {
+  int i0[3];
+  int k[3];
+  double v;
+};
+
+}  // namespace
+
+int build_gth_tables(const rsg_ctx *ctx, int natom, const double *frac,
+                     const int *kind, int nkind, const int *norb,
+                     const int *lo, const int *no, const double *rps,
+                     const double *hnl, HostTables &out) {
+  const int *ng = ctx->n;
+  const double *aa = ctx->aa;
+  std::vector<double> rcut(size_t(nkind) * 4, 0.0);
+  for (int ik = 0; ik < nkind; ++ik) {
+    if (norb[ik] < 0 || norb[ik] > 4) return RSG_ERR_ARG;
+    for (int io = 0; io < norb[ik]; ++io) {
+      int q = ik * 4 + io;
+      if (lo[q] < 0 || lo[q] > 3 || no[q] < 1 || !(rps[q] > 0)) return RSG_ERR_ARG;
+      rcut[q] = gth_cutoff(lo[q], no[q], rps[q]);
+    }
+  }
+  for (int a = 0; a < natom; ++a)
+    if (kind[a] < 0 || kind[a] >= nkind) return RSG_ERR_ARG;
+  const double c[3] = {1.0 / ng[0], 1.0 / ng[1], 1.0 / ng[2]};
+  const int64_t n12 = int64_t(ng[0]) * ng[1];
+
+  out = HostTables();
+  std::vector<std::vector<int>> lma_pts;
+  std::vector<std::vector<double>> lma_val;
+  std::vector<int> marker(size_t(ctx->N), 0);
+
+  for (int a = 0; a < natom; ++a) {
+    const double *fa = frac + 3 * a;
+    const double R[3] = {aa[0] * fa[0] + aa[3] * fa[1] + aa[6] * fa[2],
+                         aa[1] * fa[0] + aa[4] * fa[1] + aa[7] * fa[2],
+                         aa[2] * fa[0] + aa[5] * fa[1] + aa[8] * fa[2]};
+    const int ic[3] = {nint_away(fa[0] * ng[0]), nint_away(fa[1] * ng[1]), nint_away(fa[2] * ng[2])};
+    const int ik = kind[a];
+    for (int io = 0; io < norb[ik]; ++io) {
+      const int q = ik * 4 + io;
+      const int L = lo[q], nn = no[q];
+      const double rl = rps[q], rc2 = rcut[q] * rcut[q];
+      const double c1 = std::sqrt(std::fabs(hnl[q])) * std::sqrt(2.0) /
+                        (std::pow(rl, L + 2 * nn - 0.5) * std::sqrt(gamma_half(L, nn)));
+      const double c2 = 0.5 / (rl * rl);
+      // candidate offsets around the nearest grid point, i3 outer, i1 inner
+      int mm[3];
+      for (int t = 0; t < 3; ++t) mm[t] = int(std::ceil(rcut[q] / ctx->hgrid[t])) + 2;
+      std::vector<SpherePoint> pts;
+      for (int o3 = -mm[2]; o3 <= mm[2]; ++o3)
+        for (int o2 = -mm[1]; o2 <= mm[1]; ++o2)
+          for (int o1 = -mm[0]; o1 <= mm[0]; ++o1) {
+            const int id[3] = {ic[0] + o1, ic[1] + o2, ic[2] + o3};
+            SpherePoint p;
+            for (int t = 0; t < 3; ++t) {
+              int kq = id[t] / ng[t];
+              if (id[t] < 0) kq = (id[t] + 1) / ng[t] - 1;
+              p.k[t] = kq;
+              p.i0[t] = id[t] - kq * ng[t];
+            }
+            const double d1 = id[0] * c[0], d2 = id[1] * c[1], d3 = id[2] * c[2];
+            const double x = aa[0] * d1 + aa[3] * d2 + aa[6] * d3 - R[0];
+            const double y = aa[1] * d1 + aa[4] * d2 + aa[7] * d3 - R[1];
+            const double z = aa[2] * d1 + aa[5] * d2 + aa[8] * d3 - R[2];
+            const double r2 = x * x + y * y + z * z;
+            if (r2 > rc2 + 1e-10) continue;
+            p.v = c1 * powi(std::sqrt(r2), L + 2 * nn - 2) * std::exp(-r2 * c2);
+            pts.push_back(p);
+          }
+      if (pts.empty()) continue;
+      for (int m = -L; m <= L; ++m) {
+        std::vector<int> idx;
+        std::vector<double> val;
+        for (const SpherePoint &p : pts) {
+          const double d1 = c[0] * p.i0[0] + p.k[0], d2 = c[1] * p.i0[1] + p.k[1],
+                       d3 = c[2] * p.i0[2] + p.k[2];
+          const double x = aa[0] * d1 + aa[3] * d2 + aa[6] * d3 - R[0];
+          const double y = aa[1] * d1 + aa[4] * d2 + aa[7] * d3 - R[1];
+          const double z = aa[2] * d1 + aa[5] * d2 + aa[8] * d3 - R[2];
+          const double u = p.v * real_ylm(x, y, z, L, m);
+          const int64_t lin = p.i0[0] + int64_t(p.i0[1]) * ng[0] + int64_t(p.i0[2]) * n12;
+          const int seen = marker[size_t(lin)];
+          if (seen == 0 && std::fabs(u) > 1e-14) {
+            idx.push_back(int(lin));
+            val.push_back(u);
+            marker[size_t(lin)] = int(idx.size());
+          } else if (seen != 0) {
+            val[size_t(seen - 1)] += u;  // periodic image of a point already held
+          }
+        }
+        for (int v : idx) marker[size_t(v)] = 0;
+        std::vector<int> order(idx.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::sort(order.begin(), order.end(), [&](int x, int y) { return idx[x] < idx[y]; });
+        std::vector<int> si(idx.size());
+        std::vector<double> sv(idx.size());
+        for (size_t t = 0; t < order.size(); ++t) {
+          si[t] = idx[order[t]];
+          sv[t] = val[order[t]];
+        }
+        lma_pts.push_back(std::move(si));
+        lma_val.push_back(std::move(sv));
+        out.iuv.push_back(hnl[q] < 0.0 ? -1 : 1);
+        out.amap.push_back(a);
+      }
+    }
+  }
+  out.nzlma = int(lma_pts.size());
+  out.maxmjj = 0;
+  for (auto &v : lma_pts) out.maxmjj = std::max(out.maxmjj, int(v.size()));
+  out.mjj.resize(out.nzlma);
+  out.jjp.assign(size_t(out.nzlma) * out.maxmjj, 0);
+  out.uvk.assign(size_t(out.nzlma) * out.maxmjj, 0.0);
+  for (int l = 0; l < out.nzlma; ++l) {
+    out.mjj[l] = int(lma_pts[l].size());
+    std::copy(lma_pts[l].begin(), lma_pts[l].end(), out.jjp.begin() + size_t(l) * out.maxmjj);
+    std::copy(lma_val[l].begin(), lma_val[l].end(), out.uvk.begin() + size_t(l) * out.maxmjj);
+  }
+  return RSG_OK;
+}
+
+namespace {
+
+template <typename T>
+int upload_layouts(rsg_ctx *ctx, NonlocalDev<T> &d, const std::vector<int> &goff,
+                   const std::vector<int> &glma0, const std::vector<int> &gnl,
+                   const std::vector<int> &gpts, const std::vector<double> &gcoef,
+                   const std::vector<double> &scale, const std::vector<int> &poff,
+                   const std::vector<int> &plma, const std::vector<double> &puv) {
+  auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
+    if (b.ensure(bytes) != hipSuccess) return RSG_ERR_HIP;
+    if (bytes && hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return RSG_ERR_HIP;
+    return RSG_OK;
+  };
+  std::vector<T> c(gcoef.begin(), gcoef.end()), s(scale.begin(), scale.end()),
+      u(puv.begin(), puv.end());
+  d.ngroup = int(glma0.size());
+  int rc = 0;
+  rc |= up(d.grp_off, goff.data(), goff.size() * sizeof(int));
+  rc |= up(d.grp_lma0, glma0.data(), glma0.size() * sizeof(int));
+  rc |= up(d.grp_nl, gnl.data(), gnl.size() * sizeof(int));
+  rc |= up(d.grp_pts, gpts.data(), gpts.size() * sizeof(int));
+  rc |= up(d.grp_coef, c.data(), c.size() * sizeof(T));
+  rc |= up(d.scale, s.data(), s.size() * sizeof(T));
+  rc |= up(d.pt_off, poff.data(), poff.size() * sizeof(int));
+  rc |= up(d.pt_lma, plma.data(), plma.size() * sizeof(int));
+  rc |= up(d.pt_uv, u.data(), u.size() * sizeof(T));
+  return rc ? ctx->fail(RSG_ERR_HIP, "upload of projector tables failed") : RSG_OK;
+}
+
+}  // namespace
+
+// Build the atom-grouped gather layout and the per-point scatter CSR from
+// the reference-layout tables in ctx->tables, then upload FP64 and FP32
+// copies.
+int upload_nonlocal(rsg_ctx *ctx) {
+  const HostTables &t = ctx->tables;
+  const int nz = t.nzlma, mx = t.maxmjj;
+  std::vector<int> goff{0}, glma0, gnl, gpts;
+  std::vector<double> gcoef;
+  std::vector<int> mark(size_t(ctx->N), -1);
+  for (int l0 = 0; l0 < nz;) {
+    int nl = 1;
+    if (!t.amap.empty())
+      while (l0 + nl < nz && nl < 4 && t.amap[l0 + nl] == t.amap[l0]) ++nl;
+    std::vector<int> pts;
+    for (int l = l0; l < l0 + nl; ++l)
+      for (int j = 0; j < t.mjj[l]; ++j) {
+        int p = t.jjp[size_t(l) * mx + j];
+        if (mark[size_t(p)] < 0) {
+          mark[size_t(p)] = 0;
+          pts.push_back(p);
+        }
+      }
+    std::sort(pts.begin(), pts.end());
+    for (size_t i = 0; i < pts.size(); ++i) mark[size_t(pts[i])] = int(i);
+    size_t base = gcoef.size();
+    gcoef.resize(base + pts.size() * 4, 0.0);
+    for (int l = l0; l < l0 + nl; ++l)
+      for (int j = 0; j < t.mjj[l]; ++j) {
+        int p = t.jjp[size_t(l) * mx + j];
+        gcoef[base + size_t(mark[size_t(p)]) * 4 + (l - l0)] = t.uvk[size_t(l) * mx + j];
+      }
+    for (int p : pts) mark[size_t(p)] = -1;
+    gpts.insert(gpts.end(), pts.begin(), pts.end());
+    goff.push_back(int(gpts.size()));
+    glma0.push_back(l0);
+    gnl.push_back(nl);
+    l0 += nl;
+  }
+  std::vector<double> scale(static_cast<size_t>(nz));
+  for (int l = 0; l < nz; ++l) scale[l] = t.iuv[l] * ctx->dv;
+
+  // per-point CSR, entries of one point in increasing lma order
+  std::vector<int> poff(size_t(ctx->N) + 1, 0);
+  for (int l = 0; l < nz; ++l)
+    for (int j = 0; j < t.mjj[l]; ++j) ++poff[size_t(t.jjp[size_t(l) * mx + j]) + 1];
+  for (int64_t i = 0; i < ctx->N; ++i) poff[size_t(i) + 1] += poff[size_t(i)];
+  const int64_t E = poff[size_t(ctx->N)];
+  std::vector<int> plma(static_cast<size_t>(E)), fill(poff.begin(), poff.end() - 1);
+  std::vector<double> puv(static_cast<size_t>(E));
+  for (int l = 0; l < nz; ++l)
+    for (int j = 0; j < t.mjj[l]; ++j) {
+      int p = t.jjp[size_t(l) * mx + j];
+      int e = fill[size_t(p)]++;
+      plma[size_t(e)] = l;
+      puv[size_t(e)] = t.uvk[size_t(l) * mx + j];
+    }
+  int rc = upload_layouts<double>(ctx, ctx->nl64, goff, glma0, gnl, gpts, gcoef, scale, poff, plma, puv);
+  if (rc) return rc;
+  rc = upload_layouts<float>(ctx, ctx->nl32, goff, glma0, gnl, gpts, gcoef, scale, poff, plma, puv);
+  if (rc) return rc;
+  ctx->have_nl = nz > 0;
+  return RSG_OK;
+}
+
+}  // namespace rsg

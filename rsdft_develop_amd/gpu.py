@@ -1,0 +1,265 @@
+"""Python mirror of RSDFT's inner-loop module interfaces over the gfx950 C ABI.
+
+The method names and argument meanings follow the reference's Fortran
+interfaces: ``hamiltonian(tpsi, htpsi, n, k, s)`` (hamiltonian_module.f90:
+118-144), ``gram_schmidt(n0, n1, k, s)`` (gram_schmidt_module.f90:35-79),
+``subspace_diag(k, s)`` (subspace_diag_module.f90:25-65),
+``conjugate_gradient(...)`` (cg_module.f90:53-109) and ``esp_calc``
+(esp_calc_module.f90:17-68).  The reference stops the program on fatal
+errors (stop_program -> MPI_Abort); here every nonzero status of the C ABI
+raises ``RSDFTGPUError`` carrying the library's message.  Only Gamma-point,
+single-spin calculations exist on this path: ``k`` and ``s`` must be 1.
+
+There is no CPU fallback: if ``lib/librsdft_gpu.so`` is missing or cannot be
+loaded, importing the context raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "librsdft_gpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "rsdft_gpu.h")
+
+RSG_ERR_NOTPD = 4
+
+
+class RSDFTGPUError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[rsg error {code}] {msg}")
+        self.code = code
+
+
+_lib = None
+_c_int, _c_i64, _c_dbl, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+_P = ctypes.POINTER
+
+
+def header_symbols() -> list:
+    """Entry points declared in include/rsdft_gpu.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(rsg_[a-z0-9_]+)\s*\(", txt)))
+
+
+def load_library():
+    """Load the HIP library; raises (never falls back) if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run rsdft_develop_amd/build.py (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "rsg_create": (_c_int, [_P(_vp), _c_int]),
+        "rsg_destroy": (_c_int, [_vp]),
+        "rsg_last_error": (ctypes.c_char_p, [_vp]),
+        "rsg_synchronize": (_c_int, [_vp]),
+        "rsg_stream": (_vp, [_vp]),
+        "rsg_set_grid": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_int]),
+        "rsg_get_grid": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "rsg_set_vloc": (_c_int, [_vp, _vp]),
+        "rsg_set_projectors": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+        "rsg_build_gth_projectors": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
+        "rsg_get_projector_sizes": (_c_int, [_vp, _P(_c_int), _P(_c_int), _P(_c_i64)]),
+        "rsg_get_projectors": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "rsg_hpsi_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
+        "rsg_hpsi_f32": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
+        "rsg_hpsi_f64_host": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
+        "rsg_alloc_bands": (_c_int, [_vp, _c_int]),
+        "rsg_upload_bands": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int]),
+        "rsg_download_bands": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int]),
+        "rsg_bands_device_ptr": (_c_int, [_vp, _P(_vp), _P(_c_i64)]),
+        "rsg_gram_schmidt": (_c_int, [_vp, _c_int, _c_int]),
+        "rsg_subspace_diag": (_c_int, [_vp, _vp]),
+        "rsg_conjugate_gradient": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp]),
+        "rsg_esp_calc": (_c_int, [_vp, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_vp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Context:
+    """One GPU's operator state and band store (an ``rsg_ctx``)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        h = _vp()
+        rc = self.L.rsg_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise RSDFTGPUError(rc, f"rsg_create(device={device}) failed")
+        self.h = h
+        self.N = 0
+        self.mb = 0
+        self.cell = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.rsg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != 0:
+            msg = self.L.rsg_last_error(self.h)
+            raise RSDFTGPUError(rc, msg.decode() if msg else "")
+
+    @staticmethod
+    def _gamma_only(k: int, s: int):
+        if k != 1 or s != 1:
+            raise ValueError("only the Gamma point (k=1) and one spin (s=1) exist on this path")
+
+    # ------------------------------------------------------------ setup
+    def set_grid(self, ngrid, aa, md: int = 6):
+        n1, n2, n3 = (int(v) for v in ngrid)
+        self._check(self.L.rsg_set_grid(self.h, n1, n2, n3, _ptr(_f64(np.asarray(aa).T)), int(md)))
+        self.N = n1 * n2 * n3
+        self.ngrid = (n1, n2, n3)
+
+    def grid_info(self):
+        dv = _c_dbl()
+        hg = np.zeros(3)
+        c0 = _c_dbl()
+        md = self.cell.md if self.cell else 6
+        cl = np.zeros(3 * 8)
+        self._check(self.L.rsg_get_grid(self.h, ctypes.byref(dv), _ptr(hg), ctypes.byref(c0), _ptr(cl)))
+        return dv.value, hg, c0.value, cl[: 3 * md].reshape(3, md)
+
+    @property
+    def dv(self) -> float:
+        return self.grid_info()[0]
+
+    def set_vloc(self, vloc):
+        v = _f64(vloc)
+        if v.size != self.N:
+            raise ValueError("Vloc must have N entries")
+        self._check(self.L.rsg_set_vloc(self.h, _ptr(v)))
+
+    def build_gth_projectors(self, cell):
+        norb, lo, no, rps, hnl = cell.species_arrays()
+        frac, kind = _f64(cell.frac), _i32(cell.kind)
+        self._check(self.L.rsg_build_gth_projectors(
+            self.h, cell.natom, _ptr(frac), _ptr(kind), len(cell.species),
+            _ptr(norb), _ptr(lo), _ptr(no), _ptr(rps), _ptr(hnl)))
+
+    def set_projectors(self, t, amap=None):
+        mjj, jjp, uvk, iuv = _i32(t["mjj"]), _i32(t["jjp"]), _f64(t["uvk"]), _i32(t["iuv"])
+        am = _i32(amap) if amap is not None else None
+        self._check(self.L.rsg_set_projectors(
+            self.h, int(t["nzlma"]), int(t["maxmjj"]), _ptr(mjj), _ptr(jjp), _ptr(uvk), _ptr(iuv),
+            _ptr(am) if am is not None else None))
+
+    def projector_tables(self):
+        nz, mx, ne = _c_int(), _c_int(), _c_i64()
+        self._check(self.L.rsg_get_projector_sizes(self.h, ctypes.byref(nz), ctypes.byref(mx), ctypes.byref(ne)))
+        nz, mx = nz.value, mx.value
+        mjj = np.zeros(max(nz, 1), np.int32)
+        iuv = np.zeros(max(nz, 1), np.int32)
+        jjp = np.zeros(max(nz * mx, 1), np.int32)
+        uvk = np.zeros(max(nz * mx, 1), np.float64)
+        self._check(self.L.rsg_get_projectors(self.h, _ptr(mjj), _ptr(jjp), _ptr(uvk), _ptr(iuv)))
+        return dict(nzlma=nz, maxmjj=mx, nentries=ne.value, mjj=mjj[:nz], iuv=iuv[:nz],
+                    jjp=jjp[: nz * mx].reshape(nz, mx), uvk=uvk[: nz * mx].reshape(nz, mx))
+
+    def setup(self, cell, vloc=None):
+        """Grid, Vloc and GTH projector tables for ``cell``."""
+        self.cell = cell
+        self.set_grid(cell.ngrid, cell.aa, cell.md)
+        if vloc is not None:
+            self.set_vloc(vloc)
+        self.build_gth_projectors(cell)
+        return self
+
+    # -------------------------------------------------------------- hpsi
+    def hamiltonian(self, tpsi, htpsi=None, n: int = 1, k: int = 1, s: int = 1):
+        """htpsi = H tpsi for a host block ``tpsi(ML0, nb)`` (column-major)."""
+        self._gamma_only(k, s)
+        t = np.asfortranarray(tpsi, dtype=np.float64)
+        if t.ndim == 1:
+            t = t.reshape(-1, 1, order="F")
+        if t.shape[0] != self.N:
+            raise ValueError("tpsi must have N rows")
+        out = htpsi if htpsi is not None else np.empty_like(t, order="F")
+        if not (out.flags.f_contiguous and out.dtype == np.float64 and out.shape == t.shape):
+            raise ValueError("htpsi must be a Fortran-ordered float64 array shaped like tpsi")
+        self._check(self.L.rsg_hpsi_f64_host(self.h, _ptr(t), _ptr(out), t.shape[0], t.shape[1]))
+        return out
+
+    def hpsi_device(self, d_psi: int, d_hpsi: int, ld: int, nb: int, dtype=np.float64):
+        fn = self.L.rsg_hpsi_f64 if np.dtype(dtype) == np.float64 else self.L.rsg_hpsi_f32
+        self._check(fn(self.h, _vp(d_psi), _vp(d_hpsi), int(ld), int(nb)))
+
+    def synchronize(self):
+        self._check(self.L.rsg_synchronize(self.h))
+
+    @property
+    def stream(self) -> int:
+        return int(self.L.rsg_stream(self.h) or 0)
+
+    # -------------------------------------------------------- band store
+    def alloc_bands(self, mb: int):
+        self._check(self.L.rsg_alloc_bands(self.h, int(mb)))
+        self.mb = int(mb)
+
+    def upload_bands(self, u, n0: int = 0):
+        u = np.asfortranarray(u, dtype=np.float64)
+        self._check(self.L.rsg_upload_bands(self.h, _ptr(u), u.shape[0], int(n0), u.shape[1]))
+
+    def download_bands(self, n0: int = 0, nb: int | None = None):
+        nb = self.mb - n0 if nb is None else nb
+        out = np.zeros((self.N, nb), order="F")
+        self._check(self.L.rsg_download_bands(self.h, _ptr(out), self.N, int(n0), int(nb)))
+        return out
+
+    def bands_device_ptr(self):
+        p, ld = _vp(), _c_i64()
+        self._check(self.L.rsg_bands_device_ptr(self.h, ctypes.byref(p), ctypes.byref(ld)))
+        return int(p.value), int(ld.value)
+
+    # ------------------------------------------------------ eigensolver
+    def gram_schmidt(self, n0: int = 1, n1: int | None = None, k: int = 1, s: int = 1):
+        self._gamma_only(k, s)
+        self._check(self.L.rsg_gram_schmidt(self.h, int(n0), int(self.mb if n1 is None else n1)))
+
+    def subspace_diag(self, k: int = 1, s: int = 1) -> np.ndarray:
+        self._gamma_only(k, s)
+        esp = np.zeros(self.mb)
+        self._check(self.L.rsg_subspace_diag(self.h, _ptr(esp)))
+        return esp
+
+    def conjugate_gradient(self, ncg: int = 3, pc_kind: int = 4, mloop: int = 3, k: int = 1, s: int = 1):
+        self._gamma_only(k, s)
+        esp = np.zeros(self.mb)
+        res = np.zeros(self.mb)
+        self._check(self.L.rsg_conjugate_gradient(self.h, int(ncg), int(pc_kind), int(mloop), _ptr(esp), _ptr(res)))
+        return esp, res
+
+    def esp_calc(self, k: int = 1, s: int = 1) -> np.ndarray:
+        self._gamma_only(k, s)
+        esp = np.zeros(self.mb)
+        self._check(self.L.rsg_esp_calc(self.h, _ptr(esp)))
+        return esp

@@ -1,0 +1,124 @@
+"""GPU parity of the projector tables and hpsi against the CPU oracle.
+
+Tolerances: FP64 hpsi agrees with the oracle to 1e-12 of max|H psi|
+(different summation order only); FP32 hpsi (FP32 storage, accumulation and
+tables) to 2e-5 of max|H psi|.  Integer tables (MJJ, JJP, iuV) are exact.
+"""
+import numpy as np
+import pytest
+
+from rsdft_develop_amd import systems
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu_mod():
+    from rsdft_develop_amd import gpu
+
+    return gpu
+
+
+def _pair(orc, gpu_mod, cell, vloc_seed=0, projectors=True):
+    v = systems.model_vloc(cell, seed=vloc_seed)
+    ham = orc.Ham(cell, vloc=v, projectors=projectors)
+    ctx = gpu_mod.Context()
+    ctx.cell = cell
+    ctx.set_grid(cell.ngrid, cell.aa, cell.md)
+    ctx.set_vloc(v)
+    if projectors:
+        ctx.build_gth_projectors(cell)
+    return ham, ctx
+
+
+def _close(got, want, rel):
+    scale = max(np.abs(want).max(), 1e-300)
+    err = np.abs(got - want).max() / scale
+    assert err <= rel, f"max rel error {err:.3e} > {rel:.1e}"
+
+
+@pytest.mark.parametrize("ng,disp", [(36, 0.0), (24, 0.1), (30, 0.05)])
+def test_gth_tables_match_oracle(orc, gpu_mod, ng, disp):
+    cell = systems.silicon_supercell(1, ngrid_per_cell=ng, displacement=disp, seed=ng)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    a, b = ham.tables(), ctx.projector_tables()
+    assert a["nzlma"] == b["nzlma"] and a["maxmjj"] == b["maxmjj"]
+    np.testing.assert_array_equal(a["mjj"], b["mjj"])
+    np.testing.assert_array_equal(a["iuv"], b["iuv"])
+    np.testing.assert_array_equal(a["jjp"], b["jjp"])
+    np.testing.assert_allclose(b["uvk"], a["uvk"], rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("nb", [1, 7, 20])
+def test_hpsi_f64_si8(orc, gpu_mod, nb):
+    cell = systems.silicon_supercell(1)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    psi = systems.random_bands(cell.npoints, nb, seed=100 + nb)
+    _close(ctx.hamiltonian(psi), ham.hpsi(psi), 1e-12)
+
+
+def test_hpsi_f64_si64_block(orc, gpu_mod):
+    cell = systems.silicon_supercell(2, displacement=0.05, seed=1)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    psi = systems.random_bands(cell.npoints, 12, seed=7)
+    _close(ctx.hamiltonian(psi), ham.hpsi(psi), 1e-12)
+
+
+@pytest.mark.parametrize("md", [1, 4, 6, 8])
+def test_hpsi_anisotropic_grid_and_md(orc, gpu_mod, md):
+    aa = np.diag([9.7, 11.3, 10.1])
+    base = systems.silicon_supercell(1, displacement=0.1, seed=3)
+    cell = systems.Cell(aa=aa, frac=base.frac, kind=base.kind, species=base.species,
+                        ngrid=(20, 27, 23), nband=4, md=md)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    psi = systems.random_bands(cell.npoints, 5, seed=md)
+    _close(ctx.hamiltonian(psi), ham.hpsi(psi), 1e-12)
+
+
+def test_hpsi_without_projectors(orc, gpu_mod):
+    cell = systems.silicon_supercell(1, ngrid_per_cell=24)
+    ham, ctx = _pair(orc, gpu_mod, cell, projectors=False)
+    psi = systems.random_bands(cell.npoints, 3, seed=9)
+    _close(ctx.hamiltonian(psi), ham.hpsi(psi), 1e-12)
+
+
+def test_hpsi_caller_tables_without_atom_map(orc, gpu_mod):
+    """Tables handed over in the reference layout, one projector per group."""
+    cell = systems.silicon_supercell(1, ngrid_per_cell=28, displacement=0.1, seed=5)
+    ham, ctx = _pair(orc, gpu_mod, cell, projectors=True)
+    ctx.set_projectors(ham.tables(), amap=None)
+    psi = systems.random_bands(cell.npoints, 9, seed=11)
+    _close(ctx.hamiltonian(psi), ham.hpsi(psi), 1e-12)
+
+
+def test_hpsi_device_f64_and_f32_with_padded_ld(orc, gpu_mod):
+    from rsdft_develop_amd import hip
+
+    cell = systems.silicon_supercell(1)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    nb, N = 10, cell.npoints
+    ld = N + 37
+    psi = systems.random_bands(N, nb, seed=21)
+    want = ham.hpsi(psi)
+    host = np.zeros((ld, nb), order="F")
+    host[:N] = psi
+    for dt, rel in ((np.float64, 1e-12), (np.float32, 2e-5)):
+        d_in = hip.DeviceArray.from_numpy(host, dtype=dt)
+        d_out = hip.DeviceArray((ld, nb), dtype=dt)
+        d_out.fill_bytes(0xFF)  # NaN pattern in both precisions
+        ctx.hpsi_device(d_in.ptr, d_out.ptr, ld, nb, dtype=dt)
+        ctx.synchronize()
+        got = d_out.to_numpy().astype(np.float64)
+        _close(got[:N], want, rel)
+        assert np.all(np.isnan(got[N:]))  # padding rows untouched
+
+
+def test_hpsi_errors(gpu_mod):
+    ctx = gpu_mod.Context()
+    a = np.zeros((8, 1), order="F")
+    with pytest.raises(gpu_mod.RSDFTGPUError):  # no grid yet -> state error
+        ctx._check(ctx.L.rsg_hpsi_f64_host(ctx.h, a.ctypes.data, a.ctypes.data, 8, 1))
+    with pytest.raises(gpu_mod.RSDFTGPUError):
+        ctx.set_grid((8, 8, 8), np.array([[10, 1, 0], [0, 10, 0], [0, 0, 10.0]]), 6)
+    with pytest.raises(ValueError):
+        ctx.hamiltonian(np.zeros((8, 1)), k=2)

@@ -167,6 +167,7 @@ def test_tables_sorted_counts_and_size(orc):
     ham = orc.Ham(cell, vloc=np.zeros(cell.npoints))
     t = ham.tables()
     assert t["nzlma"] == 4 * cell.natom
+    assert t["maxmjj"] == t["mjj"].max()  # leading dim = maxval(MJJ) after the drop
     E = int(t["mjj"].sum())
     assert 4.5 < E / cell.npoints < 4.8  # ~27k entries per Si atom at h = 0.285
     for lma in range(t["nzlma"]):
