@@ -98,6 +98,21 @@ int rsg_hpsi_f32(rsg_ctx *ctx, const float *d_psi, float *d_hpsi, int64_t ld,
 int rsg_hpsi_f64_host(rsg_ctx *ctx, const double *psi, double *hpsi,
                       int64_t ld, int nb);
 
+/* Launch shape of the hpsi kernels for the current grid, per precision
+ * (64 or 32): rows per thread ry (1 or 2), tile rows ty, z-planes per
+ * workgroup zlen, bands per Infinity-Cache chunk, scatter band tile bt
+ * (4/8/16).  Zero fields are derived from the grid.  No reference
+ * counterpart (launch tuning of this implementation). */
+int rsg_set_hpsi_shape(rsg_ctx *ctx, int precision, int ry, int ty, int zlen,
+                       int chunk, int bt);
+/* The shape an nb-band call would use. */
+int rsg_get_hpsi_shape(rsg_ctx *ctx, int precision, int nb, int *ry, int *ty,
+                       int *zlen, int *chunk, int *bt);
+/* Time candidate shapes for nb bands (reps calls each, on internal scratch
+ * bands) and keep the fastest for this context.  best_ms may be NULL. */
+int rsg_autotune_hpsi(rsg_ctx *ctx, int precision, int nb, int reps,
+                      float *best_ms);
+
 /* Device-resident wavefunction store, the counterpart of
  * wf_module::unk(ML_0:ML_1, MB, k, s) (wf_module.f90:131). */
 int rsg_alloc_bands(rsg_ctx *ctx, int mb);

@@ -23,6 +23,11 @@ int rsg_create(rsg_ctx **out, int device) {
     delete ctx;
     return RSG_ERR_HIP;
   }
+  if (ctx->dead.ensure(64 * sizeof(double)) != hipSuccess) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RSG_ERR_HIP;
+  }
   if (rocblas_create_handle(&ctx->blas) != rocblas_status_success ||
       rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success) {
     (void)hipStreamDestroy(ctx->stream);
@@ -39,15 +44,11 @@ int rsg_destroy(rsg_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   DevBuf *bufs[] = {&ctx->vloc64, &ctx->vloc32, &ctx->uvunk, &ctx->unk, &ctx->work,
                     &ctx->hsub, &ctx->esp_dev, &ctx->info, &ctx->cgbuf, &ctx->scal,
-                    &ctx->hostio};
+                    &ctx->hostio, &ctx->dead};
   for (DevBuf *b : bufs) b->release();
-  for (auto *nl : {&ctx->nl64.grp_off, &ctx->nl64.grp_lma0, &ctx->nl64.grp_nl,
-                   &ctx->nl64.grp_pts, &ctx->nl64.grp_coef, &ctx->nl64.scale,
-                   &ctx->nl64.pt_off, &ctx->nl64.pt_lma, &ctx->nl64.pt_uv,
-                   &ctx->nl32.grp_off, &ctx->nl32.grp_lma0, &ctx->nl32.grp_nl,
-                   &ctx->nl32.grp_pts, &ctx->nl32.grp_coef, &ctx->nl32.scale,
-                   &ctx->nl32.pt_off, &ctx->nl32.pt_lma, &ctx->nl32.pt_uv})
-    nl->release();
+  ctx->uvpart.release();
+  ctx->nl64.release_all();
+  ctx->nl32.release_all();
   rsg::solver_release(ctx);
   if (ctx->blas) rocblas_destroy_handle(ctx->blas);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -231,6 +232,39 @@ int rsg_hpsi_f64_host(rsg_ctx *ctx, const double *psi, double *hpsi, int64_t ld,
   RSG_HIP(hipMemcpy2DAsync(hpsi, ld * 8, dh, ctx->N * 8, ctx->N * 8, nb, hipMemcpyDeviceToHost, ctx->stream));
   RSG_HIP(hipStreamSynchronize(ctx->stream));
   return RSG_OK;
+}
+
+int rsg_set_hpsi_shape(rsg_ctx *ctx, int precision, int ry, int ty, int zlen, int chunk, int bt) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (precision != 64 && precision != 32) return ctx->fail(RSG_ERR_ARG, "set_hpsi_shape: precision must be 64 or 32");
+  if (ry < 0 || ry > 2 || ty < 0 || zlen < 0 || chunk < 0 || (bt != 0 && bt != 4 && bt != 8 && bt != 16))
+    return ctx->fail(RSG_ERR_ARG, "set_hpsi_shape: ry in 0..2, ty/zlen/chunk >= 0, bt in {0,4,8,16}");
+  rsg::HpsiShape &sh = precision == 64 ? ctx->shape64 : ctx->shape32;
+  sh.ry = ry ? ry : 2;
+  sh.ty = ty;
+  sh.zlen = zlen;
+  sh.chunk = chunk;
+  sh.bt = bt ? bt : 8;
+  return RSG_OK;
+}
+
+int rsg_get_hpsi_shape(rsg_ctx *ctx, int precision, int nb, int *ry, int *ty, int *zlen, int *chunk, int *bt) {
+  if (!ctx) return RSG_ERR_ARG;
+  rsg::HpsiShape sh;
+  int rc = rsg::resolved_shape(ctx, precision, nb, &sh);
+  if (rc) return rc;
+  if (ry) *ry = sh.ry;
+  if (ty) *ty = sh.ty;
+  if (zlen) *zlen = sh.zlen;
+  if (chunk) *chunk = sh.chunk;
+  if (bt) *bt = sh.bt;
+  return RSG_OK;
+}
+
+int rsg_autotune_hpsi(rsg_ctx *ctx, int precision, int nb, int reps, float *best_ms) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::autotune_hpsi(ctx, precision, nb, reps, best_ms);
 }
 
 int rsg_alloc_bands(rsg_ctx *ctx, int mb) {

@@ -23,19 +23,42 @@ struct DevBuf {
 
 // Nonlocal projector operator in the two layouts the kernels stream.
 //  gather: groups of <= 4 projectors that share one point list (all
-//          projectors of one atom), coef[pt*4 + l] (zero-padded);
-//  scatter: per-grid-point CSR of (lma, coefficient) entries.
+//          projectors of one atom), coef[pt*4 + l] (zero-padded); each
+//          group's points are split into segments of bounded length so a
+//          launch has enough workgroups;
+//  scatter: per-grid-point entries (lma, coefficient) in sliced ELL: slice c
+//          covers points 64c..64c+63, padded to its longest row, stored
+//          slot-major so a wavefront reads each slot coalesced.
 template <typename T> struct NonlocalDev {
-  int ngroup = 0;
-  DevBuf grp_off;   // int[ngroup+1] into the group point list
-  DevBuf grp_lma0;  // int[ngroup]   first lma of the group
-  DevBuf grp_nl;    // int[ngroup]   projectors in the group (<= 4)
-  DevBuf grp_pts;   // int[npts]     grid index per group point
-  DevBuf grp_coef;  // T[npts*4]
-  DevBuf scale;     // T[nzlma]      iuV * dV
-  DevBuf pt_off;    // int[N+1]
-  DevBuf pt_lma;    // int[E]
-  DevBuf pt_uv;     // T[E]
+  int ngroup = 0, nseg = 0;
+  DevBuf grp_lma0;     // int[ngroup]    first lma of the group
+  DevBuf grp_nl;       // int[ngroup]    projectors in the group (<= 4)
+  DevBuf grp_seg_off;  // int[ngroup+1]  segment range of each group
+  DevBuf seg_beg;      // int[nseg]      first point of the segment
+  DevBuf seg_end;      // int[nseg]      one past its last point
+  DevBuf seg_grp;      // int[nseg]      owning group
+  DevBuf grp_pts;      // int[npts]      grid index per group point
+  DevBuf grp_coef;     // T[npts*4]
+  DevBuf lma_grp;      // int[nzlma]     group of each projector
+  DevBuf scale;        // T[nzlma]       iuV * dV
+  DevBuf sl_off;       // int[nslice]    first slot of each 64-point slice
+  DevBuf sl_len;       // int[nslice]    slots per point in the slice
+  DevBuf sl_lma;       // int[slots]     projector index (0 in padding)
+  DevBuf sl_uv;        // T[slots]       coefficient (0 in padding)
+  void release_all() {
+    for (DevBuf *b : {&grp_lma0, &grp_nl, &grp_seg_off, &seg_beg, &seg_end, &seg_grp, &grp_pts,
+                      &grp_coef, &lma_grp, &scale, &sl_off, &sl_len, &sl_lma, &sl_uv})
+      b->release();
+  }
+};
+
+// Launch shape of the hpsi kernels for one grid and precision.
+struct HpsiShape {
+  int ry = 2;     // rows per thread (y register blocking)
+  int ty = 0;     // rows per workgroup tile
+  int zlen = 0;   // z-planes per workgroup
+  int chunk = 0;  // bands per chunk (gather -> scatter -> stencil)
+  int bt = 8;     // bands per scatter thread
 };
 
 struct HostTables {
@@ -74,7 +97,9 @@ struct rsg_ctx {
   rsg::NonlocalDev<double> nl64;
   rsg::NonlocalDev<float> nl32;
   bool have_nl = false;
-  rsg::DevBuf uvunk;  // scratch: nzlma x nb partial projections
+  rsg::DevBuf uvunk;    // nzlma x chunk projections uVunk, [lma][band]
+  rsg::DevBuf uvpart;   // per-segment partial projections
+  rsg::HpsiShape shape64, shape32;  // zero fields: chosen from the grid
 
   // wavefunction store and work arrays
   int mb = 0;
@@ -84,6 +109,7 @@ struct rsg_ctx {
   rsg::DevBuf scal;                // small per-band scalars
   rsg::DevBuf hostio;              // staging for the host hpsi entry
   rsg::DevBuf dotpart;             // multidot partial sums
+  rsg::DevBuf dead;                // store target of padding lanes (64 doubles)
 
   int fail(int code, const std::string &msg) {
     err = msg;
@@ -121,4 +147,6 @@ int upload_nonlocal(rsg_ctx *ctx);
 namespace rsg {
 template <typename T>
 int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb);
+int resolved_shape(rsg_ctx *ctx, int precision, int nb, HpsiShape *out);
+int autotune_hpsi(rsg_ctx *ctx, int precision, int nb, int reps, float *best_ms);
 }

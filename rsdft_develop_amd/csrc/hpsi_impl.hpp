@@ -1,0 +1,604 @@
+// hpsi.hip -- Hamiltonian application on gfx950:
+//   hpsi = (T + Vloc) psi + V_NL psi  over a block of bands.
+//
+// Bands are processed in chunks sized so one chunk of psi and H psi stays in
+// the 256 MB Infinity Cache between the launches that touch it:
+//  1. k_nl_gather: per (atom-group segment, band tile) partial projections
+//     sum_j uV(j,l) psi(j,b) for the <= 4 projectors of one atom, reading psi
+//     at each sphere point once (d_ps_nloc2_op_module.f90:182-191);
+//     wavefront-shuffle + LDS reduction inside the workgroup.
+//  2. k_nl_reduce: uVunk(lma,b) = iuV dV * sum over the group's segments.
+//  3. k_nl_scatter: H psi(j,b) = sum_lma uV(j,lma) uVunk(lma,b)
+//     (d_ps_nloc2_op_module.f90:274-281) from a per-point CSR, each thread
+//     one point and a tile of bands so the table is read once per tile; no
+//     atomics.
+//  4. k_stencil: H psi += (c + V) psi + 13-point-per-axis FD Laplacian
+//     (d_kinetic_sol_module.f90:244-292).  A workgroup owns ty full x-rows
+//     of a z-run; each plane is staged in double-buffered LDS with its
+//     +-MD halo rows (periodic in x inside the row, in y by row wrap: the
+//     bcset_3 halo of bc/bcset_3_module.f90:15-151), the z-neighbours stay
+//     in a per-thread register queue, and each thread computes ry rows so
+//     neighbouring rows share their y-window reads.
+#include <algorithm>
+
+#include "context.hpp"
+
+namespace rsg {
+namespace {
+
+constexpr int kMaxMd = 8;
+
+template <typename T> struct StencilCoef {
+  T c[3][kMaxMd];  // coef_lap(axis, m), m = 1..md
+  T c0;            // coef_lap0 (+ const_k2 = 0 at Gamma)
+};
+
+__host__ __device__ inline int wrapi(int i, int n) {
+  i %= n;
+  return i < 0 ? i + n : i;
+}
+
+// Software pipeline of the z-stream.  Plane zz of the thread's own columns
+// lives in register slot (zz - z0 + MD) % R of a ring of R = 2*MD + 1 + PF
+// planes (PF planes fetched ahead, no register moves).  The halo rows, Vloc
+// and the accumulated H psi of plane z+HS-1 are fetched into one of HS
+// rotating register sets while plane z computes.  R is a multiple of HS so
+// every slot index is a compile-time constant in the unrolled loop.
+//
+// The loop body is branch-free: every load goes to a valid (wrapped)
+// address whether or not its value is used, padding lanes store into a
+// dead buffer, and unused halo lanes write a dead LDS slot.  A conditional
+// load or store makes the wait-count pass drain all outstanding memory
+// operations at the join, which serialises the whole pipeline.  The LDS
+// exchange synchronises with s_waitcnt lgkmcnt(0) + s_barrier only;
+// __syncthreads() would also wait for every in-flight global load.
+constexpr int kHS = 4;
+__host__ __device__ constexpr int ring_pf(int md) { return (md % 2 == 0) ? 3 : 5; }
+
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <typename T, int MD, int RY, int HM, bool ACC>
+__global__ __launch_bounds__(512) void k_stencil(const T *__restrict__ psi, T *__restrict__ hpsi,
+                                                 int64_t ld, const T *__restrict__ vloc,
+                                                 StencilCoef<T> cf, int n1, int n2, int n3,
+                                                 int ty, int zlen, T *__restrict__ dead) {
+  constexpr int PF = ring_pf(MD);
+  constexpr int R = 2 * MD + 1 + PF;
+  static_assert(R % kHS == 0 && R % 2 == 0, "ring length must be a multiple of the rotations");
+  extern __shared__ unsigned char smem_raw[];
+  T *lds = reinterpret_cast<T *>(smem_raw);
+  const int rows = ty + 2 * MD;
+  const int plane = rows * n1;  // one LDS buffer; slot [2*plane] is the dead slot
+  const int G = ty / RY;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const bool active = tid < n1 * G;
+  const int x = active ? tid % n1 : 0;
+  const int g = active ? tid / n1 : 0;
+  const int y0 = blockIdx.x * ty;
+  const int z0 = blockIdx.y * zlen, z1 = min(z0 + zlen, n3);
+  const int64_t n12 = int64_t(n1) * n2;
+  const T *__restrict__ w = psi + int64_t(blockIdx.z) * ld;
+  T *__restrict__ o = hpsi + int64_t(blockIdx.z) * ld;
+
+  int64_t colofs[RY];
+  bool own[RY];
+  int lown[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int y = y0 + g * RY + r;
+    own[r] = active && y < n2;
+    colofs[r] = x + int64_t(y % n2) * n1;  // rows past n2 in the last tile hold wrapped data
+    lown[r] = active ? (MD + g * RY + r) * n1 + x : 2 * plane;
+  }
+  int xm[MD], xp[MD];
+#pragma unroll
+  for (int m = 0; m < MD; ++m) {
+    xm[m] = wrapi(x - m - 1, n1);
+    xp[m] = wrapi(x + m + 1, n1);
+  }
+  // halo elements this thread stages: LDS offset and in-plane offset
+  const int nhalo = 2 * MD * n1;
+  int hl[HM];
+  int64_t hg[HM];
+#pragma unroll
+  for (int k = 0; k < HM; ++k) {
+    const int idx = tid + k * nthr;
+    const int i = idx < nhalo ? idx : 0;
+    const int hr = i / n1, hx = i - hr * n1;
+    const int lrow = hr < MD ? hr : hr + ty;
+    hl[k] = idx < nhalo ? lrow * n1 + hx : 2 * plane;
+    hg[k] = hx + int64_t(wrapi(y0 - MD + lrow, n2)) * n1;
+  }
+  T q[RY][R];
+#pragma unroll
+  for (int r = 0; r < RY; ++r)
+#pragma unroll
+    for (int k = 0; k < R; ++k) q[r][k] = w[colofs[r] + int64_t(wrapi(z0 - MD + k, n3)) * n12];
+  // rotating sets: halo h[s][k], Vloc v[s][r], previous H psi a[s][r]
+  T h[kHS][HM], v[kHS][RY], a[kHS][RY];
+#pragma unroll
+  for (int sidx = 0; sidx < kHS - 1; ++sidx) {
+    const int64_t zo = int64_t(wrapi(z0 + sidx, n3)) * n12;
+#pragma unroll
+    for (int k = 0; k < HM; ++k) h[sidx][k] = w[hg[k] + zo];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      v[sidx][r] = vloc[colofs[r] + zo];
+      if (ACC) a[sidx][r] = o[colofs[r] + zo];
+    }
+  }
+  T *dst_dead = dead + (tid & 63);
+
+  for (int zb = z0; zb < z1; zb += R) {
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const int z = zb + u;
+      if (z >= z1) break;
+      T *buf = lds + (u & 1) * plane;
+      T *bufw = lds + (u & 1) * plane;
+      const int sc = u % kHS;              // set holding plane z
+      const int sn = (u + kHS - 1) % kHS;  // set to fill with plane z+kHS-1
+#pragma unroll
+      for (int r = 0; r < RY; ++r) (lown[r] == 2 * plane ? lds : bufw)[lown[r]] = q[r][(u + MD) % R];
+#pragma unroll
+      for (int k = 0; k < HM; ++k) (hl[k] == 2 * plane ? lds : bufw)[hl[k]] = h[sc][k];
+      {
+        const int64_t zo = int64_t(wrapi(z + kHS - 1, n3)) * n12;
+#pragma unroll
+        for (int k = 0; k < HM; ++k) h[sn][k] = w[hg[k] + zo];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+          v[sn][r] = vloc[colofs[r] + zo];
+          if (ACC) a[sn][r] = o[colofs[r] + zo];
+        }
+        const int64_t zf = int64_t(wrapi(z + MD + PF, n3)) * n12;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) q[r][(u + R - 1) % R] = w[colofs[r] + zf];
+      }
+      lds_barrier();
+      const int64_t zb2 = int64_t(z) * n12;
+      T col[RY + 2 * MD];
+#pragma unroll
+      for (int k = 0; k < RY + 2 * MD; ++k) col[k] = buf[(g * RY + k) * n1 + x];
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const T *row = buf + (MD + g * RY + r) * n1;
+        T acc = (cf.c0 + v[sc][r]) * q[r][(u + MD) % R];
+#pragma unroll
+        for (int m = 0; m < MD; ++m) acc += cf.c[0][m] * (row[xm[m]] + row[xp[m]]);
+#pragma unroll
+        for (int m = 0; m < MD; ++m) acc += cf.c[1][m] * (col[r + MD - 1 - m] + col[r + MD + 1 + m]);
+#pragma unroll
+        for (int m = 0; m < MD; ++m)
+          acc += cf.c[2][m] * (q[r][(u + MD - 1 - m) % R] + q[r][(u + MD + 1 + m) % R]);
+        if (ACC) acc += a[sc][r];
+        T *dst = own[r] ? o + colofs[r] + zb2 : dst_dead;
+        *dst = acc;
+      }
+    }
+  }
+}
+
+template <typename T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+constexpr int kGatherThreads = 256;
+
+template <typename T, int BB>
+__global__ __launch_bounds__(kGatherThreads) void k_nl_gather(
+    const T *__restrict__ psi, int64_t ld, int nbc, const int *__restrict__ seg_beg,
+    const int *__restrict__ seg_end, const int *__restrict__ grp_pts,
+    const T *__restrict__ grp_coef, T *__restrict__ part) {
+  __shared__ T red[kGatherThreads / 64][4 * BB];
+  const int s = blockIdx.x;
+  const int b0 = blockIdx.y * BB;
+  const int nbb = min(BB, nbc - b0);
+  const int beg = seg_beg[s], end = seg_end[s];
+  T acc[4][BB];
+#pragma unroll
+  for (int l = 0; l < 4; ++l)
+#pragma unroll
+    for (int b = 0; b < BB; ++b) acc[l][b] = T(0);
+  const T *__restrict__ w = psi + int64_t(b0) * ld;
+  for (int i = beg + threadIdx.x; i < end; i += kGatherThreads) {
+    const int pt = grp_pts[i];
+    const T c0 = grp_coef[4 * i + 0], c1 = grp_coef[4 * i + 1];
+    const T c2 = grp_coef[4 * i + 2], c3 = grp_coef[4 * i + 3];
+#pragma unroll
+    for (int b = 0; b < BB; ++b) {
+      if (b < nbb) {
+        const T v = w[pt + int64_t(b) * ld];
+        acc[0][b] += c0 * v;
+        acc[1][b] += c1 * v;
+        acc[2][b] += c2 * v;
+        acc[3][b] += c3 * v;
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int l = 0; l < 4; ++l)
+#pragma unroll
+    for (int b = 0; b < BB; ++b) {
+      const T v = wave_sum(acc[l][b]);
+      if (lane == 0) red[wid][l * BB + b] = v;
+    }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < 4 * BB) {
+    const int l = t / BB, b = t % BB;
+    if (b < nbb) {
+      T v = T(0);
+#pragma unroll
+      for (int k = 0; k < kGatherThreads / 64; ++k) v += red[k][t];
+      part[(int64_t(s) * 4 + l) * nbc + b0 + b] = v;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_nl_reduce(const T *__restrict__ part, int nbc, int nz, const int *__restrict__ lma_grp,
+                            const int *__restrict__ grp_lma0, const int *__restrict__ grp_seg_off,
+                            const T *__restrict__ scale, T *__restrict__ uv) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nz * nbc) return;
+  const int lma = t / nbc, b = t - lma * nbc;
+  const int g = lma_grp[lma], l = lma - grp_lma0[g];
+  T v = T(0);
+  for (int s = grp_seg_off[g]; s < grp_seg_off[g + 1]; ++s) v += part[(int64_t(s) * 4 + l) * nbc + b];
+  uv[int64_t(lma) * nbc + b] = scale[lma] * v;
+}
+
+// Sliced-ELL scatter: slice c holds the entries of points 64c..64c+63 as
+// slot-major arrays (slot k of lane i at off[c] + 64k + i), so the slot
+// loop is uniform across the wave and every table load is coalesced.
+template <typename T, int BT>
+__global__ __launch_bounds__(256) void k_nl_scatter(T *__restrict__ hpsi, int64_t ld, int nbc, int64_t N,
+                                                    const int *__restrict__ sl_off,
+                                                    const int *__restrict__ sl_len,
+                                                    const int *__restrict__ sl_lma,
+                                                    const T *__restrict__ sl_uv,
+                                                    const T *__restrict__ uv) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t c = j >> 6;
+  const int lane = threadIdx.x & 63;
+  if (c * 64 >= N) return;  // whole slice past the grid (wave-uniform)
+  const int b0 = blockIdx.y * BT;
+  const int nbt = min(BT, nbc - b0);
+  T acc[BT];
+#pragma unroll
+  for (int b = 0; b < BT; ++b) acc[b] = T(0);
+  const int base = sl_off[c], K = sl_len[c];
+  for (int k = 0; k < K; ++k) {
+    const int e = base + k * 64 + lane;
+    const T cv = sl_uv[e];
+    const T *__restrict__ u = uv + int64_t(sl_lma[e]) * nbc + b0;
+#pragma unroll
+    for (int b = 0; b < BT; ++b)
+      if (b < nbt) acc[b] += cv * u[b];
+  }
+  if (j < N) {
+#pragma unroll
+    for (int b = 0; b < BT; ++b)
+      if (b < nbt) hpsi[j + int64_t(b0 + b) * ld] = acc[b];
+  }
+}
+
+template <typename T> StencilCoef<T> make_coef(const rsg_ctx *ctx) {
+  StencilCoef<T> cf{};
+  for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < ctx->md; ++m) cf.c[a][m] = T(ctx->coef_lap[size_t(a) * ctx->md + m]);
+  cf.c0 = T(ctx->coef_lap0);
+  return cf;
+}
+
+template <typename T> NonlocalDev<T> &nl_of(rsg_ctx *ctx);
+template <> NonlocalDev<double> &nl_of<double>(rsg_ctx *ctx) { return ctx->nl64; }
+template <> NonlocalDev<float> &nl_of<float>(rsg_ctx *ctx) { return ctx->nl32; }
+template <typename T> const T *vloc_of(rsg_ctx *ctx);
+template <> const double *vloc_of<double>(rsg_ctx *ctx) { return ctx->vloc64.as<double>(); }
+template <> const float *vloc_of<float>(rsg_ctx *ctx) { return ctx->vloc32.as<float>(); }
+template <typename T> HpsiShape &shape_of(rsg_ctx *ctx);
+template <> HpsiShape &shape_of<double>(rsg_ctx *ctx) { return ctx->shape64; }
+template <> HpsiShape &shape_of<float>(rsg_ctx *ctx) { return ctx->shape32; }
+
+constexpr size_t kMaxDynLds = 64 * 1024;
+
+inline int round64(int v) { return (v + 63) / 64 * 64; }
+
+// Tile rows per workgroup for ry rows per thread: balance idle lanes, the
+// partial last tile and the halo rows each plane re-reads.
+int pick_ty(int n1, int n2, int md, int ry, size_t s) {
+  int best_ty = 0;
+  double best = 1e300;
+  for (int G = 1; n1 * G <= 512; ++G) {
+    const int ty = G * ry;
+    if (ty > ((n2 + ry - 1) / ry) * ry) break;
+    if ((2 * size_t(ty + 2 * md) * n1 + 1) * s > kMaxDynLds) break;
+    const int th = n1 * G;
+    if (2 * md * n1 > 4 * round64(th)) continue;
+    const int tiles = (n2 + ty - 1) / ty;
+    const double cost = double(round64(th)) / th * double(tiles * ty) / n2 *
+                        (1.0 + 0.5 * 2.0 * md / ty) * (th < 128 ? 2.0 : 1.0);
+    if (cost < best) {
+      best = cost;
+      best_ty = ty;
+    }
+  }
+  return best_ty;
+}
+
+template <typename T, int MD, int RY, int HM>
+int launch_stencil_t(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const HpsiShape &sh,
+                     int accumulate) {
+  const int n1 = ctx->n[0], n2 = ctx->n[1], n3 = ctx->n[2];
+  const int G = sh.ty / RY;
+  const int threads = round64(n1 * G);
+  const size_t lds = (2 * size_t(sh.ty + 2 * MD) * n1 + 1) * sizeof(T);
+  if (threads > 512 || lds > kMaxDynLds || 2 * MD * n1 > HM * threads)
+    return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: stencil tile does not fit");
+  dim3 grid((n2 + sh.ty - 1) / sh.ty, (n3 + sh.zlen - 1) / sh.zlen, nb);
+  if (accumulate)
+    hipLaunchKernelGGL((k_stencil<T, MD, RY, HM, true>), grid, dim3(threads), lds, ctx->stream, psi, hpsi, ld,
+                       vloc_of<T>(ctx), make_coef<T>(ctx), n1, n2, n3, sh.ty, sh.zlen, ctx->dead.template as<T>());
+  else
+    hipLaunchKernelGGL((k_stencil<T, MD, RY, HM, false>), grid, dim3(threads), lds, ctx->stream, psi, hpsi, ld,
+                       vloc_of<T>(ctx), make_coef<T>(ctx), n1, n2, n3, sh.ty, sh.zlen, ctx->dead.template as<T>());
+  return RSG_OK;
+}
+
+template <typename T, int MD, int RY>
+int launch_stencil_hm(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const HpsiShape &sh,
+                      int accumulate) {
+  const int threads = round64(ctx->n[0] * (sh.ty / RY));
+  const int need = (2 * MD * ctx->n[0] + threads - 1) / threads;
+  if (need <= 1) return launch_stencil_t<T, MD, RY, 1>(ctx, psi, hpsi, ld, nb, sh, accumulate);
+  if (need <= 2) return launch_stencil_t<T, MD, RY, 2>(ctx, psi, hpsi, ld, nb, sh, accumulate);
+  return launch_stencil_t<T, MD, RY, 4>(ctx, psi, hpsi, ld, nb, sh, accumulate);
+}
+
+template <typename T, int MD>
+int launch_stencil_md(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const HpsiShape &sh,
+                      int accumulate) {
+  switch (sh.ry) {
+    case 1: return launch_stencil_hm<T, MD, 1>(ctx, psi, hpsi, ld, nb, sh, accumulate);
+    case 2: return launch_stencil_hm<T, MD, 2>(ctx, psi, hpsi, ld, nb, sh, accumulate);
+    default: return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: ry must be 1 or 2");
+  }
+}
+
+template <typename T>
+int launch_stencil(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const HpsiShape &sh,
+                   int accumulate) {
+  switch (ctx->md) {
+#define RSG_MD_CASE(M) \
+  case M: return launch_stencil_md<T, M>(ctx, psi, hpsi, ld, nb, sh, accumulate);
+    RSG_MD_CASE(1)
+    RSG_MD_CASE(2)
+    RSG_MD_CASE(3)
+    RSG_MD_CASE(4)
+    RSG_MD_CASE(5)
+    RSG_MD_CASE(6)
+    RSG_MD_CASE(7)
+    RSG_MD_CASE(8)
+#undef RSG_MD_CASE
+    default: return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: md outside 1..8");
+  }
+}
+
+// Complete a launch shape: fields left at zero are derived from the grid,
+// the precision and the number of bands of the call.
+HpsiShape resolve_shape(const rsg_ctx *ctx, const HpsiShape &in, size_t s, int nb, bool chunked) {
+  HpsiShape sh = in;
+  const int n1 = ctx->n[0], n2 = ctx->n[1], n3 = ctx->n[2];
+  if (sh.ry < 1 || sh.ry > 2) sh.ry = 2;
+  if (sh.ty <= 0 || sh.ty % sh.ry || round64(n1 * (sh.ty / sh.ry)) > 512 ||
+      2 * ctx->md * n1 > 4 * round64(n1 * (sh.ty / sh.ry)) ||
+      (2 * size_t(sh.ty + 2 * ctx->md) * n1 + 1) * s > kMaxDynLds) {
+    sh.ty = pick_ty(n1, n2, ctx->md, sh.ry, s);
+    if (sh.ty == 0) {
+      sh.ry = 1;
+      sh.ty = pick_ty(n1, n2, ctx->md, 1, s);
+    }
+  }
+  if (sh.chunk <= 0) {
+    // psi + H psi of one chunk within ~128 MB of Infinity Cache
+    const size_t per_band = size_t(ctx->N) * s;
+    sh.chunk = chunked ? int(std::max<size_t>(1, (size_t(64) << 20) / per_band)) : nb;
+  }
+  sh.chunk = std::max(1, std::min(sh.chunk, nb));
+  if (sh.zlen <= 0) {
+    const int tiles = (n2 + sh.ty - 1) / sh.ty;
+    const int64_t want = 3 * 256;
+    int zr = int(std::max<int64_t>(1, (want + int64_t(tiles) * sh.chunk - 1) / (int64_t(tiles) * sh.chunk)));
+    zr = std::min(zr, std::max(1, n3 / 8));
+    sh.zlen = (n3 + zr - 1) / zr;
+  }
+  sh.zlen = std::max(1, std::min(sh.zlen, n3));
+  if (sh.bt != 4 && sh.bt != 8 && sh.bt != 16) sh.bt = 8;
+  return sh;
+}
+
+template <typename T, int BT>
+void launch_scatter_t(rsg_ctx *ctx, NonlocalDev<T> &nl, T *hpsi, int64_t ld, int nbc, const T *uv) {
+  dim3 grid(unsigned((ctx->N + 255) / 256), (nbc + BT - 1) / BT);
+  hipLaunchKernelGGL((k_nl_scatter<T, BT>), grid, dim3(256), 0, ctx->stream, hpsi, ld, nbc, ctx->N,
+                     nl.sl_off.template as<int>(), nl.sl_len.template as<int>(),
+                     nl.sl_lma.template as<int>(), nl.sl_uv.template as<T>(), uv);
+}
+
+}  // namespace
+
+template <typename T>
+int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb) {
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "hpsi: grid not set");
+  if (nb < 0 || ld < ctx->N) return ctx->fail(RSG_ERR_ARG, "hpsi: bad nb or ld");
+  if (nb == 0) return RSG_OK;
+  if (nb > 65535) return ctx->fail(RSG_ERR_ARG, "hpsi: nb above 65535");
+  if (!psi || !hpsi) return ctx->fail(RSG_ERR_ARG, "hpsi: null buffer");
+  const HpsiShape sh = resolve_shape(ctx, shape_of<T>(ctx), sizeof(T), nb, ctx->have_nl);
+  if (!ctx->have_nl) {
+    int rc = launch_stencil<T>(ctx, psi, hpsi, ld, nb, sh, 0);
+    if (rc) return rc;
+  } else {
+    NonlocalDev<T> &nl = nl_of<T>(ctx);
+    const int nz = ctx->tables.nzlma;
+    constexpr int BB = 8;
+    if (ctx->uvunk.ensure(size_t(nz) * sh.chunk * sizeof(T)) != hipSuccess ||
+        ctx->uvpart.ensure(size_t(nl.nseg) * 4 * sh.chunk * sizeof(T)) != hipSuccess)
+      return ctx->fail(RSG_ERR_HIP, "hpsi: projection scratch allocation failed");
+    T *uv = ctx->uvunk.as<T>(), *part = ctx->uvpart.as<T>();
+    for (int c0 = 0; c0 < nb; c0 += sh.chunk) {
+      const int nbc = std::min(sh.chunk, nb - c0);
+      const T *pc = psi + int64_t(c0) * ld;
+      T *hc = hpsi + int64_t(c0) * ld;
+      hipLaunchKernelGGL((k_nl_gather<T, BB>), dim3(nl.nseg, (nbc + BB - 1) / BB), dim3(kGatherThreads), 0,
+                         ctx->stream, pc, ld, nbc, nl.seg_beg.template as<int>(),
+                         nl.seg_end.template as<int>(), nl.grp_pts.template as<int>(),
+                         nl.grp_coef.template as<T>(), part);
+      const int nred = nz * nbc;
+      hipLaunchKernelGGL((k_nl_reduce<T>), dim3((nred + 255) / 256), dim3(256), 0, ctx->stream, part, nbc, nz,
+                         nl.lma_grp.template as<int>(), nl.grp_lma0.template as<int>(),
+                         nl.grp_seg_off.template as<int>(), nl.scale.template as<T>(), uv);
+      switch (sh.bt) {
+        case 4: launch_scatter_t<T, 4>(ctx, nl, hc, ld, nbc, uv); break;
+        case 16: launch_scatter_t<T, 16>(ctx, nl, hc, ld, nbc, uv); break;
+        default: launch_scatter_t<T, 8>(ctx, nl, hc, ld, nbc, uv); break;
+      }
+      int rc = launch_stencil<T>(ctx, pc, hc, ld, nbc, sh, 1);
+      if (rc) return rc;
+    }
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ctx->fail(RSG_ERR_HIP, std::string("hpsi launch: ") + hipGetErrorString(e));
+  return RSG_OK;
+}
+
+#if RSG_HPSI_TYPE_IS_DOUBLE
+int resolved_shape(rsg_ctx *ctx, int precision, int nb, HpsiShape *out) {
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "hpsi shape: grid not set");
+  if (precision != 64 && precision != 32) return ctx->fail(RSG_ERR_ARG, "hpsi shape: precision must be 64 or 32");
+  if (nb < 1) return ctx->fail(RSG_ERR_ARG, "hpsi shape: nb < 1");
+  const size_t s = precision == 64 ? 8 : 4;
+  *out = resolve_shape(ctx, precision == 64 ? ctx->shape64 : ctx->shape32, s, nb, ctx->have_nl);
+  return RSG_OK;
+}
+
+namespace {
+
+template <typename T>
+int time_shape(rsg_ctx *ctx, const T *psi, T *hpsi, int nb, const HpsiShape &sh, int reps, float *ms) {
+  HpsiShape saved = shape_of<T>(ctx);
+  shape_of<T>(ctx) = sh;
+  int rc = launch_hpsi<T>(ctx, psi, hpsi, ctx->N, nb);  // warm-up
+  hipEvent_t a = nullptr, b = nullptr;
+  if (!rc && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)) rc = RSG_ERR_HIP;
+  if (!rc && hipEventRecord(a, ctx->stream) != hipSuccess) rc = RSG_ERR_HIP;
+  for (int i = 0; i < reps && !rc; ++i) rc = launch_hpsi<T>(ctx, psi, hpsi, ctx->N, nb);
+  if (!rc && (hipEventRecord(b, ctx->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess)) rc = RSG_ERR_HIP;
+  if (!rc && hipEventElapsedTime(ms, a, b) != hipSuccess) rc = RSG_ERR_HIP;
+  if (!rc) *ms /= reps;
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  shape_of<T>(ctx) = saved;
+  return rc;
+}
+
+// Coordinate search over the launch shape on seeded scratch bands; keeps
+// the fastest shape in the context.
+template <typename T> int autotune_t(rsg_ctx *ctx, int nb, int reps, float *best_ms) {
+  const size_t bytes = size_t(ctx->N) * nb * sizeof(T);
+  T *psi = nullptr, *hpsi = nullptr;
+  if (hipMalloc(&psi, bytes) != hipSuccess) return ctx->fail(RSG_ERR_HIP, "autotune: allocation failed");
+  if (hipMalloc(&hpsi, bytes) != hipSuccess) {
+    (void)hipFree(psi);
+    return ctx->fail(RSG_ERR_HIP, "autotune: allocation failed");
+  }
+  // deterministic non-trivial content: timing does not depend on values
+  (void)hipMemsetAsync(psi, 0x3f, bytes, ctx->stream);
+  const size_t s = sizeof(T);
+  HpsiShape best = resolve_shape(ctx, HpsiShape(), s, nb, ctx->have_nl);
+  float tbest = 0.f;
+  int rc = time_shape<T>(ctx, psi, hpsi, nb, best, reps, &tbest);
+  auto consider = [&](HpsiShape cand) {
+    if (rc) return;
+    cand = resolve_shape(ctx, cand, s, nb, ctx->have_nl);
+    float t = 0.f;
+    int r = time_shape<T>(ctx, psi, hpsi, nb, cand, reps, &t);
+    if (r == RSG_ERR_UNSUPPORTED) return;  // shape does not fit this grid
+    if (r) {
+      rc = r;
+      return;
+    }
+    if (t < tbest) {
+      tbest = t;
+      best = cand;
+    }
+  };
+  for (int ry = 1; ry <= 2; ++ry) {
+    HpsiShape c = best;
+    c.ry = ry;
+    c.ty = 0;
+    c.zlen = 0;
+    consider(c);
+  }
+  for (int f : {-2, -1, 1, 2}) {
+    HpsiShape c = best;
+    c.ty = best.ty + f * best.ry;
+    c.zlen = 0;
+    if (c.ty > 0) consider(c);
+  }
+  if (ctx->have_nl) {
+    for (int f : {2, 4}) {
+      HpsiShape c = best;
+      c.chunk = std::min(nb, best.chunk * f);
+      c.zlen = 0;
+      consider(c);
+    }
+    HpsiShape ch = best;
+    ch.chunk = std::max(1, best.chunk / 2);
+    ch.zlen = 0;
+    consider(ch);
+    for (int bt : {4, 8, 16}) {
+      HpsiShape c = best;
+      c.bt = bt;
+      consider(c);
+    }
+  }
+  for (int zf : {2, 4}) {
+    HpsiShape c = best;
+    c.zlen = std::max(1, best.zlen / zf);
+    consider(c);
+  }
+  {
+    HpsiShape c = best;
+    c.zlen = ctx->n[2];
+    consider(c);
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(psi);
+  (void)hipFree(hpsi);
+  if (rc) return rc;
+  shape_of<T>(ctx) = best;
+  if (best_ms) *best_ms = tbest;
+  return RSG_OK;
+}
+
+}  // namespace
+
+int autotune_hpsi(rsg_ctx *ctx, int precision, int nb, int reps, float *best_ms) {
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "autotune: grid not set");
+  if (nb < 1 || reps < 1) return ctx->fail(RSG_ERR_ARG, "autotune: nb and reps must be >= 1");
+  if (precision == 64) return autotune_t<double>(ctx, nb, reps, best_ms);
+  if (precision == 32) return autotune_t<float>(ctx, nb, reps, best_ms);
+  return ctx->fail(RSG_ERR_ARG, "autotune: precision must be 64 or 32");
+}
+
+#endif  // RSG_HPSI_TYPE_IS_DOUBLE
+
+template int launch_hpsi<RSG_HPSI_TYPE>(rsg_ctx *, const RSG_HPSI_TYPE *, RSG_HPSI_TYPE *, int64_t, int);
+
+}  // namespace rsg

@@ -250,30 +250,40 @@ int build_gth_tables(const rsg_ctx *ctx, int natom, const double *frac,
 
 namespace {
 
+constexpr int kSegLen = 4096;  // points per gather segment
+
+struct HostLayouts {
+  std::vector<int> grp_lma0, grp_nl, grp_seg_off{0}, seg_beg, seg_end, seg_grp, grp_pts, lma_grp;
+  std::vector<int> sl_off, sl_len, sl_lma;
+  std::vector<double> grp_coef, scale, sl_uv;
+};
+
 template <typename T>
-int upload_layouts(rsg_ctx *ctx, NonlocalDev<T> &d, const std::vector<int> &goff,
-                   const std::vector<int> &glma0, const std::vector<int> &gnl,
-                   const std::vector<int> &gpts, const std::vector<double> &gcoef,
-                   const std::vector<double> &scale, const std::vector<int> &poff,
-                   const std::vector<int> &plma, const std::vector<double> &puv) {
+int upload_layouts(rsg_ctx *ctx, NonlocalDev<T> &d, const HostLayouts &h) {
   auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
     if (b.ensure(bytes) != hipSuccess) return RSG_ERR_HIP;
     if (bytes && hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return RSG_ERR_HIP;
     return RSG_OK;
   };
-  std::vector<T> c(gcoef.begin(), gcoef.end()), s(scale.begin(), scale.end()),
-      u(puv.begin(), puv.end());
-  d.ngroup = int(glma0.size());
+  std::vector<T> c(h.grp_coef.begin(), h.grp_coef.end()), s(h.scale.begin(), h.scale.end()),
+      u(h.sl_uv.begin(), h.sl_uv.end());
+  d.ngroup = int(h.grp_lma0.size());
+  d.nseg = int(h.seg_beg.size());
   int rc = 0;
-  rc |= up(d.grp_off, goff.data(), goff.size() * sizeof(int));
-  rc |= up(d.grp_lma0, glma0.data(), glma0.size() * sizeof(int));
-  rc |= up(d.grp_nl, gnl.data(), gnl.size() * sizeof(int));
-  rc |= up(d.grp_pts, gpts.data(), gpts.size() * sizeof(int));
+  rc |= up(d.grp_lma0, h.grp_lma0.data(), h.grp_lma0.size() * sizeof(int));
+  rc |= up(d.grp_nl, h.grp_nl.data(), h.grp_nl.size() * sizeof(int));
+  rc |= up(d.grp_seg_off, h.grp_seg_off.data(), h.grp_seg_off.size() * sizeof(int));
+  rc |= up(d.seg_beg, h.seg_beg.data(), h.seg_beg.size() * sizeof(int));
+  rc |= up(d.seg_end, h.seg_end.data(), h.seg_end.size() * sizeof(int));
+  rc |= up(d.seg_grp, h.seg_grp.data(), h.seg_grp.size() * sizeof(int));
+  rc |= up(d.grp_pts, h.grp_pts.data(), h.grp_pts.size() * sizeof(int));
   rc |= up(d.grp_coef, c.data(), c.size() * sizeof(T));
+  rc |= up(d.lma_grp, h.lma_grp.data(), h.lma_grp.size() * sizeof(int));
   rc |= up(d.scale, s.data(), s.size() * sizeof(T));
-  rc |= up(d.pt_off, poff.data(), poff.size() * sizeof(int));
-  rc |= up(d.pt_lma, plma.data(), plma.size() * sizeof(int));
-  rc |= up(d.pt_uv, u.data(), u.size() * sizeof(T));
+  rc |= up(d.sl_off, h.sl_off.data(), h.sl_off.size() * sizeof(int));
+  rc |= up(d.sl_len, h.sl_len.data(), h.sl_len.size() * sizeof(int));
+  rc |= up(d.sl_lma, h.sl_lma.data(), h.sl_lma.size() * sizeof(int));
+  rc |= up(d.sl_uv, u.data(), u.size() * sizeof(T));
   return rc ? ctx->fail(RSG_ERR_HIP, "upload of projector tables failed") : RSG_OK;
 }
 
@@ -285,8 +295,8 @@ int upload_layouts(rsg_ctx *ctx, NonlocalDev<T> &d, const std::vector<int> &goff
 int upload_nonlocal(rsg_ctx *ctx) {
   const HostTables &t = ctx->tables;
   const int nz = t.nzlma, mx = t.maxmjj;
-  std::vector<int> goff{0}, glma0, gnl, gpts;
-  std::vector<double> gcoef;
+  HostLayouts h;
+  h.lma_grp.assign(size_t(nz), 0);
   std::vector<int> mark(size_t(ctx->N), -1);
   for (int l0 = 0; l0 < nz;) {
     int nl = 1;
@@ -303,41 +313,74 @@ int upload_nonlocal(rsg_ctx *ctx) {
       }
     std::sort(pts.begin(), pts.end());
     for (size_t i = 0; i < pts.size(); ++i) mark[size_t(pts[i])] = int(i);
-    size_t base = gcoef.size();
-    gcoef.resize(base + pts.size() * 4, 0.0);
-    for (int l = l0; l < l0 + nl; ++l)
+    const size_t base = h.grp_coef.size();
+    const int pbase = int(h.grp_pts.size());
+    h.grp_coef.resize(base + pts.size() * 4, 0.0);
+    for (int l = l0; l < l0 + nl; ++l) {
+      h.lma_grp[size_t(l)] = int(h.grp_lma0.size());
       for (int j = 0; j < t.mjj[l]; ++j) {
         int p = t.jjp[size_t(l) * mx + j];
-        gcoef[base + size_t(mark[size_t(p)]) * 4 + (l - l0)] = t.uvk[size_t(l) * mx + j];
+        h.grp_coef[base + size_t(mark[size_t(p)]) * 4 + (l - l0)] = t.uvk[size_t(l) * mx + j];
       }
+    }
     for (int p : pts) mark[size_t(p)] = -1;
-    gpts.insert(gpts.end(), pts.begin(), pts.end());
-    goff.push_back(int(gpts.size()));
-    glma0.push_back(l0);
-    gnl.push_back(nl);
+    h.grp_pts.insert(h.grp_pts.end(), pts.begin(), pts.end());
+    const int g = int(h.grp_lma0.size());
+    for (int b = 0; b < int(pts.size()); b += kSegLen) {
+      h.seg_beg.push_back(pbase + b);
+      h.seg_end.push_back(pbase + std::min(int(pts.size()), b + kSegLen));
+      h.seg_grp.push_back(g);
+    }
+    h.grp_seg_off.push_back(int(h.seg_beg.size()));
+    h.grp_lma0.push_back(l0);
+    h.grp_nl.push_back(nl);
     l0 += nl;
   }
-  std::vector<double> scale(static_cast<size_t>(nz));
-  for (int l = 0; l < nz; ++l) scale[l] = t.iuv[l] * ctx->dv;
+  h.scale.resize(size_t(nz));
+  for (int l = 0; l < nz; ++l) h.scale[size_t(l)] = t.iuv[size_t(l)] * ctx->dv;
 
-  // per-point CSR, entries of one point in increasing lma order
-  std::vector<int> poff(size_t(ctx->N) + 1, 0);
+  // per-point entries in increasing lma order (CSR), then sliced ELL
+  std::vector<int> pt_off(size_t(ctx->N) + 1, 0);
   for (int l = 0; l < nz; ++l)
-    for (int j = 0; j < t.mjj[l]; ++j) ++poff[size_t(t.jjp[size_t(l) * mx + j]) + 1];
-  for (int64_t i = 0; i < ctx->N; ++i) poff[size_t(i) + 1] += poff[size_t(i)];
-  const int64_t E = poff[size_t(ctx->N)];
-  std::vector<int> plma(static_cast<size_t>(E)), fill(poff.begin(), poff.end() - 1);
-  std::vector<double> puv(static_cast<size_t>(E));
+    for (int j = 0; j < t.mjj[l]; ++j) ++pt_off[size_t(t.jjp[size_t(l) * mx + j]) + 1];
+  for (int64_t i = 0; i < ctx->N; ++i) pt_off[size_t(i) + 1] += pt_off[size_t(i)];
+  const int64_t E = pt_off[size_t(ctx->N)];
+  std::vector<int> pt_lma(static_cast<size_t>(E));
+  std::vector<double> pt_uv(static_cast<size_t>(E));
+  std::vector<int> fill(pt_off.begin(), pt_off.end() - 1);
   for (int l = 0; l < nz; ++l)
     for (int j = 0; j < t.mjj[l]; ++j) {
       int p = t.jjp[size_t(l) * mx + j];
       int e = fill[size_t(p)]++;
-      plma[size_t(e)] = l;
-      puv[size_t(e)] = t.uvk[size_t(l) * mx + j];
+      pt_lma[size_t(e)] = l;
+      pt_uv[size_t(e)] = t.uvk[size_t(l) * mx + j];
     }
-  int rc = upload_layouts<double>(ctx, ctx->nl64, goff, glma0, gnl, gpts, gcoef, scale, poff, plma, puv);
+  const int64_t nslice = (ctx->N + 63) / 64;
+  h.sl_off.resize(size_t(nslice));
+  h.sl_len.resize(size_t(nslice));
+  int64_t slots = 0;
+  for (int64_t c = 0; c < nslice; ++c) {
+    int K = 0;
+    for (int64_t i = c * 64; i < std::min(ctx->N, c * 64 + 64); ++i)
+      K = std::max(K, pt_off[size_t(i) + 1] - pt_off[size_t(i)]);
+    h.sl_off[size_t(c)] = int(slots);
+    h.sl_len[size_t(c)] = K;
+    slots += int64_t(K) * 64;
+  }
+  if (slots > (int64_t(1) << 31) - 1) return ctx->fail(RSG_ERR_UNSUPPORTED, "projector table above 2^31 slots");
+  h.sl_lma.assign(size_t(slots), 0);
+  h.sl_uv.assign(size_t(slots), 0.0);
+  for (int64_t i = 0; i < ctx->N; ++i) {
+    const int64_t c = i / 64, lane = i % 64;
+    for (int k = 0; k < pt_off[size_t(i) + 1] - pt_off[size_t(i)]; ++k) {
+      const size_t e = size_t(h.sl_off[size_t(c)] + int64_t(k) * 64 + lane);
+      h.sl_lma[e] = pt_lma[size_t(pt_off[size_t(i)] + k)];
+      h.sl_uv[e] = pt_uv[size_t(pt_off[size_t(i)] + k)];
+    }
+  }
+  int rc = upload_layouts<double>(ctx, ctx->nl64, h);
   if (rc) return rc;
-  rc = upload_layouts<float>(ctx, ctx->nl32, goff, glma0, gnl, gpts, gcoef, scale, poff, plma, puv);
+  rc = upload_layouts<float>(ctx, ctx->nl32, h);
   if (rc) return rc;
   ctx->have_nl = nz > 0;
   return RSG_OK;

@@ -69,6 +69,9 @@ def load_library():
         "rsg_hpsi_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
         "rsg_hpsi_f32": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
         "rsg_hpsi_f64_host": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
+        "rsg_set_hpsi_shape": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
+        "rsg_get_hpsi_shape": (_c_int, [_vp, _c_int, _c_int, _P(_c_int), _P(_c_int), _P(_c_int), _P(_c_int), _P(_c_int)]),
+        "rsg_autotune_hpsi": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(ctypes.c_float)]),
         "rsg_alloc_bands": (_c_int, [_vp, _c_int]),
         "rsg_upload_bands": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int]),
         "rsg_download_bands": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int]),
@@ -212,6 +215,19 @@ class Context:
     def hpsi_device(self, d_psi: int, d_hpsi: int, ld: int, nb: int, dtype=np.float64):
         fn = self.L.rsg_hpsi_f64 if np.dtype(dtype) == np.float64 else self.L.rsg_hpsi_f32
         self._check(fn(self.h, _vp(d_psi), _vp(d_hpsi), int(ld), int(nb)))
+
+    def set_hpsi_shape(self, precision=64, ry=0, ty=0, zlen=0, chunk=0, bt=0):
+        self._check(self.L.rsg_set_hpsi_shape(self.h, int(precision), int(ry), int(ty), int(zlen), int(chunk), int(bt)))
+
+    def hpsi_shape(self, precision=64, nb=1) -> dict:
+        v = [_c_int() for _ in range(5)]
+        self._check(self.L.rsg_get_hpsi_shape(self.h, int(precision), int(nb), *[ctypes.byref(x) for x in v]))
+        return dict(zip(("ry", "ty", "zlen", "chunk", "bt"), (x.value for x in v)))
+
+    def autotune_hpsi(self, precision=64, nb=1, reps=5) -> float:
+        ms = ctypes.c_float()
+        self._check(self.L.rsg_autotune_hpsi(self.h, int(precision), int(nb), int(reps), ctypes.byref(ms)))
+        return float(ms.value)
 
     def synchronize(self):
         self._check(self.L.rsg_synchronize(self.h))
