@@ -144,6 +144,54 @@ int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop,
 /* e_n = dV <psi_n|H psi_n>.  Replaces d_esp_calc (esp_calc_module.f90:17-68). */
 int rsg_esp_calc(rsg_ctx *ctx, double *esp);
 
+/* ---- density/potential half of the SCF step (FP64, device-resident) ----
+ * Fields the context holds on the grid (N doubles each). */
+#define RSG_FIELD_RHO 0     /* input density of the step (mixing state) */
+#define RSG_FIELD_RHO_OUT 1 /* density of the current bands */
+#define RSG_FIELD_VION 2    /* local pseudopotential */
+#define RSG_FIELD_VH 3      /* Hartree potential */
+#define RSG_FIELD_VXC 4     /* exchange-correlation potential */
+#define RSG_FIELD_VLOC 5    /* Vion + VH + Vxc, the potential hpsi applies */
+
+/* GTH local potential in G-space on the density grid's cutoff sphere
+ * (ps_local_gth_module.f90:13-58; construct_ps_local_fft0,
+ * ps_local_fft0_module.f90:14-86), the Ewald energy of the ions
+ * (ewald_module.f90:50-62) and the electron count.  Per species k:
+ * zps[k], rloc[k], cloc[4k..4k+3] = C1..C4. */
+int rsg_set_local_gth(rsg_ctx *ctx, int natom, const double *frac,
+                      const int *kind, int nkind, const double *zps,
+                      const double *rloc, const double *cloc);
+int rsg_get_scf_constants(rsg_ctx *ctx, double *eewald, double *nelectron);
+int rsg_get_field(rsg_ctx *ctx, int field, double *out);
+/* Only RSG_FIELD_RHO and RSG_FIELD_VLOC can be set. */
+int rsg_set_field(rsg_ctx *ctx, int field, const double *in);
+/* Initial density: atom-centred Gaussians of the given width (bohr)
+ * normalised to the electron count.  No reference counterpart: the
+ * reference starts GTH runs from a compiler-RNG random density
+ * (construct_RandomInitrho, ps_initrho_module.f90:58-134). */
+int rsg_init_density(rsg_ctx *ctx, double width);
+/* rho <- rho * Nel / integral(rho) (normalize_density). */
+int rsg_normalize_density(rsg_ctx *ctx);
+/* RHO_OUT = sum_n occ[n] psi_n^2 over the band store
+ * (calc_density, density_module.f90:120-151). */
+int rsg_calc_density(rsg_ctx *ctx, const double *occ);
+/* VH and Vxc (LDA-PZ81) of field RHO or RHO_OUT; returns the Hartree
+ * energy 1/2 int rho VH and Exc (hartree_sol_fftw_module.f90:29-130,
+ * xc/xc_ldapz81_module.f90:17-216). */
+int rsg_hartree_xc(rsg_ctx *ctx, int field, double *e_hartree, double *e_xc);
+/* Vloc = Vion + VH + Vxc (scf_module.f90:446-456), FP64 and FP32 copies. */
+int rsg_update_vloc(rsg_ctx *ctx);
+/* int rho Vloc and int rho Vion for field RHO or RHO_OUT
+ * (total_energy_module.f90:320-337). */
+int rsg_density_integrals(rsg_ctx *ctx, int field, double *e_loc,
+                          double *e_ion);
+/* Pulay density mixing (mixing_pulay_module.f90:29-121; simple mixing
+ * until mmix steps are held): mix_init stores RHO as the first input; each
+ * mix_pulay call takes RHO_OUT as the output of the latest input and writes
+ * the next input density into RHO. */
+int rsg_mix_init(rsg_ctx *ctx, int mmix, double beta);
+int rsg_mix_pulay(rsg_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

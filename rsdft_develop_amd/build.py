@@ -28,7 +28,7 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", INCLUDE, "-I", CSRC, "-Wall", "-Wno-unused-result",
 ]
-SOURCES = ["capi.cpp", "tables.cpp", "hpsi_f64.hip", "hpsi_f32.hip", "solver.hip"]
+SOURCES = ["capi.cpp", "tables.cpp", "hpsi_f64.hip", "hpsi_f32.hip", "solver.hip", "scf.hip"]
 
 
 def _compile(src: str) -> str:
@@ -48,7 +48,7 @@ def build_library(verbose: bool = False) -> str:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
-           "-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-Wl,-rpath,/opt/rocm/lib"]
+           "-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

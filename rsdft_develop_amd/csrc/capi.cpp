@@ -50,6 +50,7 @@ int rsg_destroy(rsg_ctx *ctx) {
   ctx->nl64.release_all();
   ctx->nl32.release_all();
   rsg::solver_release(ctx);
+  rsg::scf_release(ctx);
   if (ctx->blas) rocblas_destroy_handle(ctx->blas);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -111,6 +112,7 @@ int rsg_set_grid(rsg_ctx *ctx, int n1, int n2, int n3, const double *aa, int md)
   RSG_HIP(hipMemsetAsync(ctx->vloc64.p, 0, size_t(N) * 8, ctx->stream));
   RSG_HIP(hipMemsetAsync(ctx->vloc32.p, 0, size_t(N) * 4, ctx->stream));
   RSG_HIP(hipStreamSynchronize(ctx->stream));
+  rsg::scf_release(ctx);  // fields and FFT plan belong to the old grid
   ctx->have_grid = true;
   ctx->have_vloc = false;
   ctx->have_nl = false;
@@ -335,6 +337,80 @@ int rsg_esp_calc(rsg_ctx *ctx, double *esp) {
   if (!ctx || !esp) return RSG_ERR_ARG;
   RSG_HIP(hipSetDevice(ctx->device));
   return rsg::esp_calc(ctx, esp);
+}
+
+int rsg_set_local_gth(rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind,
+                      const double *zps, const double *rloc, const double *cloc) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_set_local_gth(ctx, natom, frac, kind, nkind, zps, rloc, cloc);
+}
+
+int rsg_get_scf_constants(rsg_ctx *ctx, double *eewald, double *nelectron) {
+  if (!ctx) return RSG_ERR_ARG;
+  return rsg::scf_constants(ctx, eewald, nelectron);
+}
+
+int rsg_get_field(rsg_ctx *ctx, int field, double *out) {
+  if (!ctx || !out) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_field(ctx, field, out, true);
+}
+
+int rsg_set_field(rsg_ctx *ctx, int field, const double *in) {
+  if (!ctx || !in) return RSG_ERR_ARG;
+  if (field != RSG_FIELD_RHO && field != RSG_FIELD_VLOC)
+    return ctx->fail(RSG_ERR_ARG, "set_field: only RHO and VLOC can be set");
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_field(ctx, field, const_cast<double *>(in), false);
+}
+
+int rsg_init_density(rsg_ctx *ctx, double width) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_init_density(ctx, width);
+}
+
+int rsg_normalize_density(rsg_ctx *ctx) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_normalize(ctx);
+}
+
+int rsg_calc_density(rsg_ctx *ctx, const double *occ) {
+  if (!ctx || !occ) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_calc_density(ctx, occ);
+}
+
+int rsg_hartree_xc(rsg_ctx *ctx, int field, double *e_hartree, double *e_xc) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_hartree_xc(ctx, field, e_hartree, e_xc);
+}
+
+int rsg_update_vloc(rsg_ctx *ctx) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_update_vloc(ctx);
+}
+
+int rsg_density_integrals(rsg_ctx *ctx, int field, double *e_loc, double *e_ion) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_density_integrals(ctx, field, e_loc, e_ion);
+}
+
+int rsg_mix_init(rsg_ctx *ctx, int mmix, double beta) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_mix_init(ctx, mmix, beta);
+}
+
+int rsg_mix_pulay(rsg_ctx *ctx) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_mix_pulay(ctx);
 }
 
 }  // extern "C"

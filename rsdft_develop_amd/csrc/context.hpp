@@ -12,6 +12,8 @@
 
 namespace rsg {
 
+struct ScfState;
+
 // Device buffer owned by the context (hipMalloc/hipFree).
 struct DevBuf {
   void *p = nullptr;
@@ -110,6 +112,7 @@ struct rsg_ctx {
   rsg::DevBuf hostio;              // staging for the host hpsi entry
   rsg::DevBuf dotpart;             // multidot partial sums
   rsg::DevBuf dead;                // store target of padding lanes (64 doubles)
+  rsg::ScfState *scf = nullptr;    // density/potential state of the SCF step
 
   int fail(int code, const std::string &msg) {
     err = msg;

@@ -1,0 +1,688 @@
+// scf.hip -- the density/potential half of an SCF step on the device:
+// GTH local potential (G-space form factor + hipFFT), density from the band
+// store, FFT Hartree, LDA-PZ81, Pulay mixing of the density, and the
+// energy integrals; Ewald energy on the host.  FP64 throughout.
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "solver.hpp"
+
+namespace rsg {
+
+struct ScfState {
+  bool have_local = false;
+  double eewald = 0.0, nelectron = 0.0;
+  DevBuf rho, rho_out, vion, vh, vxc, zbuf, work, occ;
+  // reciprocal-space cutoff of the density grid (ggrid_module.f90:55-91)
+  double ecut = 0.0;
+  double bb[9];
+  hipfftHandle plan = 0;
+  bool have_plan = false;
+  // Pulay history (mixing_pulay_module.f90): pointer rings of mmix slots
+  int mmix = 0, count = 0;
+  double beta = 0.3;
+  std::vector<DevBuf> xin, xou;
+  // GTH local parameters per species
+  std::vector<double> zps, rloc, cloc;
+  std::vector<double> frac;
+  std::vector<int> kind;
+};
+
+namespace {
+
+constexpr int kT = 256;
+const double kPi = 3.14159265358979323846;
+
+ScfState *state(rsg_ctx *ctx) {
+  if (!ctx->scf) ctx->scf = new ScfState();
+  return ctx->scf;
+}
+
+struct GridG {
+  int n1, n2, n3;
+  double b[9];  // reciprocal vectors b[3*k + r]
+  double ecut;
+};
+
+// Reduced frequency of FFT index i, or a sentinel outside -(n-1)/2..(n-1)/2
+__device__ inline bool freq(int i, int n, int &m) {
+  m = i <= (n - 1) / 2 ? i : i - n;
+  return m >= -(n - 1) / 2 && m <= (n - 1) / 2;
+}
+
+__device__ inline bool gvec(const GridG &g, int64_t j, double &G2, int m[3]) {
+  const int i1 = int(j % g.n1), i2 = int((j / g.n1) % g.n2), i3 = int(j / (int64_t(g.n1) * g.n2));
+  if (!freq(i1, g.n1, m[0]) || !freq(i2, g.n2, m[1]) || !freq(i3, g.n3, m[2])) return false;
+  double gx = g.b[0] * m[0] + g.b[3] * m[1] + g.b[6] * m[2];
+  double gy = g.b[1] * m[0] + g.b[4] * m[1] + g.b[7] * m[2];
+  double gz = g.b[2] * m[0] + g.b[5] * m[1] + g.b[8] * m[2];
+  G2 = gx * gx + gy * gy + gz * gz;
+  return G2 < g.ecut - 1e-14;
+}
+
+// V_loc(G) of one GTH species (ps_local_gth_module.f90:13-58), divided by
+// the cell volume.
+__device__ inline double gth_local_g(double G2, double zps, double rl, const double *c, double inv_vol) {
+  const double twopi3 = sqrt(8.0 * kPi * kPi * kPi);
+  if (G2 == 0.0)
+    return inv_vol * (2.0 * kPi * zps * rl * rl + twopi3 * rl * rl * rl * (c[0] + 3.0 * c[1] + 15.0 * c[2] + 105.0 * c[3]));
+  const double x2 = G2 * rl * rl, e = exp(-0.5 * x2);
+  double v = -4.0 * kPi * zps * e / G2;
+  v += twopi3 * rl * rl * rl * e *
+       (c[0] + c[1] * (3.0 - x2) + c[2] * (15.0 - 10.0 * x2 + x2 * x2) +
+        c[3] * (105.0 - 105.0 * x2 + 21.0 * x2 * x2 - x2 * x2 * x2));
+  return inv_vol * v;
+}
+
+// vg(G) = sum_a V_s(a)(|G|) exp(-i G.R_a) inside the cutoff sphere
+// (construct_ps_local_fft0, ps_local_fft0_module.f90:14-86)
+__global__ void k_vion_g(GridG g, int natom, const double *__restrict__ frac, const int *__restrict__ kind,
+                         const double *__restrict__ sp /* per species: zps, rloc, c1..c4 */, double inv_vol,
+                         double2 *__restrict__ z) {
+  const int64_t N = int64_t(g.n1) * g.n2 * g.n3;
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  double G2;
+  int m[3];
+  double2 acc = make_double2(0.0, 0.0);
+  if (gvec(g, j, G2, m)) {
+    for (int a = 0; a < natom; ++a) {
+      const double *p = sp + 6 * kind[a];
+      const double v = gth_local_g(G2, p[0], p[1], p + 2, inv_vol);
+      const double ph = -2.0 * kPi * (m[0] * frac[3 * a] + m[1] * frac[3 * a + 1] + m[2] * frac[3 * a + 2]);
+      double s, c;
+      sincos(ph, &s, &c);
+      acc.x += v * c;
+      acc.y += v * s;
+    }
+  }
+  z[j] = acc;
+}
+
+__global__ void k_real_part(const double2 *__restrict__ z, double *__restrict__ out, int64_t N) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < N) out[j] = z[j].x;
+}
+
+__global__ void k_to_complex(const double *__restrict__ r, double2 *__restrict__ z, int64_t N) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < N) z[j] = make_double2(r[j], 0.0);
+}
+
+// 4 pi / (N G^2) inside the cutoff, G = 0 dropped (hartree_sol_fftw_module.f90:29-130)
+__global__ void k_hartree_g(GridG g, double2 *__restrict__ z) {
+  const int64_t N = int64_t(g.n1) * g.n2 * g.n3;
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  double G2;
+  int m[3];
+  double f = 0.0;
+  if (gvec(g, j, G2, m) && !(m[0] == 0 && m[1] == 0 && m[2] == 0)) f = 4.0 * kPi / (double(N) * G2);
+  z[j] = make_double2(z[j].x * f, z[j].y * f);
+}
+
+// rho = sum_n occ_n psi_n^2 (density_module.f90:120-151)
+__global__ void k_density(const double *__restrict__ u, int64_t N, int mb, const double *__restrict__ occ,
+                          double *__restrict__ rho) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  double s = 0.0;
+  for (int n = 0; n < mb; ++n) {
+    const double o = occ[n];
+    if (o != 0.0) {
+      const double v = u[j + int64_t(n) * N];
+      s += o * v * v;
+    }
+  }
+  rho[j] = s;
+}
+
+// LDA Perdew-Zunger 81, spin-unpolarised (xc/xc_ldapz81_module.f90:78-216).
+// out: vxc, and exc_dens = rho*e_x + |rho|*e_c for the energy sum.
+__global__ void k_ldapz81(const double *__restrict__ rho, double *__restrict__ vxc, double *__restrict__ ed,
+                          int64_t N) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const double r = rho[j];
+  double vx = 0.0, ex = 0.0;
+  if (r > 0.0) {
+    const double exd = -0.75 * cbrt(3.0 / kPi * r);
+    ex = r * exd;
+    vx = (4.0 / 3.0) * exd;
+  }
+  double vc = 0.0, ec = 0.0;
+  const double t = fabs(r);  // rhoa + rhob with rhoa = rhob = |rho|/2
+  if (t > 0.0) {
+    const double gam = -0.1423, b1 = 1.0529, b2 = 0.3334;
+    const double A = 0.0311, B = -0.048, C = 0.002, D = -0.0116;
+    const double rs = cbrt(3.0 / (4.0 * kPi) / t);
+    double ecd, mu;
+    if (rs >= 1.0) {
+      const double sq = sqrt(rs);
+      ecd = gam / (1.0 + b1 * sq + b2 * rs);
+      mu = ecd + ecd * (1.0 / 3.0) * (0.5 * b1 / sq + b2) / (1.0 / rs + b1 / sq + b2);
+    } else {
+      const double ln = log(rs);
+      ecd = A * ln + B + C * rs * ln + D * rs;
+      mu = ecd - (1.0 / 3.0) * (A + C * rs * (1.0 + ln) + rs * D);
+    }
+    ec = t * ecd;
+    vc = mu;
+  }
+  vxc[j] = vx + vc;
+  ed[j] = ex + ec;
+}
+
+__global__ void k_add3(const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ c,
+                       double *__restrict__ out, float *__restrict__ out32, int64_t N) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const double v = a[j] + b[j] + c[j];
+  out[j] = v;
+  out32[j] = float(v);
+}
+
+__global__ void k_axpby(double *__restrict__ y, const double *__restrict__ x, double a, double b, int64_t N) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < N) y[j] = a * x[j] + b * y[j];
+}
+
+__global__ void k_gauss_rho(GridG g, const double *__restrict__ aa, int natom, const double *__restrict__ frac,
+                            const double *__restrict__ zat, double width, double *__restrict__ rho) {
+  const int64_t N = int64_t(g.n1) * g.n2 * g.n3;
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const int i1 = int(j % g.n1), i2 = int((j / g.n1) % g.n2), i3 = int(j / (int64_t(g.n1) * g.n2));
+  const double f[3] = {double(i1) / g.n1, double(i2) / g.n2, double(i3) / g.n3};
+  double s = 0.0;
+  const double w2 = width * width;
+  for (int a = 0; a < natom; ++a) {
+    double d[3];
+    for (int k = 0; k < 3; ++k) {
+      d[k] = f[k] - frac[3 * a + k];
+      d[k] -= rint(d[k]);
+    }
+    const double x = aa[0] * d[0] + aa[3] * d[1] + aa[6] * d[2];
+    const double y = aa[1] * d[0] + aa[4] * d[1] + aa[7] * d[2];
+    const double z = aa[2] * d[0] + aa[5] * d[1] + aa[8] * d[2];
+    s += zat[a] * exp(-(x * x + y * y + z * z) / w2);
+  }
+  rho[j] = s;
+}
+
+__global__ void k_fill_value(double *__restrict__ y, double v, int64_t N) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < N) y[j] = v;
+}
+
+inline unsigned nblk(int64_t N) { return unsigned((N + kT - 1) / kT); }
+
+GridG make_grid(const rsg_ctx *ctx, const ScfState *st) {
+  GridG g;
+  g.n1 = ctx->n[0];
+  g.n2 = ctx->n[1];
+  g.n3 = ctx->n[2];
+  for (int i = 0; i < 9; ++i) g.b[i] = st->bb[i];
+  g.ecut = st->ecut;
+  return g;
+}
+
+double *field_ptr(rsg_ctx *ctx, int field) {
+  ScfState *st = ctx->scf;
+  switch (field) {
+    case RSG_FIELD_RHO: return st->rho.as<double>();
+    case RSG_FIELD_RHO_OUT: return st->rho_out.as<double>();
+    case RSG_FIELD_VION: return st->vion.as<double>();
+    case RSG_FIELD_VH: return st->vh.as<double>();
+    case RSG_FIELD_VXC: return st->vxc.as<double>();
+    case RSG_FIELD_VLOC: return ctx->vloc64.as<double>();
+    default: return nullptr;
+  }
+}
+
+int ensure_state(rsg_ctx *ctx) {
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "scf: grid not set");
+  ScfState *st = state(ctx);
+  const size_t n8 = size_t(ctx->N) * 8;
+  if (st->rho.ensure(n8) || st->rho_out.ensure(n8) || st->vion.ensure(n8) || st->vh.ensure(n8) ||
+      st->vxc.ensure(n8) || st->work.ensure(n8) || st->zbuf.ensure(2 * n8))
+    return ctx->fail(RSG_ERR_HIP, "scf: field allocation failed");
+  if (!st->have_plan) {
+    if (hipfftPlan3d(&st->plan, ctx->n[2], ctx->n[1], ctx->n[0], HIPFFT_Z2Z) != HIPFFT_SUCCESS)
+      return ctx->fail(RSG_ERR_HIP, "scf: hipfftPlan3d failed");
+    if (hipfftSetStream(st->plan, ctx->stream) != HIPFFT_SUCCESS)
+      return ctx->fail(RSG_ERR_HIP, "scf: hipfftSetStream failed");
+    st->have_plan = true;
+  }
+  // reciprocal vectors and the density-grid cutoff of CalcCutoff_Ggrid
+  // (orthorhombic cell: Gcut = min_k |b_k| (2*floor((N_k-1)/2)+1) / 2)
+  const double *aa = ctx->aa;
+  const double vol = aa[0] * (aa[4] * aa[8] - aa[5] * aa[7]) - aa[1] * (aa[3] * aa[8] - aa[5] * aa[6]) +
+                     aa[2] * (aa[3] * aa[7] - aa[4] * aa[6]);
+  const double *a1 = aa, *a2 = aa + 3, *a3 = aa + 6;
+  const double f = 2.0 * kPi / vol;
+  double c[3][3] = {{a2[1] * a3[2] - a2[2] * a3[1], a2[2] * a3[0] - a2[0] * a3[2], a2[0] * a3[1] - a2[1] * a3[0]},
+                    {a3[1] * a1[2] - a3[2] * a1[1], a3[2] * a1[0] - a3[0] * a1[2], a3[0] * a1[1] - a3[1] * a1[0]},
+                    {a1[1] * a2[2] - a1[2] * a2[1], a1[2] * a2[0] - a1[0] * a2[2], a1[0] * a2[1] - a1[1] * a2[0]}};
+  double gc = 1e300;
+  for (int k = 0; k < 3; ++k) {
+    double bn = 0.0;
+    for (int r = 0; r < 3; ++r) {
+      st->bb[3 * k + r] = f * c[k][r];
+      bn += st->bb[3 * k + r] * st->bb[3 * k + r];
+    }
+    gc = std::min(gc, std::sqrt(bn) * (2 * ((ctx->n[k] - 1) / 2) + 1));
+  }
+  gc *= 0.5;
+  st->ecut = gc * gc;
+  return RSG_OK;
+}
+
+int fft(rsg_ctx *ctx, int dir) {
+  ScfState *st = ctx->scf;
+  hipfftDoubleComplex *z = reinterpret_cast<hipfftDoubleComplex *>(st->zbuf.p);
+  if (hipfftExecZ2Z(st->plan, z, z, dir) != HIPFFT_SUCCESS) return ctx->fail(RSG_ERR_HIP, "scf: hipfftExecZ2Z failed");
+  return RSG_OK;
+}
+
+int dot1(rsg_ctx *ctx, const double *x, const double *y, double alpha, double *out_host) {
+  ScfState *st = ctx->scf;
+  if (st->occ.ensure(64 * 8)) return ctx->fail(RSG_ERR_HIP, "scf: scratch allocation failed");
+  const double *xs[1] = {x}, *ys[1] = {y};
+  double *d = st->occ.as<double>() + 48;  // scalar slot past the occupations block
+  int rc = multidot(ctx, 1, xs, ys, 1, alpha, d);
+  if (rc) return rc;
+  RSG_HIP(hipMemcpyAsync(out_host, d, 8, hipMemcpyDeviceToHost, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+// Ewald energy of point ions in a neutralising background (the lattice sum
+// cal_ewald computes, ewald_module.f90:50-62), by the standard split into
+// real-space erfc and reciprocal Gaussian sums.
+double ewald_energy(const double *aa, const double *bb, double vol, int natom, const double *frac,
+                    const double *z) {
+  double q = 0.0, q2 = 0.0;
+  for (int a = 0; a < natom; ++a) {
+    q += z[a];
+    q2 += z[a] * z[a];
+  }
+  const double eta = kPi / std::pow(vol, 2.0 / 3.0);  // Gaussian width parameter
+  const double se = std::sqrt(eta);
+  const double rcut = 8.0 / se, gcut = 16.0 * se;
+  auto len = [](const double *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+  int nr[3], ng[3];
+  for (int k = 0; k < 3; ++k) {
+    nr[k] = int(std::ceil(rcut * len(bb + 3 * k) / (2 * kPi))) + 1;
+    ng[k] = int(std::ceil(gcut / len(bb + 3 * k))) + 1;
+  }
+  double er = 0.0;
+  for (int a = 0; a < natom; ++a)
+    for (int b = 0; b < natom; ++b)
+      for (int l1 = -nr[0]; l1 <= nr[0]; ++l1)
+        for (int l2 = -nr[1]; l2 <= nr[1]; ++l2)
+          for (int l3 = -nr[2]; l3 <= nr[2]; ++l3) {
+            if (a == b && l1 == 0 && l2 == 0 && l3 == 0) continue;
+            const double d[3] = {frac[3 * b] - frac[3 * a] + l1, frac[3 * b + 1] - frac[3 * a + 1] + l2,
+                                 frac[3 * b + 2] - frac[3 * a + 2] + l3};
+            const double x = aa[0] * d[0] + aa[3] * d[1] + aa[6] * d[2];
+            const double y = aa[1] * d[0] + aa[4] * d[1] + aa[7] * d[2];
+            const double zz = aa[2] * d[0] + aa[5] * d[1] + aa[8] * d[2];
+            const double r = std::sqrt(x * x + y * y + zz * zz);
+            if (r < rcut) er += z[a] * z[b] * std::erfc(se * r) / r;
+          }
+  er *= 0.5;
+  double eg = 0.0;
+  for (int m1 = -ng[0]; m1 <= ng[0]; ++m1)
+    for (int m2 = -ng[1]; m2 <= ng[1]; ++m2)
+      for (int m3 = -ng[2]; m3 <= ng[2]; ++m3) {
+        if (m1 == 0 && m2 == 0 && m3 == 0) continue;
+        const double gx = bb[0] * m1 + bb[3] * m2 + bb[6] * m3;
+        const double gy = bb[1] * m1 + bb[4] * m2 + bb[7] * m3;
+        const double gz = bb[2] * m1 + bb[5] * m2 + bb[8] * m3;
+        const double g2 = gx * gx + gy * gy + gz * gz;
+        if (g2 > gcut * gcut) continue;
+        double sr = 0.0, si = 0.0;
+        for (int a = 0; a < natom; ++a) {
+          const double ph = 2.0 * kPi * (m1 * frac[3 * a] + m2 * frac[3 * a + 1] + m3 * frac[3 * a + 2]);
+          sr += z[a] * std::cos(ph);
+          si += z[a] * std::sin(ph);
+        }
+        eg += std::exp(-g2 / (4.0 * eta)) / g2 * (sr * sr + si * si);
+      }
+  eg *= 2.0 * kPi / vol;
+  return er + eg - std::sqrt(eta / kPi) * q2 - kPi * q * q / (2.0 * eta * vol);
+}
+
+// small dense solve A x = b (partial pivoting), in place
+bool solve_small(int n, std::vector<double> &A, std::vector<double> &b) {
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(A[i + n * k]) > std::fabs(A[p + n * k])) p = i;
+    if (A[p + n * k] == 0.0) return false;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[k + n * j], A[p + n * j]);
+      std::swap(b[k], b[p]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[i + n * k] / A[k + n * k];
+      for (int j = k; j < n; ++j) A[i + n * j] -= f * A[k + n * j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    double s = b[k];
+    for (int j = k + 1; j < n; ++j) s -= A[k + n * j] * b[j];
+    b[k] = s / A[k + n * k];
+  }
+  return true;
+}
+
+}  // namespace
+
+int scf_set_local_gth(rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind, const double *zps,
+                      const double *rloc, const double *cloc) {
+  if (natom < 0 || nkind < 1 || (natom > 0 && (!frac || !kind)) || !zps || !rloc || !cloc)
+    return ctx->fail(RSG_ERR_ARG, "set_local_gth: bad arguments");
+  int rc = ensure_state(ctx);
+  if (rc) return rc;
+  ScfState *st = ctx->scf;
+  const int64_t N = ctx->N;
+  std::vector<double> sp(static_cast<size_t>(6 * nkind));
+  for (int k = 0; k < nkind; ++k) {
+    if (!(rloc[k] > 0)) return ctx->fail(RSG_ERR_ARG, "set_local_gth: rloc must be > 0");
+    sp[6 * k] = zps[k];
+    sp[6 * k + 1] = rloc[k];
+    for (int i = 0; i < 4; ++i) sp[6 * k + 2 + i] = cloc[4 * k + i];
+  }
+  for (int a = 0; a < natom; ++a)
+    if (kind[a] < 0 || kind[a] >= nkind) return ctx->fail(RSG_ERR_ARG, "set_local_gth: kind out of range");
+  DevBuf dfrac, dkind, dsp;
+  if (dfrac.ensure(size_t(std::max(natom, 1)) * 24) || dkind.ensure(size_t(std::max(natom, 1)) * 4) ||
+      dsp.ensure(sp.size() * 8))
+    return ctx->fail(RSG_ERR_HIP, "set_local_gth: allocation failed");
+  if (natom > 0) {
+    RSG_HIP(hipMemcpyAsync(dfrac.p, frac, size_t(natom) * 24, hipMemcpyHostToDevice, ctx->stream));
+    RSG_HIP(hipMemcpyAsync(dkind.p, kind, size_t(natom) * 4, hipMemcpyHostToDevice, ctx->stream));
+  }
+  RSG_HIP(hipMemcpyAsync(dsp.p, sp.data(), sp.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  const double vol = ctx->dv * double(N);
+  GridG g = make_grid(ctx, st);
+  hipLaunchKernelGGL(k_vion_g, dim3(nblk(N)), dim3(kT), 0, ctx->stream, g, natom, dfrac.as<double>(),
+                     dkind.as<int>(), dsp.as<double>(), 1.0 / vol, st->zbuf.as<double2>());
+  if ((rc = fft(ctx, HIPFFT_BACKWARD))) return rc;
+  hipLaunchKernelGGL(k_real_part, dim3(nblk(N)), dim3(kT), 0, ctx->stream, st->zbuf.as<double2>(),
+                     st->vion.as<double>(), N);
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  dfrac.release();
+  dkind.release();
+  dsp.release();
+  std::vector<double> zat(static_cast<size_t>(natom));
+  st->nelectron = 0.0;
+  for (int a = 0; a < natom; ++a) {
+    zat[size_t(a)] = zps[kind[a]];
+    st->nelectron += zat[size_t(a)];
+  }
+  st->eewald = ewald_energy(ctx->aa, st->bb, vol, natom, frac, zat.data());
+  st->frac.assign(frac, frac + 3 * natom);
+  st->kind.assign(kind, kind + natom);
+  st->zps.assign(zps, zps + nkind);
+  st->have_local = true;
+  return RSG_OK;
+}
+
+int scf_constants(rsg_ctx *ctx, double *eewald, double *nel) {
+  if (!ctx->scf || !ctx->scf->have_local) return ctx->fail(RSG_ERR_STATE, "scf: local potential not set");
+  if (eewald) *eewald = ctx->scf->eewald;
+  if (nel) *nel = ctx->scf->nelectron;
+  return RSG_OK;
+}
+
+int scf_field(rsg_ctx *ctx, int field, double *host, bool get) {
+  int rc = ensure_state(ctx);
+  if (rc) return rc;
+  double *d = field_ptr(ctx, field);
+  if (!d) return ctx->fail(RSG_ERR_ARG, "scf: unknown field");
+  if (get) {
+    RSG_HIP(hipMemcpyAsync(host, d, size_t(ctx->N) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  } else {
+    RSG_HIP(hipMemcpyAsync(d, host, size_t(ctx->N) * 8, hipMemcpyHostToDevice, ctx->stream));
+    if (field == RSG_FIELD_VLOC) {
+      std::vector<float> v(host, host + ctx->N);
+      RSG_HIP(hipMemcpyAsync(ctx->vloc32.p, v.data(), size_t(ctx->N) * 4, hipMemcpyHostToDevice, ctx->stream));
+      RSG_HIP(hipStreamSynchronize(ctx->stream));
+      ctx->have_vloc = true;
+    }
+  }
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+int scf_init_density(rsg_ctx *ctx, double width) {
+  int rc = ensure_state(ctx);
+  if (rc) return rc;
+  ScfState *st = ctx->scf;
+  if (!st->have_local) return ctx->fail(RSG_ERR_STATE, "init_density: local potential not set");
+  if (!(width > 0)) return ctx->fail(RSG_ERR_ARG, "init_density: width must be > 0");
+  const int natom = int(st->kind.size());
+  const int64_t N = ctx->N;
+  std::vector<double> zat(static_cast<size_t>(natom));
+  for (int a = 0; a < natom; ++a) zat[size_t(a)] = st->zps[size_t(st->kind[size_t(a)])];
+  DevBuf dfrac, dz, daa;
+  if (dfrac.ensure(size_t(std::max(natom, 1)) * 24) || dz.ensure(size_t(std::max(natom, 1)) * 8) || daa.ensure(72))
+    return ctx->fail(RSG_ERR_HIP, "init_density: allocation failed");
+  if (natom > 0) {
+    RSG_HIP(hipMemcpyAsync(dfrac.p, st->frac.data(), size_t(natom) * 24, hipMemcpyHostToDevice, ctx->stream));
+    RSG_HIP(hipMemcpyAsync(dz.p, zat.data(), size_t(natom) * 8, hipMemcpyHostToDevice, ctx->stream));
+  }
+  RSG_HIP(hipMemcpyAsync(daa.p, ctx->aa, 72, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_gauss_rho, dim3(nblk(N)), dim3(kT), 0, ctx->stream, make_grid(ctx, st), daa.as<double>(),
+                     natom, dfrac.as<double>(), dz.as<double>(), width, st->rho.as<double>());
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return scf_normalize(ctx);
+}
+
+// rho <- rho * Nel / (dV sum rho)  (normalize_initrho, ps_initrho_module.f90:550-560)
+int scf_normalize(rsg_ctx *ctx) {
+  int rc = ensure_state(ctx);
+  if (rc) return rc;
+  ScfState *st = ctx->scf;
+  const int64_t N = ctx->N;
+  double *ones = st->work.as<double>();
+  hipLaunchKernelGGL(k_fill_value, dim3(nblk(N)), dim3(kT), 0, ctx->stream, ones, 1.0, N);
+  double q = 0.0;
+  if ((rc = dot1(ctx, st->rho.as<double>(), ones, ctx->dv, &q))) return rc;
+  if (!(q > 0.0)) return ctx->fail(RSG_ERR_ARG, "normalize: density integrates to a non-positive charge");
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(N)), dim3(kT), 0, ctx->stream, st->rho.as<double>(), st->rho.as<double>(),
+                     0.0, st->nelectron / q, N);
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+int scf_calc_density(rsg_ctx *ctx, const double *occ) {
+  int rc = ensure_state(ctx);
+  if (rc) return rc;
+  if (ctx->mb <= 0) return ctx->fail(RSG_ERR_STATE, "calc_density: no band store");
+  ScfState *st = ctx->scf;
+  if (st->occ.ensure(size_t(std::max(ctx->mb, 64)) * 8 + 64 * 8))
+    return ctx->fail(RSG_ERR_HIP, "calc_density: allocation failed");
+  double *docc = st->occ.as<double>() + 64;  // slots 0..63 are dot scratch
+  RSG_HIP(hipMemcpyAsync(docc, occ, size_t(ctx->mb) * 8, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_density, dim3(nblk(ctx->N)), dim3(kT), 0, ctx->stream, ctx->unk.as<double>(), ctx->N,
+                     ctx->mb, docc, st->rho_out.as<double>());
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+int scf_hartree_xc(rsg_ctx *ctx, int field, double *eh, double *exc) {
+  int rc = ensure_state(ctx);
+  if (rc) return rc;
+  if (field != RSG_FIELD_RHO && field != RSG_FIELD_RHO_OUT)
+    return ctx->fail(RSG_ERR_ARG, "hartree_xc: field must be RHO or RHO_OUT");
+  ScfState *st = ctx->scf;
+  const int64_t N = ctx->N;
+  const double *rho = field_ptr(ctx, field);
+  hipLaunchKernelGGL(k_to_complex, dim3(nblk(N)), dim3(kT), 0, ctx->stream, rho, st->zbuf.as<double2>(), N);
+  if ((rc = fft(ctx, HIPFFT_FORWARD))) return rc;
+  hipLaunchKernelGGL(k_hartree_g, dim3(nblk(N)), dim3(kT), 0, ctx->stream, make_grid(ctx, st), st->zbuf.as<double2>());
+  if ((rc = fft(ctx, HIPFFT_BACKWARD))) return rc;
+  hipLaunchKernelGGL(k_real_part, dim3(nblk(N)), dim3(kT), 0, ctx->stream, st->zbuf.as<double2>(),
+                     st->vh.as<double>(), N);
+  hipLaunchKernelGGL(k_ldapz81, dim3(nblk(N)), dim3(kT), 0, ctx->stream, rho, st->vxc.as<double>(),
+                     st->work.as<double>(), N);
+  double e = 0.0;
+  if ((rc = dot1(ctx, rho, st->vh.as<double>(), 0.5 * ctx->dv, &e))) return rc;
+  if (eh) *eh = e;
+  // exc: dV * sum of the energy density (dotted with ones held in zbuf)
+  double *ones = reinterpret_cast<double *>(st->zbuf.p);
+  hipLaunchKernelGGL(k_fill_value, dim3(nblk(N)), dim3(kT), 0, ctx->stream, ones, 1.0, N);
+  if ((rc = dot1(ctx, st->work.as<double>(), ones, ctx->dv, &e))) return rc;
+  if (exc) *exc = e;
+  return RSG_OK;
+}
+
+int scf_update_vloc(rsg_ctx *ctx) {
+  int rc = ensure_state(ctx);
+  if (rc) return rc;
+  ScfState *st = ctx->scf;
+  if (!st->have_local) return ctx->fail(RSG_ERR_STATE, "update_vloc: local potential not set");
+  hipLaunchKernelGGL(k_add3, dim3(nblk(ctx->N)), dim3(kT), 0, ctx->stream, st->vion.as<double>(),
+                     st->vh.as<double>(), st->vxc.as<double>(), ctx->vloc64.as<double>(), ctx->vloc32.as<float>(),
+                     ctx->N);
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->have_vloc = true;
+  return RSG_OK;
+}
+
+int scf_density_integrals(rsg_ctx *ctx, int field, double *eloc, double *eion) {
+  int rc = ensure_state(ctx);
+  if (rc) return rc;
+  if (field != RSG_FIELD_RHO && field != RSG_FIELD_RHO_OUT)
+    return ctx->fail(RSG_ERR_ARG, "density_integrals: field must be RHO or RHO_OUT");
+  const double *rho = field_ptr(ctx, field);
+  double e = 0.0;
+  if ((rc = dot1(ctx, rho, ctx->vloc64.as<double>(), ctx->dv, &e))) return rc;
+  if (eloc) *eloc = e;
+  if ((rc = dot1(ctx, rho, ctx->scf->vion.as<double>(), ctx->dv, &e))) return rc;
+  if (eion) *eion = e;
+  return RSG_OK;
+}
+
+// init_mixing (mixing_module.f90:120-126): history slot mmix holds rho_in
+int scf_mix_init(rsg_ctx *ctx, int mmix, double beta) {
+  int rc = ensure_state(ctx);
+  if (rc) return rc;
+  if (mmix < 1 || mmix > 32 || !(beta > 0.0 && beta <= 1.0))
+    return ctx->fail(RSG_ERR_ARG, "mix_init: mmix in 1..32 and 0 < beta <= 1");
+  ScfState *st = ctx->scf;
+  for (auto &b : st->xin) b.release();
+  for (auto &b : st->xou) b.release();
+  st->xin.assign(size_t(mmix), DevBuf());
+  st->xou.assign(size_t(mmix), DevBuf());
+  const size_t n8 = size_t(ctx->N) * 8;
+  for (int k = 0; k < mmix; ++k)
+    if (st->xin[size_t(k)].ensure(n8) || st->xou[size_t(k)].ensure(n8))
+      return ctx->fail(RSG_ERR_HIP, "mix_init: history allocation failed");
+  st->mmix = mmix;
+  st->beta = beta;
+  st->count = 0;
+  RSG_HIP(hipMemcpyAsync(st->xin[size_t(mmix - 1)].p, st->rho.p, n8, hipMemcpyDeviceToDevice, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+// pulay_mixing (mixing_pulay_module.f90:29-121) on f = rho_out; the mixed
+// input density lands in history slot mmix and in rho.
+int scf_mix_pulay(rsg_ctx *ctx) {
+  int rc = ensure_state(ctx);
+  if (rc) return rc;
+  ScfState *st = ctx->scf;
+  const int mm = st->mmix;
+  if (mm < 1) return ctx->fail(RSG_ERR_STATE, "mix_pulay: mixing not initialised");
+  const int64_t N = ctx->N;
+  const size_t n8 = size_t(N) * 8;
+  auto rotate = [&](std::vector<DevBuf> &h) -> int {
+    // Xin(:,:,i-1) = Xin(:,:,i), i = 2..mmix; slot mmix keeps its value
+    DevBuf first = h[0];
+    for (int i = 1; i < mm; ++i) h[size_t(i - 1)] = h[size_t(i)];
+    h[size_t(mm - 1)] = first;
+    if (mm > 1) {
+      if (hipMemcpyAsync(h[size_t(mm - 1)].p, h[size_t(mm - 2)].p, n8, hipMemcpyDeviceToDevice, ctx->stream))
+        return RSG_ERR_HIP;
+    }
+    return RSG_OK;
+  };
+  RSG_HIP(hipMemcpyAsync(st->xou[size_t(mm - 1)].p, st->rho_out.p, n8, hipMemcpyDeviceToDevice, ctx->stream));
+  st->count += 1;
+  const int m0 = std::min(st->count, mm);
+  double *X = st->work.as<double>();
+  if (mm == 1 || m0 < mm) {
+    if (rotate(st->xin) || rotate(st->xou)) return ctx->fail(RSG_ERR_HIP, "mix_pulay: history copy failed");
+    double *xin = st->xin[size_t(mm - 1)].as<double>();
+    const double *xou = st->xou[size_t(mm - 1)].as<double>();
+    // Xin = Xin + beta (Xou - Xin)
+    hipLaunchKernelGGL(k_axpby, dim3(nblk(N)), dim3(kT), 0, ctx->stream, xin, xou, st->beta, 1.0 - st->beta, N);
+  } else {
+    // residuals R_i = Xou_i - Xin_i held in zbuf-sized scratch one at a time
+    std::vector<double> A(size_t(m0) * m0), b(size_t(m0), 1.0);
+    const int base = mm - m0;
+    // A(i,j) = sum R_i R_j = sum (Xou_i Xou_j - Xou_i Xin_j - Xin_i Xou_j + Xin_i Xin_j)
+    for (int i = 0; i < m0; ++i)
+      for (int j = 0; j <= i; ++j) {
+        const double *xs[4] = {st->xou[size_t(base + i)].as<double>(), st->xou[size_t(base + i)].as<double>(),
+                               st->xin[size_t(base + i)].as<double>(), st->xin[size_t(base + i)].as<double>()};
+        const double *ys[4] = {st->xou[size_t(base + j)].as<double>(), st->xin[size_t(base + j)].as<double>(),
+                               st->xou[size_t(base + j)].as<double>(), st->xin[size_t(base + j)].as<double>()};
+        if (st->occ.ensure(64 * 8 + size_t(std::max(ctx->mb, 64)) * 8))
+          return ctx->fail(RSG_ERR_HIP, "mix_pulay: scratch allocation failed");
+        double *d = st->occ.as<double>();
+        if ((rc = multidot(ctx, 4, xs, ys, 1, 1.0, d))) return rc;
+        double h[4];
+        RSG_HIP(hipMemcpyAsync(h, d, 32, hipMemcpyDeviceToHost, ctx->stream));
+        RSG_HIP(hipStreamSynchronize(ctx->stream));
+        const double v = h[0] - h[1] - h[2] + h[3];
+        A[size_t(i) + size_t(m0) * j] = A[size_t(j) + size_t(m0) * i] = v;
+      }
+    if (!solve_small(m0, A, b)) return ctx->fail(RSG_ERR_SOLVER, "mix_pulay: singular residual matrix");
+    double sb = 0.0;
+    for (double v : b) sb += v;
+    for (double &v : b) v /= sb;
+    // X = sum b_i Xin_i, Y = sum b_i Xou_i (Y into rho_out scratch)
+    double *Y = st->rho_out.as<double>();
+    hipLaunchKernelGGL(k_fill_value, dim3(nblk(N)), dim3(kT), 0, ctx->stream, X, 0.0, N);
+    hipLaunchKernelGGL(k_fill_value, dim3(nblk(N)), dim3(kT), 0, ctx->stream, Y, 0.0, N);
+    for (int i = 0; i < m0; ++i) {
+      hipLaunchKernelGGL(k_axpby, dim3(nblk(N)), dim3(kT), 0, ctx->stream, X, st->xin[size_t(base + i)].as<double>(),
+                         b[size_t(i)], 1.0, N);
+      hipLaunchKernelGGL(k_axpby, dim3(nblk(N)), dim3(kT), 0, ctx->stream, Y, st->xou[size_t(base + i)].as<double>(),
+                         b[size_t(i)], 1.0, N);
+    }
+    if (rotate(st->xin) || rotate(st->xou)) return ctx->fail(RSG_ERR_HIP, "mix_pulay: history copy failed");
+    // Xin(mmix) = X + beta (Y - X)
+    double *xin = st->xin[size_t(mm - 1)].as<double>();
+    RSG_HIP(hipMemcpyAsync(xin, X, n8, hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_axpby, dim3(nblk(N)), dim3(kT), 0, ctx->stream, xin, Y, st->beta, 1.0 - st->beta, N);
+    // restore rho_out (the unmixed output) from the history
+    RSG_HIP(hipMemcpyAsync(Y, st->xou[size_t(mm - 1)].p, n8, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  RSG_HIP(hipMemcpyAsync(st->rho.p, st->xin[size_t(mm - 1)].p, n8, hipMemcpyDeviceToDevice, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+void scf_release(rsg_ctx *ctx) {
+  ScfState *st = ctx->scf;
+  if (!st) return;
+  if (st->have_plan) hipfftDestroy(st->plan);
+  for (DevBuf *b : {&st->rho, &st->rho_out, &st->vion, &st->vh, &st->vxc, &st->zbuf, &st->work, &st->occ}) b->release();
+  for (auto &b : st->xin) b.release();
+  for (auto &b : st->xou) b.release();
+  delete st;
+  ctx->scf = nullptr;
+}
+
+}  // namespace rsg

@@ -14,4 +14,18 @@ void solver_release(rsg_ctx *ctx);
 // device.  Deterministic two-pass reduction.
 int multidot(rsg_ctx *ctx, int npair, const double *const *x, const double *const *y, int nb,
              double alpha, double *out);
+// scf.hip
+int scf_set_local_gth(rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind, const double *zps,
+                      const double *rloc, const double *cloc);
+int scf_constants(rsg_ctx *ctx, double *eewald, double *nel);
+int scf_field(rsg_ctx *ctx, int field, double *host, bool get);
+int scf_init_density(rsg_ctx *ctx, double width);
+int scf_normalize(rsg_ctx *ctx);
+int scf_calc_density(rsg_ctx *ctx, const double *occ);
+int scf_hartree_xc(rsg_ctx *ctx, int field, double *eh, double *exc);
+int scf_update_vloc(rsg_ctx *ctx);
+int scf_density_integrals(rsg_ctx *ctx, int field, double *eloc, double *eion);
+int scf_mix_init(rsg_ctx *ctx, int mmix, double beta);
+int scf_mix_pulay(rsg_ctx *ctx);
+void scf_release(rsg_ctx *ctx);
 }  // namespace rsg

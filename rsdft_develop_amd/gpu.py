@@ -26,6 +26,7 @@ LIB_PATH = os.path.join(HERE, "lib", "librsdft_gpu.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "rsdft_gpu.h")
 
 RSG_ERR_NOTPD = 4
+FIELD_RHO, FIELD_RHO_OUT, FIELD_VION, FIELD_VH, FIELD_VXC, FIELD_VLOC = range(6)
 
 
 class RSDFTGPUError(RuntimeError):
@@ -80,6 +81,18 @@ def load_library():
         "rsg_subspace_diag": (_c_int, [_vp, _vp]),
         "rsg_conjugate_gradient": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp]),
         "rsg_esp_calc": (_c_int, [_vp, _vp]),
+        "rsg_set_local_gth": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _vp, _vp, _vp]),
+        "rsg_get_scf_constants": (_c_int, [_vp, _P(_c_dbl), _P(_c_dbl)]),
+        "rsg_get_field": (_c_int, [_vp, _c_int, _vp]),
+        "rsg_set_field": (_c_int, [_vp, _c_int, _vp]),
+        "rsg_init_density": (_c_int, [_vp, _c_dbl]),
+        "rsg_normalize_density": (_c_int, [_vp]),
+        "rsg_calc_density": (_c_int, [_vp, _vp]),
+        "rsg_hartree_xc": (_c_int, [_vp, _c_int, _P(_c_dbl), _P(_c_dbl)]),
+        "rsg_update_vloc": (_c_int, [_vp]),
+        "rsg_density_integrals": (_c_int, [_vp, _c_int, _P(_c_dbl), _P(_c_dbl)]),
+        "rsg_mix_init": (_c_int, [_vp, _c_int, _c_dbl]),
+        "rsg_mix_pulay": (_c_int, [_vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -279,3 +292,57 @@ class Context:
         esp = np.zeros(self.mb)
         self._check(self.L.rsg_esp_calc(self.h, _ptr(esp)))
         return esp
+
+    # ------------------------------------------- density/potential half
+    def set_local_gth(self, cell):
+        """Vion (GTH local part), Ewald energy and electron count for ``cell``."""
+        nk = len(cell.species)
+        zps = np.array([sp.zps for sp in cell.species], dtype=np.float64)
+        rloc = np.array([sp.rloc for sp in cell.species], dtype=np.float64)
+        cloc = np.array([c for sp in cell.species for c in sp.cloc], dtype=np.float64)
+        frac, kind = _f64(cell.frac), _i32(cell.kind)
+        self._check(self.L.rsg_set_local_gth(self.h, cell.natom, _ptr(frac), _ptr(kind), nk,
+                                             _ptr(zps), _ptr(rloc), _ptr(cloc)))
+
+    def scf_constants(self):
+        e, n = _c_dbl(), _c_dbl()
+        self._check(self.L.rsg_get_scf_constants(self.h, ctypes.byref(e), ctypes.byref(n)))
+        return e.value, n.value
+
+    def get_field(self, field: int) -> np.ndarray:
+        out = np.zeros(self.N)
+        self._check(self.L.rsg_get_field(self.h, int(field), _ptr(out)))
+        return out
+
+    def set_field(self, field: int, values):
+        v = _f64(values)
+        self._check(self.L.rsg_set_field(self.h, int(field), _ptr(v)))
+
+    def init_density(self, width: float = 1.0):
+        self._check(self.L.rsg_init_density(self.h, float(width)))
+
+    def normalize_density(self):
+        self._check(self.L.rsg_normalize_density(self.h))
+
+    def calc_density(self, occ):
+        o = _f64(occ)
+        self._check(self.L.rsg_calc_density(self.h, _ptr(o)))
+
+    def hartree_xc(self, field: int):
+        eh, exc = _c_dbl(), _c_dbl()
+        self._check(self.L.rsg_hartree_xc(self.h, int(field), ctypes.byref(eh), ctypes.byref(exc)))
+        return eh.value, exc.value
+
+    def update_vloc(self):
+        self._check(self.L.rsg_update_vloc(self.h))
+
+    def density_integrals(self, field: int):
+        a, b = _c_dbl(), _c_dbl()
+        self._check(self.L.rsg_density_integrals(self.h, int(field), ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def mix_init(self, mmix: int = 4, beta: float = 0.3):
+        self._check(self.L.rsg_mix_init(self.h, int(mmix), float(beta)))
+
+    def mix_pulay(self):
+        self._check(self.L.rsg_mix_pulay(self.h))

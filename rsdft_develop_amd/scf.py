@@ -1,0 +1,165 @@
+"""Kohn-Sham SCF driver over the C ABI, in the order of RSDFT's calc_scf.
+
+One iteration (scf_module.f90:183-481, defaults Ndiag = 1, second_diag = 2):
+subspace_diag -> conjugate_gradient -> gram_schmidt -> esp_calc -> Fermi
+occupations -> Harris energy with the input density -> output density ->
+Hartree and XC of the output density -> Kohn-Sham total energy -> convergence
+test -> Pulay mixing of the density -> Hartree/XC of the mixed density ->
+new Vloc.  Bands, densities and potentials stay on the GPU; only band
+energies, occupations and scalars cross to the host.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+
+import numpy as np
+
+from . import gpu, systems
+
+
+@dataclasses.dataclass
+class ScfParams:
+    ncg: int = 4            # NCG
+    pc_kind: int = 4        # IPC: 4 = hprsdft, 0 = none
+    mloop: int = 3          # MLOOP
+    mmix: int = 4           # MMIX
+    beta: float = 0.3       # BETA
+    ekbt: float = 1e-5      # EKBT (Fermi-Dirac temperature, Ha)
+    etol_per_atom: float = 1e-8   # stop when |E_k - E_{k-1}| / natom < etol
+    max_iter: int = 200     # DITER
+    init_width: float = 1.0  # width (bohr) of the initial atomic Gaussians
+    seed: int = 0           # N(0,1) initial bands
+
+
+@dataclasses.dataclass
+class ScfIteration:
+    iter: int
+    etot: float
+    eharris: float
+    de_per_atom: float
+    esp: np.ndarray
+    times: dict
+
+
+@dataclasses.dataclass
+class ScfResult:
+    converged: bool
+    iterations: list
+    etot: float
+    esp: np.ndarray
+    occ: np.ndarray
+    eewald: float
+    wall: float
+
+    @property
+    def niter(self) -> int:
+        return len(self.iterations)
+
+
+def fermi_dirac(esp: np.ndarray, nelectron: float, kT: float, nspin: int = 1):
+    """Occupations 2/nspin * f((e - mu)/kT) with mu by bisection so that the
+    occupations sum to the electron count (fermi_dirac in fermi_module.f90,
+    mp_version = -1).  Returns (occ, mu)."""
+    esp = np.asarray(esp, dtype=np.float64)
+    w = 2.0 / nspin
+
+    def count(mu):
+        x = np.clip((esp - mu) / kT, -700.0, 700.0)
+        return np.sum(w / (1.0 + np.exp(x)))
+
+    lo, hi = esp.min() - 50.0 * kT - 1.0, esp.max() + 50.0 * kT + 1.0
+    if count(hi) < nelectron - 1e-12:
+        raise ValueError("not enough bands for the electron count")
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if count(mid) > nelectron:
+            hi = mid
+        else:
+            lo = mid
+        if hi - lo < 1e-15 * max(1.0, abs(mid)):
+            break
+    mu = 0.5 * (lo + hi)
+    x = np.clip((esp - mu) / kT, -700.0, 700.0)
+    return w / (1.0 + np.exp(x)), mu
+
+
+class SCF:
+    """Plain FP64 SCF of one cell on one GPU."""
+
+    def __init__(self, cell: systems.Cell, params: ScfParams | None = None, ctx: gpu.Context | None = None,
+                 bands: np.ndarray | None = None):
+        self.cell = cell
+        self.p = params or ScfParams()
+        self.ctx = ctx or gpu.Context(0)
+        c = self.ctx
+        c.cell = cell
+        c.set_grid(cell.ngrid, cell.aa, cell.md)
+        c.build_gth_projectors(cell)
+        c.set_local_gth(cell)
+        self.eewald, self.nelectron = c.scf_constants()
+        c.init_density(self.p.init_width)
+        self.eh_in, self.exc_in = c.hartree_xc(gpu.FIELD_RHO)
+        c.update_vloc()
+        c.alloc_bands(cell.nband)
+        u0 = bands if bands is not None else systems.random_bands(cell.npoints, cell.nband, seed=self.p.seed)
+        c.upload_bands(u0)
+        c.gram_schmidt(1, cell.nband)   # main.f90:392
+        c.mix_init(self.p.mmix, self.p.beta)
+
+    def step(self, it: int, e_prev: float | None) -> ScfIteration:
+        c, p = self.ctx, self.p
+        t = {}
+        t0 = time.perf_counter()
+        c.subspace_diag()
+        t["sd"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        c.conjugate_gradient(p.ncg, p.pc_kind, p.mloop)
+        t["cg"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        c.gram_schmidt()
+        t["gs"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        esp = c.esp_calc()
+        t["esp"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        occ, _ = fermi_dirac(esp, self.nelectron, p.ekbt)
+        eeig = float(np.sum(occ * esp))
+        eloc_in, eion_in = c.density_integrals(gpu.FIELD_RHO)
+        eharris = eeig - eloc_in + self.eh_in + self.exc_in + eion_in + self.eewald
+        c.calc_density(occ)
+        eh, exc = c.hartree_xc(gpu.FIELD_RHO_OUT)
+        eloc, eion = c.density_integrals(gpu.FIELD_RHO_OUT)
+        etot = eeig - eloc + eh + exc + eion + self.eewald
+        t["eto"] = time.perf_counter() - t0
+        de = abs(etot - e_prev) / self.cell.natom if e_prev is not None else np.inf
+        self.occ = occ
+        return ScfIteration(it, etot, eharris, de, esp, t)
+
+    def update_potential(self) -> float:
+        t0 = time.perf_counter()
+        c = self.ctx
+        c.mix_pulay()
+        c.normalize_density()
+        self.eh_in, self.exc_in = c.hartree_xc(gpu.FIELD_RHO)
+        c.update_vloc()
+        return time.perf_counter() - t0
+
+    def run(self, callback=None) -> ScfResult:
+        its = []
+        e_prev = None
+        conv = False
+        w0 = time.perf_counter()
+        for it in range(1, self.p.max_iter + 1):
+            rec = self.step(it, e_prev)
+            conv = rec.de_per_atom < self.p.etol_per_atom
+            if not conv and it < self.p.max_iter:
+                rec.times["pot"] = self.update_potential()
+            its.append(rec)
+            if callback:
+                callback(rec)
+            e_prev = rec.etot
+            if conv:
+                break
+        wall = time.perf_counter() - w0
+        return ScfResult(conv, its, its[-1].etot, its[-1].esp, self.occ, self.eewald, wall)
