@@ -168,6 +168,8 @@ def gauss_density(cell, width, nelectron):
 
 
 def fermi_dirac(esp, nel, kT):
+    """Spin-unpolarised occupations (0..2) with the chemical potential found by
+    bisection so that they sum to ``nel`` (fermi_dirac_module.f90:11-110)."""
     lo, hi = esp.min() - 1.0, esp.max() + 1.0
     def cnt(mu):
         return np.sum(2.0 / (1.0 + np.exp(np.clip((esp - mu) / kT, -700, 700))))
@@ -179,6 +181,19 @@ def fermi_dirac(esp, nel, kT):
             lo = mid
     mu = 0.5 * (lo + hi)
     return 2.0 / (1.0 + np.exp(np.clip((esp - mu) / kT, -700, 700)))
+
+
+def entropy_fd(occ, kT):
+    """-T S of the occupations, as fermi_dirac_module.f90:325-350 (ns = 1):
+    2 kT sum_n [-f ln f - (1-f) ln(1-f)], f = occ/2."""
+    f = np.asarray(occ) * 0.5
+    s = np.zeros_like(f)
+    m = f > 0
+    s[m] -= f[m] * np.log(f[m])
+    g = 1.0 - f
+    m = g > 0
+    s[m] -= g[m] * np.log(g[m])
+    return float(2.0 * kT * s.sum())
 
 
 class Pulay:
@@ -214,7 +229,12 @@ class Pulay:
 
 def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-8, max_iter=200,
             init_width=1.0, seed=0, gs_variant="t", bands=None, callback=None):
-    """Plain FP64 SCF in the reference's order; returns a dict of results."""
+    """Plain FP64 SCF in the reference's order; returns a dict of results.
+
+    Convergence is the reference's energy test (scf_module.f90:335-339):
+    |Etot - Eharris| / natom < etol_per_atom, Etot the Kohn-Sham energy of the
+    output density and Eharris the Harris-Foulkes energy of the input density
+    (total_energy_module.f90:341-345).  Fene = Etot - Eentropy."""
     from rsdft_develop_amd import systems
 
     nel = sum(cell.species[k].zps for k in cell.kind)
@@ -230,7 +250,6 @@ def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-
     u = ham.gram_schmidt(u, gs_variant)
     mixer = Pulay(rho, mmix, beta)
     hist = []
-    e_prev = None
     for it in range(1, max_iter + 1):
         u, esp = ham.subspace_diag(u)
         u, esp, res = ham.conjugate_gradient(u, ncg=ncg, mloop=mloop)
@@ -244,14 +263,14 @@ def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-
         vxc_o, ed = ldapz81(rho_out)
         exc = cell.dv * ed.sum()
         etot = eeig - cell.dv * rho_out @ vloc + eh + exc + cell.dv * rho_out @ vi + eew
-        de = abs(etot - e_prev) / cell.natom if e_prev is not None else np.inf
-        hist.append(dict(iter=it, etot=etot, eharris=eharris, de=de, esp=esp.copy()))
+        de = abs(etot - eharris) / cell.natom
+        ent = entropy_fd(occ, ekbt)
+        hist.append(dict(iter=it, etot=etot, eharris=eharris, fene=etot - ent, de=de, esp=esp.copy()))
         if callback:
             callback(hist[-1])
         if de < etol_per_atom or it == max_iter:
-            return dict(converged=de < etol_per_atom, etot=etot, esp=esp, occ=occ, history=hist,
-                        eewald=eew, bands=u, rho=rho_out)
-        e_prev = etot
+            return dict(converged=de < etol_per_atom, etot=etot, fene=etot - ent, esp=esp, occ=occ,
+                        history=hist, eewald=eew, bands=u, rho=rho_out)
         rho = mixer.mix(rho_out)
         rho = rho * nel / (np.sum(rho) * cell.dv)
         vh, eh_in = hartree(cell, rho)

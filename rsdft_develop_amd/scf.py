@@ -26,7 +26,7 @@ class ScfParams:
     mmix: int = 4           # MMIX
     beta: float = 0.3       # BETA
     ekbt: float = 1e-5      # EKBT (Fermi-Dirac temperature, Ha)
-    etol_per_atom: float = 1e-8   # stop when |E_k - E_{k-1}| / natom < etol
+    etol_per_atom: float = 1e-8   # stop when |Etot - Eharris| / natom < etol
     max_iter: int = 200     # DITER
     init_width: float = 1.0  # width (bohr) of the initial atomic Gaussians
     seed: int = 0           # N(0,1) initial bands
@@ -37,6 +37,7 @@ class ScfIteration:
     iter: int
     etot: float
     eharris: float
+    fene: float             # Etot - Eentropy
     de_per_atom: float
     esp: np.ndarray
     times: dict
@@ -47,6 +48,7 @@ class ScfResult:
     converged: bool
     iterations: list
     etot: float
+    fene: float
     esp: np.ndarray
     occ: np.ndarray
     eewald: float
@@ -84,6 +86,16 @@ def fermi_dirac(esp: np.ndarray, nelectron: float, kT: float, nspin: int = 1):
     return w / (1.0 + np.exp(x)), mu
 
 
+def entropy_fd(occ: np.ndarray, kT: float, nspin: int = 1) -> float:
+    """Eentropy = (2/nspin) kT sum_n [-f ln f - (1-f) ln(1-f)] with
+    f = occ * nspin/2 (entropy_fd in fermi_dirac_module.f90:325-350)."""
+    f = np.asarray(occ, dtype=np.float64) * (nspin / 2.0)
+    g = 1.0 - f
+    s = -np.sum(np.where(f > 0, f * np.log(np.where(f > 0, f, 1.0)), 0.0))
+    s -= np.sum(np.where(g > 0, g * np.log(np.where(g > 0, g, 1.0)), 0.0))
+    return float((2.0 / nspin) * kT * s)
+
+
 class SCF:
     """Plain FP64 SCF of one cell on one GPU."""
 
@@ -107,7 +119,7 @@ class SCF:
         c.gram_schmidt(1, cell.nband)   # main.f90:392
         c.mix_init(self.p.mmix, self.p.beta)
 
-    def step(self, it: int, e_prev: float | None) -> ScfIteration:
+    def step(self, it: int) -> ScfIteration:
         c, p = self.ctx, self.p
         t = {}
         t0 = time.perf_counter()
@@ -132,9 +144,11 @@ class SCF:
         eloc, eion = c.density_integrals(gpu.FIELD_RHO_OUT)
         etot = eeig - eloc + eh + exc + eion + self.eewald
         t["eto"] = time.perf_counter() - t0
-        de = abs(etot - e_prev) / self.cell.natom if e_prev is not None else np.inf
+        # energy convergence test of scf_module.f90:335-339
+        de = abs(etot - eharris) / self.cell.natom
+        fene = etot - entropy_fd(occ, p.ekbt)
         self.occ = occ
-        return ScfIteration(it, etot, eharris, de, esp, t)
+        return ScfIteration(it, etot, eharris, fene, de, esp, t)
 
     def update_potential(self) -> float:
         t0 = time.perf_counter()
@@ -147,19 +161,17 @@ class SCF:
 
     def run(self, callback=None) -> ScfResult:
         its = []
-        e_prev = None
         conv = False
         w0 = time.perf_counter()
         for it in range(1, self.p.max_iter + 1):
-            rec = self.step(it, e_prev)
+            rec = self.step(it)
             conv = rec.de_per_atom < self.p.etol_per_atom
             if not conv and it < self.p.max_iter:
                 rec.times["pot"] = self.update_potential()
             its.append(rec)
             if callback:
                 callback(rec)
-            e_prev = rec.etot
             if conv:
                 break
         wall = time.perf_counter() - w0
-        return ScfResult(conv, its, its[-1].etot, its[-1].esp, self.occ, self.eewald, wall)
+        return ScfResult(conv, its, its[-1].etot, its[-1].fene, its[-1].esp, self.occ, self.eewald, wall)
