@@ -163,7 +163,7 @@ int rsg_set_local_gth(rsg_ctx *ctx, int natom, const double *frac,
                       const double *rloc, const double *cloc);
 int rsg_get_scf_constants(rsg_ctx *ctx, double *eewald, double *nelectron);
 int rsg_get_field(rsg_ctx *ctx, int field, double *out);
-/* Only RSG_FIELD_RHO and RSG_FIELD_VLOC can be set. */
+/* Only RSG_FIELD_RHO, RSG_FIELD_RHO_OUT and RSG_FIELD_VLOC can be set. */
 int rsg_set_field(rsg_ctx *ctx, int field, const double *in);
 /* Initial density: atom-centred Gaussians of the given width (bohr)
  * normalised to the electron count.  No reference counterpart: the

@@ -359,8 +359,8 @@ int rsg_get_field(rsg_ctx *ctx, int field, double *out) {
 
 int rsg_set_field(rsg_ctx *ctx, int field, const double *in) {
   if (!ctx || !in) return RSG_ERR_ARG;
-  if (field != RSG_FIELD_RHO && field != RSG_FIELD_VLOC)
-    return ctx->fail(RSG_ERR_ARG, "set_field: only RHO and VLOC can be set");
+  if (field != RSG_FIELD_RHO && field != RSG_FIELD_RHO_OUT && field != RSG_FIELD_VLOC)
+    return ctx->fail(RSG_ERR_ARG, "set_field: only RHO, RHO_OUT and VLOC can be set");
   RSG_HIP(hipSetDevice(ctx->device));
   return rsg::scf_field(ctx, field, const_cast<double *>(in), false);
 }

@@ -25,6 +25,7 @@ struct ScfState {
   int mmix = 0, count = 0;
   double beta = 0.3;
   std::vector<DevBuf> xin, xou;
+  DevBuf res;  // mmix residuals Xou_i - Xin_i, N doubles each
   // GTH local parameters per species
   std::vector<double> zps, rloc, cloc;
   std::vector<double> frac;
@@ -211,6 +212,11 @@ __global__ void k_gauss_rho(GridG g, const double *__restrict__ aa, int natom, c
     s += zat[a] * exp(-(x * x + y * y + z * z) / w2);
   }
   rho[j] = s;
+}
+
+__global__ void k_sub(double *__restrict__ y, const double *__restrict__ a, const double *__restrict__ b, int64_t N) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < N) y[j] = a[j] - b[j];
 }
 
 __global__ void k_fill_value(double *__restrict__ y, double v, int64_t N) {
@@ -587,6 +593,8 @@ int scf_mix_init(rsg_ctx *ctx, int mmix, double beta) {
   for (int k = 0; k < mmix; ++k)
     if (st->xin[size_t(k)].ensure(n8) || st->xou[size_t(k)].ensure(n8))
       return ctx->fail(RSG_ERR_HIP, "mix_init: history allocation failed");
+  if (st->res.ensure(n8 * size_t(mmix)) || st->occ.ensure(size_t(mmix) * mmix * 8 + 64 * 8))
+    return ctx->fail(RSG_ERR_HIP, "mix_init: residual allocation failed");
   st->mmix = mmix;
   st->beta = beta;
   st->count = 0;
@@ -627,26 +635,34 @@ int scf_mix_pulay(rsg_ctx *ctx) {
     // Xin = Xin + beta (Xou - Xin)
     hipLaunchKernelGGL(k_axpby, dim3(nblk(N)), dim3(kT), 0, ctx->stream, xin, xou, st->beta, 1.0 - st->beta, N);
   } else {
-    // residuals R_i = Xou_i - Xin_i held in zbuf-sized scratch one at a time
+    // A(i,j) = <R_i, R_j> over explicit residuals R_i = Xou_i - Xin_i, so the
+    // small residuals near convergence do not cancel against |rho|^2
     std::vector<double> A(size_t(m0) * m0), b(size_t(m0), 1.0);
     const int base = mm - m0;
-    // A(i,j) = sum R_i R_j = sum (Xou_i Xou_j - Xou_i Xin_j - Xin_i Xou_j + Xin_i Xin_j)
+    double *R = st->res.as<double>();
     for (int i = 0; i < m0; ++i)
-      for (int j = 0; j <= i; ++j) {
-        const double *xs[4] = {st->xou[size_t(base + i)].as<double>(), st->xou[size_t(base + i)].as<double>(),
-                               st->xin[size_t(base + i)].as<double>(), st->xin[size_t(base + i)].as<double>()};
-        const double *ys[4] = {st->xou[size_t(base + j)].as<double>(), st->xin[size_t(base + j)].as<double>(),
-                               st->xou[size_t(base + j)].as<double>(), st->xin[size_t(base + j)].as<double>()};
-        if (st->occ.ensure(64 * 8 + size_t(std::max(ctx->mb, 64)) * 8))
-          return ctx->fail(RSG_ERR_HIP, "mix_pulay: scratch allocation failed");
-        double *d = st->occ.as<double>();
-        if ((rc = multidot(ctx, 4, xs, ys, 1, 1.0, d))) return rc;
-        double h[4];
-        RSG_HIP(hipMemcpyAsync(h, d, 32, hipMemcpyDeviceToHost, ctx->stream));
-        RSG_HIP(hipStreamSynchronize(ctx->stream));
-        const double v = h[0] - h[1] - h[2] + h[3];
-        A[size_t(i) + size_t(m0) * j] = A[size_t(j) + size_t(m0) * i] = v;
+      hipLaunchKernelGGL(k_sub, dim3(nblk(N)), dim3(kT), 0, ctx->stream, R + size_t(i) * N,
+                         st->xou[size_t(base + i)].as<double>(), st->xin[size_t(base + i)].as<double>(), N);
+    std::vector<std::pair<int, int>> pairs;
+    for (int i = 0; i < m0; ++i)
+      for (int j = 0; j <= i; ++j) pairs.emplace_back(i, j);
+    double *d = st->occ.as<double>();
+    for (size_t q0 = 0; q0 < pairs.size(); q0 += 6) {
+      const int np = int(std::min<size_t>(6, pairs.size() - q0));
+      const double *xs[6], *ys[6];
+      for (int q = 0; q < np; ++q) {
+        xs[q] = R + size_t(pairs[q0 + q].first) * N;
+        ys[q] = R + size_t(pairs[q0 + q].second) * N;
       }
+      if ((rc = multidot(ctx, np, xs, ys, 1, 1.0, d + q0))) return rc;
+    }
+    std::vector<double> h(pairs.size());
+    RSG_HIP(hipMemcpyAsync(h.data(), d, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    RSG_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t q = 0; q < pairs.size(); ++q) {
+      const int i = pairs[q].first, j = pairs[q].second;
+      A[size_t(i) + size_t(m0) * j] = A[size_t(j) + size_t(m0) * i] = h[q];
+    }
     if (!solve_small(m0, A, b)) return ctx->fail(RSG_ERR_SOLVER, "mix_pulay: singular residual matrix");
     double sb = 0.0;
     for (double v : b) sb += v;
@@ -681,6 +697,7 @@ void scf_release(rsg_ctx *ctx) {
   for (DevBuf *b : {&st->rho, &st->rho_out, &st->vion, &st->vh, &st->vxc, &st->zbuf, &st->work, &st->occ}) b->release();
   for (auto &b : st->xin) b.release();
   for (auto &b : st->xou) b.release();
+  st->res.release();
   delete st;
   ctx->scf = nullptr;
 }
