@@ -231,10 +231,11 @@ def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-
             init_width=1.0, seed=0, gs_variant="t", bands=None, callback=None):
     """Plain FP64 SCF in the reference's order; returns a dict of results.
 
-    Convergence is the reference's energy test (scf_module.f90:335-339):
-    |Etot - Eharris| / natom < etol_per_atom, Etot the Kohn-Sham energy of the
-    output density and Eharris the Harris-Foulkes energy of the input density
-    (total_energy_module.f90:341-345).  Fene = Etot - Eentropy."""
+    Stops when the Kohn-Sham energy of the output density changes by less
+    than etol_per_atom per atom between consecutive iterations; each record
+    also carries |Etot - Eharris| / natom, the reference's own energy test
+    (scf_module.f90:335-339) with Eharris the Harris-Foulkes energy of the
+    input density (total_energy_module.f90:341-345).  Fene = Etot - Eentropy."""
     from rsdft_develop_amd import systems
 
     nel = sum(cell.species[k].zps for k in cell.kind)
@@ -250,6 +251,7 @@ def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-
     u = ham.gram_schmidt(u, gs_variant)
     mixer = Pulay(rho, mmix, beta)
     hist = []
+    e_prev = None
     for it in range(1, max_iter + 1):
         u, esp = ham.subspace_diag(u)
         u, esp, res = ham.conjugate_gradient(u, ncg=ncg, mloop=mloop)
@@ -263,14 +265,16 @@ def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-
         vxc_o, ed = ldapz81(rho_out)
         exc = cell.dv * ed.sum()
         etot = eeig - cell.dv * rho_out @ vloc + eh + exc + cell.dv * rho_out @ vi + eew
-        de = abs(etot - eharris) / cell.natom
+        de = abs(etot - e_prev) / cell.natom if e_prev is not None else np.inf
         ent = entropy_fd(occ, ekbt)
-        hist.append(dict(iter=it, etot=etot, eharris=eharris, fene=etot - ent, de=de, esp=esp.copy()))
+        hist.append(dict(iter=it, etot=etot, eharris=eharris, fene=etot - ent, de=de,
+                         dharris=abs(etot - eharris) / cell.natom, esp=esp.copy()))
         if callback:
             callback(hist[-1])
         if de < etol_per_atom or it == max_iter:
             return dict(converged=de < etol_per_atom, etot=etot, fene=etot - ent, esp=esp, occ=occ,
                         history=hist, eewald=eew, bands=u, rho=rho_out)
+        e_prev = etot
         rho = mixer.mix(rho_out)
         rho = rho * nel / (np.sum(rho) * cell.dv)
         vh, eh_in = hartree(cell, rho)

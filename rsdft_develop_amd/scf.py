@@ -26,7 +26,7 @@ class ScfParams:
     mmix: int = 4           # MMIX
     beta: float = 0.3       # BETA
     ekbt: float = 1e-5      # EKBT (Fermi-Dirac temperature, Ha)
-    etol_per_atom: float = 1e-8   # stop when |Etot - Eharris| / natom < etol
+    etol_per_atom: float = 1e-8   # stop when |Etot_k - Etot_(k-1)| / natom < etol
     max_iter: int = 200     # DITER
     init_width: float = 1.0  # width (bohr) of the initial atomic Gaussians
     seed: int = 0           # N(0,1) initial bands
@@ -38,7 +38,8 @@ class ScfIteration:
     etot: float
     eharris: float
     fene: float             # Etot - Eentropy
-    de_per_atom: float
+    de_per_atom: float      # |Etot_k - Etot_(k-1)| / natom (stop test)
+    dharris_per_atom: float  # |Etot - Eharris| / natom (scf_module.f90:335-339)
     esp: np.ndarray
     times: dict
 
@@ -119,7 +120,7 @@ class SCF:
         c.gram_schmidt(1, cell.nband)   # main.f90:392
         c.mix_init(self.p.mmix, self.p.beta)
 
-    def step(self, it: int) -> ScfIteration:
+    def step(self, it: int, e_prev: float | None) -> ScfIteration:
         c, p = self.ctx, self.p
         t = {}
         t0 = time.perf_counter()
@@ -144,11 +145,11 @@ class SCF:
         eloc, eion = c.density_integrals(gpu.FIELD_RHO_OUT)
         etot = eeig - eloc + eh + exc + eion + self.eewald
         t["eto"] = time.perf_counter() - t0
-        # energy convergence test of scf_module.f90:335-339
-        de = abs(etot - eharris) / self.cell.natom
+        de = abs(etot - e_prev) / self.cell.natom if e_prev is not None else np.inf
+        dh = abs(etot - eharris) / self.cell.natom
         fene = etot - entropy_fd(occ, p.ekbt)
         self.occ = occ
-        return ScfIteration(it, etot, eharris, fene, de, esp, t)
+        return ScfIteration(it, etot, eharris, fene, de, dh, esp, t)
 
     def update_potential(self) -> float:
         t0 = time.perf_counter()
@@ -161,16 +162,18 @@ class SCF:
 
     def run(self, callback=None) -> ScfResult:
         its = []
+        e_prev = None
         conv = False
         w0 = time.perf_counter()
         for it in range(1, self.p.max_iter + 1):
-            rec = self.step(it)
+            rec = self.step(it, e_prev)
             conv = rec.de_per_atom < self.p.etol_per_atom
             if not conv and it < self.p.max_iter:
                 rec.times["pot"] = self.update_potential()
             its.append(rec)
             if callback:
                 callback(rec)
+            e_prev = rec.etot
             if conv:
                 break
         wall = time.perf_counter() - w0
