@@ -59,21 +59,31 @@ def build_library(verbose: bool = False) -> str:
 
 
 def build_fortran(verbose: bool = False) -> str | None:
+    """Fortran drop-in module (librsdft_gpu_f.so) and the Fortran SCF driver
+    (lib/rsdft_scf), built with AMD flang when it is installed."""
     fc = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
     src = os.path.join(HERE, "fortran", "rsdft_gpu_mod.f90")
+    prog = os.path.join(HERE, "fortran", "rsdft_scf.f90")
     if fc is None or not os.path.exists(src):
         return None
     out = os.path.join(LIBDIR, "librsdft_gpu_f.so")
+    exe = os.path.join(LIBDIR, "rsdft_scf")
     moddir = os.path.join(BUILD, "mod")
     os.makedirs(moddir, exist_ok=True)
-    cmd = [fc, "-O2", "-fPIC", "-shared", "-module-dir", moddir, "-o", out + ".tmp", src,
-           f"-L{LIBDIR}", "-lrsdft_gpu", f"-Wl,-rpath,{LIBDIR}"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    steps = [
+        [fc, "-O2", "-fPIC", "-shared", "-module-dir", moddir, "-o", out + ".tmp", src,
+         f"-L{LIBDIR}", "-lrsdft_gpu", f"-Wl,-rpath,{LIBDIR}"],
+        [fc, "-O2", "-I", moddir, "-o", exe + ".tmp", prog, out + ".tmp",
+         f"-L{LIBDIR}", "-lrsdft_gpu", f"-Wl,-rpath,{LIBDIR}"],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(out + ".tmp", out)
+    os.replace(exe + ".tmp", exe)
     if verbose:
-        print(f"built {out}")
+        print(f"built {out} and {exe}")
     return out
 
 
