@@ -77,6 +77,15 @@ int rsg_build_gth_projectors(rsg_ctx *ctx, int natom, const double *frac,
                              const int *kind, int nkind, const int *norb,
                              const int *lo, const int *no, const double *rps,
                              const double *hnl);
+/* How hpsi applies V_NL: 0 = automatic (analytic when available), 1 =
+ * the tables, 2 = analytic (RSG_ERR_STATE at hpsi time if unavailable).
+ * The analytic form evaluates GTH (l <= 1, n = 1) projectors on the fly
+ * from per-atom separable Gaussian factors instead of streaming uVk; it is
+ * built by rsg_build_gth_projectors for orthorhombic cells longer than
+ * twice the largest cutoff, and dropped by rsg_set_projectors.  No
+ * reference counterpart (an implementation choice of this library). */
+int rsg_set_nonlocal_mode(rsg_ctx *ctx, int mode);
+int rsg_get_nonlocal_mode(rsg_ctx *ctx, int *mode, int *analytic_available);
 int rsg_get_projector_sizes(rsg_ctx *ctx, int *nzlma, int *maxmjj,
                             int64_t *nentries);
 int rsg_get_projectors(rsg_ctx *ctx, int *mjj, int *jjp, double *uvk,

@@ -47,6 +47,9 @@ int rsg_destroy(rsg_ctx *ctx) {
                     &ctx->hostio, &ctx->dead};
   for (DevBuf *b : bufs) b->release();
   ctx->uvpart.release();
+  ctx->cproj.release();
+  ctx->ana64.release_all();
+  ctx->ana32.release_all();
   ctx->nl64.release_all();
   ctx->nl32.release_all();
   rsg::solver_release(ctx);
@@ -116,6 +119,7 @@ int rsg_set_grid(rsg_ctx *ctx, int n1, int n2, int n3, const double *aa, int md)
   ctx->have_grid = true;
   ctx->have_vloc = false;
   ctx->have_nl = false;
+  ctx->have_ana = false;
   ctx->tables = rsg::HostTables();
   ctx->mb = 0;
   return RSG_OK;
@@ -170,6 +174,7 @@ int rsg_set_projectors(rsg_ctx *ctx, int nzlma, int maxmjj, const int *mjj, cons
   RSG_HIP(hipSetDevice(ctx->device));
   ctx->tables = std::move(t);
   ctx->have_nl = false;
+  ctx->have_ana = false;  // caller tables carry no analytic form
   return rsg::upload_nonlocal(ctx);
 }
 
@@ -186,7 +191,27 @@ int rsg_build_gth_projectors(rsg_ctx *ctx, int natom, const double *frac, const 
   if (rc) return ctx->fail(rc, "build_gth: invalid species parameters");
   ctx->tables = std::move(t);
   ctx->have_nl = false;
-  return rsg::upload_nonlocal(ctx);
+  ctx->have_ana = false;
+  if ((rc = rsg::upload_nonlocal(ctx))) return rc;
+  rsg::HostAnalytic h;
+  rc = rsg::build_gth_analytic(ctx, natom, frac, kind, nkind, norb, lo, no, rps, hnl, h);
+  if (rc == RSG_ERR_UNSUPPORTED) return RSG_OK;  // the table path covers this set
+  if (rc) return ctx->fail(rc, "build_gth: invalid atom kinds");
+  return rsg::upload_analytic(ctx, h);
+}
+
+int rsg_set_nonlocal_mode(rsg_ctx *ctx, int mode) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (mode < 0 || mode > 2) return ctx->fail(RSG_ERR_ARG, "set_nonlocal_mode: mode must be 0, 1 or 2");
+  ctx->nl_mode = mode;
+  return RSG_OK;
+}
+
+int rsg_get_nonlocal_mode(rsg_ctx *ctx, int *mode, int *analytic_available) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (mode) *mode = ctx->nl_mode;
+  if (analytic_available) *analytic_available = ctx->have_ana ? 1 : 0;
+  return RSG_OK;
 }
 
 int rsg_get_projector_sizes(rsg_ctx *ctx, int *nzlma, int *maxmjj, int64_t *nent) {

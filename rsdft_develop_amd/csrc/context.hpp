@@ -54,6 +54,41 @@ template <typename T> struct NonlocalDev {
   }
 };
 
+// GTH projectors of the form (l = 0, n = 1) and (l = 1, n = 1) evaluated on
+// the fly instead of streamed from tables.  For an atom at R and a grid
+// point at displacement d (the image nearest the atom; unique because every
+// cell length exceeds twice the largest cutoff), the four projector values
+// are
+//   s:       amp[0] * es_x(d_x) es_y(d_y) es_z(d_z)
+//   p_x,y,z: amp[1] * ep_x ep_y ep_z * d_{x,y,z}
+// with e?_axis = exp(-d_axis^2 / (2 r_l^2)) tabulated per atom and axis
+// over the grid coordinates.  This equals v_l(r) Y_lm(r^) of the tables up
+// to rounding and up to an orthogonal change of basis inside l = 1, which
+// leaves sum_m |p_m><p_m| unchanged.  Which points belong to which sphere
+// is decided on the host with the table builder's own arithmetic and
+// stored as flags.
+template <typename T> struct AnalyticDev {
+  int ngroup = 0, nseg = 0, S = 0;
+  DevBuf es, ep, dd;    // T[ngroup * S]: axis x at [0, n1), y at [n1, n1+n2), z after
+  DevBuf amp, scale;    // T[ngroup * 4]: amplitudes (s, p, p, p), iuV * dV
+  DevBuf pts;           // uint32 per group point: x | y << 10 | z << 20 | s << 30 | p << 31
+  DevBuf seg_beg, seg_end, seg_grp;  // int[nseg]
+  DevBuf grp_seg_off;   // int[ngroup + 1]
+  DevBuf hits;          // uint64 per grid point: 4 x uint16 (group + 1 | s << 14 | p << 15)
+  void release_all() {
+    for (DevBuf *b : {&es, &ep, &dd, &amp, &scale, &pts, &seg_beg, &seg_end, &seg_grp, &grp_seg_off, &hits})
+      b->release();
+  }
+};
+
+struct HostAnalytic {
+  int ngroup = 0, S = 0;
+  std::vector<double> es, ep, dd, amp, scale;
+  std::vector<uint32_t> pts;
+  std::vector<int> seg_beg, seg_end, seg_grp, grp_seg_off{0};
+  std::vector<uint64_t> hits;
+};
+
 // Launch shape of the hpsi kernels for one grid and precision.
 struct HpsiShape {
   int ry = 2;     // rows per thread (y register blocking)
@@ -99,6 +134,11 @@ struct rsg_ctx {
   rsg::NonlocalDev<double> nl64;
   rsg::NonlocalDev<float> nl32;
   bool have_nl = false;
+  rsg::AnalyticDev<double> ana64;
+  rsg::AnalyticDev<float> ana32;
+  bool have_ana = false;  // analytic projector description available
+  int nl_mode = 0;        // 0 auto, 1 tables, 2 analytic
+  rsg::DevBuf cproj;      // analytic path: projections C[(g*4 + l) * chunk + b]
   rsg::DevBuf uvunk;    // nzlma x chunk projections uVunk, [lma][band]
   rsg::DevBuf uvpart;   // per-segment partial projections
   rsg::HpsiShape shape64, shape32;  // zero fields: chosen from the grid
@@ -144,6 +184,11 @@ int build_gth_tables(const rsg_ctx *ctx, int natom, const double *frac,
                      const int *lo, const int *no, const double *rps,
                      const double *hnl, HostTables &out);
 int upload_nonlocal(rsg_ctx *ctx);
+int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac,
+                       const int *kind, int nkind, const int *norb,
+                       const int *lo, const int *no, const double *rps,
+                       const double *hnl, HostAnalytic &out);
+int upload_analytic(rsg_ctx *ctx, const HostAnalytic &h);
 }  // namespace rsg
 
 // hpsi.hip

@@ -287,6 +287,134 @@ __global__ __launch_bounds__(256) void k_nl_scatter(T *__restrict__ hpsi, int64_
   }
 }
 
+// ---- analytic GTH projectors (l <= 1, n = 1), evaluated on the fly ----
+//
+// Projector values at a sphere point of group (atom) g with grid coordinates
+// (x, y, z): s = amp_s * es_x es_y es_z, p_m = amp_p * ep_x ep_y ep_z * d_m,
+// from per-atom 1-D tables (see AnalyticDev).  The sphere flags come from
+// the host.  Compared with the table path this streams 4 bytes per sphere
+// point instead of 36.
+
+template <typename T>
+__device__ inline void ana_values(const T *__restrict__ es, const T *__restrict__ ep, const T *__restrict__ dd,
+                                  int64_t gS, int oy, int oz, int x, int y, int z, bool fs, bool fp, T amp_s,
+                                  T amp_p, T &cs, T &cx, T &cy, T &cz) {
+  const int64_t ix = gS + x, iy = gS + oy + y, iz = gS + oz + z;
+  const T vs = fs ? amp_s * es[ix] * es[iy] * es[iz] : T(0);
+  const T vp = fp ? amp_p * ep[ix] * ep[iy] * ep[iz] : T(0);
+  cs = vs;
+  cx = vp * dd[ix];
+  cy = vp * dd[iy];
+  cz = vp * dd[iz];
+}
+
+template <typename T, int BB>
+__global__ __launch_bounds__(kGatherThreads) void k_proj_a(
+    const T *__restrict__ psi, int64_t ld, int nbc, int n1, int n2, const int *__restrict__ seg_beg,
+    const int *__restrict__ seg_end, const int *__restrict__ seg_grp, const uint32_t *__restrict__ pts,
+    const T *__restrict__ es, const T *__restrict__ ep, const T *__restrict__ dd, const T *__restrict__ amp,
+    int S, T *__restrict__ part) {
+  __shared__ T red[kGatherThreads / 64][4 * BB];
+  const int s = blockIdx.x;
+  const int b0 = blockIdx.y * BB;
+  const int nbb = min(BB, nbc - b0);
+  const int beg = seg_beg[s], end = seg_end[s], g = seg_grp[s];
+  const int64_t gS = int64_t(g) * S;
+  const T amp_s = amp[4 * g], amp_p = amp[4 * g + 1];
+  const int64_t n12 = int64_t(n1) * n2;
+  T acc[4][BB];
+#pragma unroll
+  for (int l = 0; l < 4; ++l)
+#pragma unroll
+    for (int b = 0; b < BB; ++b) acc[l][b] = T(0);
+  const T *__restrict__ w = psi + int64_t(b0) * ld;
+  for (int i = beg + threadIdx.x; i < end; i += kGatherThreads) {
+    const uint32_t wd = pts[i];
+    const int x = int(wd & 1023u), y = int((wd >> 10) & 1023u), z = int((wd >> 20) & 1023u);
+    T c0, c1, c2, c3;
+    ana_values<T>(es, ep, dd, gS, n1, n1 + n2, x, y, z, (wd >> 30) & 1u, wd >> 31, amp_s, amp_p, c0, c1, c2, c3);
+    const int64_t pt = x + int64_t(y) * n1 + int64_t(z) * n12;
+#pragma unroll
+    for (int b = 0; b < BB; ++b) {
+      if (b < nbb) {
+        const T v = w[pt + int64_t(b) * ld];
+        acc[0][b] += c0 * v;
+        acc[1][b] += c1 * v;
+        acc[2][b] += c2 * v;
+        acc[3][b] += c3 * v;
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int l = 0; l < 4; ++l)
+#pragma unroll
+    for (int b = 0; b < BB; ++b) {
+      const T v = wave_sum(acc[l][b]);
+      if (lane == 0) red[wid][l * BB + b] = v;
+    }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < 4 * BB) {
+    const int l = t / BB, b = t % BB;
+    if (b < nbb) {
+      T v = T(0);
+#pragma unroll
+      for (int k = 0; k < kGatherThreads / 64; ++k) v += red[k][t];
+      part[(int64_t(s) * 4 + l) * nbc + b0 + b] = v;
+    }
+  }
+}
+
+// C[(g*4 + l)*nbc + b] = iuV dV * sum over the group's segments
+template <typename T>
+__global__ void k_proj_reduce(const T *__restrict__ part, int nbc, int ngroup, const int *__restrict__ grp_seg_off,
+                              const T *__restrict__ scale, T *__restrict__ cproj) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= int64_t(ngroup) * 4 * nbc) return;
+  const int gl = int(t / nbc), b = int(t - int64_t(gl) * nbc);
+  const int g = gl >> 2, l = gl & 3;
+  T v = T(0);
+  for (int s = grp_seg_off[g]; s < grp_seg_off[g + 1]; ++s) v += part[(int64_t(s) * 4 + l) * nbc + b];
+  cproj[t] = scale[gl] * v;
+}
+
+// H psi(j, b) (=|+=) sum over the spheres holding j of
+//   s(j) C(g,0,b) + p_x(j) C(g,1,b) + p_y(j) C(g,2,b) + p_z(j) C(g,3,b)
+// One thread per grid point and a tile of BT bands: the projector values of
+// a point are formed once per tile.
+template <typename T, int BT, bool ACC>
+__global__ __launch_bounds__(256) void k_nl_apply(T *__restrict__ hpsi, int64_t ld, int nbc, int64_t N, int n1,
+                                                  int n2, const uint64_t *__restrict__ hits,
+                                                  const T *__restrict__ es, const T *__restrict__ ep,
+                                                  const T *__restrict__ dd, const T *__restrict__ amp, int S,
+                                                  const T *__restrict__ cproj) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const int b0 = blockIdx.y * BT;
+  const int nbt = min(BT, nbc - b0);
+  const int x = int(j % n1), y = int((j / n1) % n2), z = int(j / (int64_t(n1) * n2));
+  T acc[BT];
+#pragma unroll
+  for (int b = 0; b < BT; ++b) acc[b] = (ACC && b < nbt) ? hpsi[j + int64_t(b0 + b) * ld] : T(0);
+  uint64_t hw = hits[j];
+  while (hw) {
+    const uint32_t e = uint32_t(hw & 0xffffu);
+    hw >>= 16;
+    const int g = int(e & 0x3fffu) - 1;
+    T cs, cx, cy, cz;
+    ana_values<T>(es, ep, dd, int64_t(g) * S, n1, n1 + n2, x, y, z, (e >> 14) & 1u, (e >> 15) & 1u, amp[4 * g],
+                  amp[4 * g + 1], cs, cx, cy, cz);
+    const T *__restrict__ c = cproj + int64_t(g) * 4 * nbc + b0;
+#pragma unroll
+    for (int b = 0; b < BT; ++b)
+      if (b < nbt) acc[b] += cs * c[b] + cx * c[nbc + b] + cy * c[2 * nbc + b] + cz * c[3 * nbc + b];
+  }
+#pragma unroll
+  for (int b = 0; b < BT; ++b)
+    if (b < nbt) hpsi[j + int64_t(b0 + b) * ld] = acc[b];
+}
+
 template <typename T> StencilCoef<T> make_coef(const rsg_ctx *ctx) {
   StencilCoef<T> cf{};
   for (int a = 0; a < 3; ++a)
@@ -298,6 +426,9 @@ template <typename T> StencilCoef<T> make_coef(const rsg_ctx *ctx) {
 template <typename T> NonlocalDev<T> &nl_of(rsg_ctx *ctx);
 template <> NonlocalDev<double> &nl_of<double>(rsg_ctx *ctx) { return ctx->nl64; }
 template <> NonlocalDev<float> &nl_of<float>(rsg_ctx *ctx) { return ctx->nl32; }
+template <typename T> AnalyticDev<T> &ana_of(rsg_ctx *ctx);
+template <> AnalyticDev<double> &ana_of<double>(rsg_ctx *ctx) { return ctx->ana64; }
+template <> AnalyticDev<float> &ana_of<float>(rsg_ctx *ctx) { return ctx->ana32; }
 template <typename T> const T *vloc_of(rsg_ctx *ctx);
 template <> const double *vloc_of<double>(rsg_ctx *ctx) { return ctx->vloc64.as<double>(); }
 template <> const float *vloc_of<float>(rsg_ctx *ctx) { return ctx->vloc32.as<float>(); }
@@ -423,6 +554,14 @@ HpsiShape resolve_shape(const rsg_ctx *ctx, const HpsiShape &in, size_t s, int n
 }
 
 template <typename T, int BT>
+void launch_apply_t(rsg_ctx *ctx, AnalyticDev<T> &an, T *hpsi, int64_t ld, int nbc, const T *cproj) {
+  dim3 grid(unsigned((ctx->N + 255) / 256), (nbc + BT - 1) / BT);
+  hipLaunchKernelGGL((k_nl_apply<T, BT, false>), grid, dim3(256), 0, ctx->stream, hpsi, ld, nbc, ctx->N, ctx->n[0],
+                     ctx->n[1], an.hits.template as<uint64_t>(), an.es.template as<T>(), an.ep.template as<T>(),
+                     an.dd.template as<T>(), an.amp.template as<T>(), an.S, cproj);
+}
+
+template <typename T, int BT>
 void launch_scatter_t(rsg_ctx *ctx, NonlocalDev<T> &nl, T *hpsi, int64_t ld, int nbc, const T *uv) {
   dim3 grid(unsigned((ctx->N + 255) / 256), (nbc + BT - 1) / BT);
   hipLaunchKernelGGL((k_nl_scatter<T, BT>), grid, dim3(256), 0, ctx->stream, hpsi, ld, nbc, ctx->N,
@@ -443,7 +582,35 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb) {
   if (!ctx->have_nl) {
     int rc = launch_stencil<T>(ctx, psi, hpsi, ld, nb, sh, 0);
     if (rc) return rc;
+  } else if (ctx->nl_mode != 1 && ctx->have_ana) {
+    AnalyticDev<T> &an = ana_of<T>(ctx);
+    constexpr int BB = 8;
+    if (ctx->cproj.ensure(size_t(an.ngroup) * 4 * sh.chunk * sizeof(T)) != hipSuccess ||
+        ctx->uvpart.ensure(size_t(an.nseg) * 4 * sh.chunk * sizeof(T)) != hipSuccess)
+      return ctx->fail(RSG_ERR_HIP, "hpsi: projection scratch allocation failed");
+    T *cp = ctx->cproj.as<T>(), *part = ctx->uvpart.as<T>();
+    for (int c0 = 0; c0 < nb; c0 += sh.chunk) {
+      const int nbc = std::min(sh.chunk, nb - c0);
+      const T *pc = psi + int64_t(c0) * ld;
+      T *hc = hpsi + int64_t(c0) * ld;
+      hipLaunchKernelGGL((k_proj_a<T, BB>), dim3(an.nseg, (nbc + BB - 1) / BB), dim3(kGatherThreads), 0,
+                         ctx->stream, pc, ld, nbc, ctx->n[0], ctx->n[1], an.seg_beg.template as<int>(),
+                         an.seg_end.template as<int>(), an.seg_grp.template as<int>(),
+                         an.pts.template as<uint32_t>(), an.es.template as<T>(), an.ep.template as<T>(),
+                         an.dd.template as<T>(), an.amp.template as<T>(), an.S, part);
+      const int64_t nred = int64_t(an.ngroup) * 4 * nbc;
+      hipLaunchKernelGGL((k_proj_reduce<T>), dim3(unsigned((nred + 255) / 256)), dim3(256), 0, ctx->stream, part,
+                         nbc, an.ngroup, an.grp_seg_off.template as<int>(), an.scale.template as<T>(), cp);
+      switch (sh.bt) {
+        case 4: launch_apply_t<T, 4>(ctx, an, hc, ld, nbc, cp); break;
+        case 16: launch_apply_t<T, 16>(ctx, an, hc, ld, nbc, cp); break;
+        default: launch_apply_t<T, 8>(ctx, an, hc, ld, nbc, cp); break;
+      }
+      int rc = launch_stencil<T>(ctx, pc, hc, ld, nbc, sh, 1);
+      if (rc) return rc;
+    }
   } else {
+    if (ctx->nl_mode == 2) return ctx->fail(RSG_ERR_STATE, "hpsi: analytic projectors requested but not available");
     NonlocalDev<T> &nl = nl_of<T>(ctx);
     const int nz = ctx->tables.nzlma;
     constexpr int BB = 8;

@@ -386,4 +386,165 @@ int upload_nonlocal(rsg_ctx *ctx) {
   return RSG_OK;
 }
 
+// Analytic description of GTH sets made only of (l = 0, n = 1) and
+// (l = 1, n = 1) projectors, at most one of each per species.  Returns
+// RSG_ERR_UNSUPPORTED (not an error for the caller) when the set, the cell
+// or the grid does not fit that form; the table path then stays in use.
+int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind,
+                       const int *norb, const int *lo, const int *no, const double *rps, const double *hnl,
+                       HostAnalytic &out) {
+  out = HostAnalytic();
+  const double *aa = ctx->aa;
+  if (aa[1] != 0.0 || aa[2] != 0.0 || aa[3] != 0.0 || aa[5] != 0.0 || aa[6] != 0.0 || aa[7] != 0.0)
+    return RSG_ERR_UNSUPPORTED;
+  const int *ng = ctx->n;
+  if (ng[0] > 1023 || ng[1] > 1023 || ng[2] > 1023 || natom > 16382) return RSG_ERR_UNSUPPORTED;
+  // per species: orbital index of s and p (or -1), cutoffs, amplitudes
+  std::vector<int> qs(size_t(nkind), -1), qp(size_t(nkind), -1);
+  std::vector<double> rcs(size_t(nkind), 0.0), rcp(size_t(nkind), 0.0);
+  double rmax = 0.0;
+  for (int ik = 0; ik < nkind; ++ik) {
+    for (int io = 0; io < norb[ik]; ++io) {
+      const int q = ik * 4 + io;
+      if (no[q] != 1 || lo[q] > 1) return RSG_ERR_UNSUPPORTED;
+      int &slot = lo[q] == 0 ? qs[size_t(ik)] : qp[size_t(ik)];
+      if (slot >= 0) return RSG_ERR_UNSUPPORTED;
+      slot = q;
+      const double rc = gth_cutoff(lo[q], 1, rps[q]);
+      (lo[q] == 0 ? rcs : rcp)[size_t(ik)] = rc;
+      rmax = std::max(rmax, rc);
+    }
+  }
+  for (int t = 0; t < 3; ++t)
+    if (!(aa[4 * t] > 2.0 * rmax + 2.0 * ctx->hgrid[t])) return RSG_ERR_UNSUPPORTED;
+  const int S = ng[0] + ng[1] + ng[2];
+  const int axoff[3] = {0, ng[0], ng[0] + ng[1]};
+  const double c[3] = {1.0 / ng[0], 1.0 / ng[1], 1.0 / ng[2]};
+  const int64_t n12 = int64_t(ng[0]) * ng[1];
+  std::vector<int> nhit(size_t(ctx->N), 0);
+  out.hits.assign(size_t(ctx->N), 0);
+  out.S = S;
+  int g = 0;
+  for (int a = 0; a < natom; ++a) {
+    const int ik = kind[a];
+    if (ik < 0 || ik >= nkind) return RSG_ERR_ARG;
+    const int q0 = qs[size_t(ik)], q1 = qp[size_t(ik)];
+    if (q0 < 0 && q1 < 0) continue;
+    const double *fa = frac + 3 * a;
+    // same arithmetic as build_gth_tables, so sphere membership agrees
+    const double R[3] = {aa[0] * fa[0] + aa[3] * fa[1] + aa[6] * fa[2],
+                         aa[1] * fa[0] + aa[4] * fa[1] + aa[7] * fa[2],
+                         aa[2] * fa[0] + aa[5] * fa[1] + aa[8] * fa[2]};
+    const int ic[3] = {nint_away(fa[0] * ng[0]), nint_away(fa[1] * ng[1]), nint_away(fa[2] * ng[2])};
+    const double rs = q0 >= 0 ? rcs[size_t(ik)] : 0.0, rp = q1 >= 0 ? rcp[size_t(ik)] : 0.0;
+    const double rc = std::max(rs, rp);
+    const double c2s = q0 >= 0 ? 0.5 / (rps[q0] * rps[q0]) : 0.0;
+    const double c2p = q1 >= 0 ? 0.5 / (rps[q1] * rps[q1]) : 0.0;
+    const double pi4 = 4.0 * kPi;
+    const double amp_s = q0 >= 0 ? std::sqrt(std::fabs(hnl[q0])) * std::sqrt(2.0) /
+                                       (std::pow(rps[q0], 1.5) * std::sqrt(gamma_half(0, 1))) / std::sqrt(pi4)
+                                 : 0.0;
+    const double amp_p = q1 >= 0 ? std::sqrt(std::fabs(hnl[q1])) * std::sqrt(2.0) /
+                                       (std::pow(rps[q1], 2.5) * std::sqrt(gamma_half(1, 1))) *
+                                       std::sqrt(3.0 / pi4)
+                                 : 0.0;
+    out.amp.insert(out.amp.end(), {amp_s, amp_p, amp_p, amp_p});
+    const double sg_s = q0 >= 0 && hnl[q0] < 0.0 ? -1.0 : 1.0, sg_p = q1 >= 0 && hnl[q1] < 0.0 ? -1.0 : 1.0;
+    out.scale.insert(out.scale.end(), {q0 >= 0 ? sg_s * ctx->dv : 0.0, q1 >= 0 ? sg_p * ctx->dv : 0.0,
+                                       q1 >= 0 ? sg_p * ctx->dv : 0.0, q1 >= 0 ? sg_p * ctx->dv : 0.0});
+    // 1-D tables over each axis at the image nearest the atom
+    const size_t base = out.es.size();
+    out.es.resize(base + size_t(S));
+    out.ep.resize(base + size_t(S));
+    out.dd.resize(base + size_t(S));
+    for (int t = 0; t < 3; ++t)
+      for (int i = 0; i < ng[t]; ++i) {
+        int id = i;
+        while (id - ic[t] > ng[t] / 2) id -= ng[t];
+        while (ic[t] - id > ng[t] / 2) id += ng[t];
+        const double d = aa[4 * t] * (id * c[t]) - R[t];
+        out.dd[base + size_t(axoff[t] + i)] = d;
+        out.es[base + size_t(axoff[t] + i)] = std::exp(-d * d * c2s);
+        out.ep[base + size_t(axoff[t] + i)] = std::exp(-d * d * c2p);
+      }
+    // sphere points, in grid order
+    int mm[3];
+    for (int t = 0; t < 3; ++t) mm[t] = int(std::ceil(rc / ctx->hgrid[t])) + 2;
+    std::vector<std::pair<int64_t, uint32_t>> pts;
+    for (int o3 = -mm[2]; o3 <= mm[2]; ++o3)
+      for (int o2 = -mm[1]; o2 <= mm[1]; ++o2)
+        for (int o1 = -mm[0]; o1 <= mm[0]; ++o1) {
+          const int id[3] = {ic[0] + o1, ic[1] + o2, ic[2] + o3};
+          const double d1 = id[0] * c[0], d2 = id[1] * c[1], d3 = id[2] * c[2];
+          const double x = aa[0] * d1 + aa[3] * d2 + aa[6] * d3 - R[0];
+          const double y = aa[1] * d1 + aa[4] * d2 + aa[7] * d3 - R[1];
+          const double z = aa[2] * d1 + aa[5] * d2 + aa[8] * d3 - R[2];
+          const double r2 = x * x + y * y + z * z;
+          const bool in_s = q0 >= 0 && r2 <= rs * rs + 1e-10;
+          const bool in_p = q1 >= 0 && r2 <= rp * rp + 1e-10;
+          if (!in_s && !in_p) continue;
+          int w[3];
+          for (int t = 0; t < 3; ++t) w[t] = ((id[t] % ng[t]) + ng[t]) % ng[t];
+          const int64_t lin = w[0] + int64_t(w[1]) * ng[0] + int64_t(w[2]) * n12;
+          const uint32_t word = uint32_t(w[0]) | (uint32_t(w[1]) << 10) | (uint32_t(w[2]) << 20) |
+                                (uint32_t(in_s) << 30) | (uint32_t(in_p) << 31);
+          pts.emplace_back(lin, word);
+          int &k = nhit[size_t(lin)];
+          if (k >= 4) return RSG_ERR_UNSUPPORTED;  // more than four spheres at one point
+          const uint64_t e = uint64_t(g + 1) | (uint64_t(in_s) << 14) | (uint64_t(in_p) << 15);
+          out.hits[size_t(lin)] |= e << (16 * k);
+          ++k;
+        }
+    std::sort(pts.begin(), pts.end());
+    const int pbase = int(out.pts.size());
+    for (auto &pw : pts) out.pts.push_back(pw.second);
+    for (int b = 0; b < int(pts.size()); b += kSegLen) {
+      out.seg_beg.push_back(pbase + b);
+      out.seg_end.push_back(pbase + std::min(int(pts.size()), b + kSegLen));
+      out.seg_grp.push_back(g);
+    }
+    out.grp_seg_off.push_back(int(out.seg_beg.size()));
+    ++g;
+  }
+  out.ngroup = g;
+  return g > 0 ? RSG_OK : RSG_ERR_UNSUPPORTED;
+}
+
+namespace {
+
+template <typename T> int upload_analytic_t(rsg_ctx *ctx, AnalyticDev<T> &d, const HostAnalytic &h) {
+  auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
+    if (b.ensure(bytes) != hipSuccess) return RSG_ERR_HIP;
+    if (bytes && hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return RSG_ERR_HIP;
+    return RSG_OK;
+  };
+  auto conv = [](const std::vector<double> &v) { return std::vector<T>(v.begin(), v.end()); };
+  const std::vector<T> es = conv(h.es), ep = conv(h.ep), dd = conv(h.dd), amp = conv(h.amp), sc = conv(h.scale);
+  d.ngroup = h.ngroup;
+  d.nseg = int(h.seg_beg.size());
+  d.S = h.S;
+  int rc = 0;
+  rc |= up(d.es, es.data(), es.size() * sizeof(T));
+  rc |= up(d.ep, ep.data(), ep.size() * sizeof(T));
+  rc |= up(d.dd, dd.data(), dd.size() * sizeof(T));
+  rc |= up(d.amp, amp.data(), amp.size() * sizeof(T));
+  rc |= up(d.scale, sc.data(), sc.size() * sizeof(T));
+  rc |= up(d.pts, h.pts.data(), h.pts.size() * sizeof(uint32_t));
+  rc |= up(d.seg_beg, h.seg_beg.data(), h.seg_beg.size() * sizeof(int));
+  rc |= up(d.seg_end, h.seg_end.data(), h.seg_end.size() * sizeof(int));
+  rc |= up(d.seg_grp, h.seg_grp.data(), h.seg_grp.size() * sizeof(int));
+  rc |= up(d.grp_seg_off, h.grp_seg_off.data(), h.grp_seg_off.size() * sizeof(int));
+  rc |= up(d.hits, h.hits.data(), h.hits.size() * sizeof(uint64_t));
+  return rc ? ctx->fail(RSG_ERR_HIP, "upload of analytic projector data failed") : RSG_OK;
+}
+
+}  // namespace
+
+int upload_analytic(rsg_ctx *ctx, const HostAnalytic &h) {
+  int rc = upload_analytic_t<double>(ctx, ctx->ana64, h);
+  if (!rc) rc = upload_analytic_t<float>(ctx, ctx->ana32, h);
+  ctx->have_ana = rc == RSG_OK;
+  return rc;
+}
+
 }  // namespace rsg

@@ -65,6 +65,8 @@ def load_library():
         "rsg_set_vloc": (_c_int, [_vp, _vp]),
         "rsg_set_projectors": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
         "rsg_build_gth_projectors": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
+        "rsg_set_nonlocal_mode": (_c_int, [_vp, _c_int]),
+        "rsg_get_nonlocal_mode": (_c_int, [_vp, _P(_c_int), _P(_c_int)]),
         "rsg_get_projector_sizes": (_c_int, [_vp, _P(_c_int), _P(_c_int), _P(_c_i64)]),
         "rsg_get_projectors": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
         "rsg_hpsi_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
@@ -188,6 +190,16 @@ class Context:
         self._check(self.L.rsg_set_projectors(
             self.h, int(t["nzlma"]), int(t["maxmjj"]), _ptr(mjj), _ptr(jjp), _ptr(uvk), _ptr(iuv),
             _ptr(am) if am is not None else None))
+
+    def set_nonlocal_mode(self, mode: int):
+        """0 automatic, 1 tables, 2 analytic GTH projectors."""
+        self._check(self.L.rsg_set_nonlocal_mode(self.h, int(mode)))
+
+    def nonlocal_mode(self) -> tuple:
+        """(mode, analytic_available)."""
+        m, a = ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.rsg_get_nonlocal_mode(self.h, ctypes.byref(m), ctypes.byref(a)))
+        return m.value, bool(a.value)
 
     def projector_tables(self):
         nz, mx, ne = _c_int(), _c_int(), _c_i64()

@@ -122,3 +122,43 @@ def test_hpsi_errors(gpu_mod):
         ctx.set_grid((8, 8, 8), np.array([[10, 1, 0], [0, 10, 0], [0, 0, 10.0]]), 6)
     with pytest.raises(ValueError):
         ctx.hamiltonian(np.zeros((8, 1)), k=2)
+
+
+@pytest.mark.parametrize("nrep,ng,disp,nb", [(1, 36, 0.1, 13), (2, 36, 0.05, 19)])
+def test_nonlocal_modes_agree_with_oracle(orc, gpu_mod, nrep, ng, disp, nb):
+    """Tables (mode 1) and on-the-fly GTH projectors (mode 2), FP64 and FP32."""
+    from rsdft_develop_amd import hip
+
+    cell = systems.silicon_supercell(nrep, ngrid_per_cell=ng, displacement=disp, seed=nb)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    assert ctx.nonlocal_mode() == (0, True)
+    psi = systems.random_bands(cell.npoints, nb, seed=300 + nb)
+    want = ham.hpsi(psi)
+    for mode in (1, 2):
+        ctx.set_nonlocal_mode(mode)
+        print(f"case: mode {mode}, Si{8 * nrep ** 3}, {nb} bands", flush=True)
+        _close(ctx.hamiltonian(psi), want, 1e-12)
+        d_in = hip.DeviceArray.from_numpy(psi, dtype=np.float32)
+        d_out = hip.DeviceArray(psi.shape, dtype=np.float32)
+        ctx.hpsi_device(d_in.ptr, d_out.ptr, cell.npoints, nb, dtype=np.float32)
+        ctx.synchronize()
+        _close(d_out.to_numpy().astype(np.float64), want, 2e-5)
+        d_in.free()
+        d_out.free()
+
+
+def test_analytic_form_needs_a_cell_longer_than_two_cutoffs(orc, gpu_mod):
+    # one Si atom in a 6.5 bohr cube: the p sphere (3.52 bohr) meets its own
+    # periodic images, so only the tables (which fold images) apply
+    sp = systems.SI_GTH
+    cell = systems.Cell(aa=np.eye(3) * 6.5, frac=np.array([[0.1, 0.2, 0.3]]), kind=np.zeros(1, np.int32),
+                        species=(sp,), ngrid=(24, 24, 24), nband=4)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    assert ctx.nonlocal_mode() == (0, False)
+    psi = systems.random_bands(cell.npoints, 3, seed=5)
+    _close(ctx.hamiltonian(psi), ham.hpsi(psi), 1e-12)
+    ctx.set_nonlocal_mode(2)
+    with pytest.raises(gpu_mod.RSDFTGPUError):
+        ctx.hamiltonian(psi)
+    with pytest.raises(gpu_mod.RSDFTGPUError):
+        ctx.set_nonlocal_mode(3)
