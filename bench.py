@@ -95,12 +95,38 @@ def cpu_leg(cell, vloc, target_s: float = 12.0):
                        f"scaled linearly to {cell.nband} bands ({full * 1e3:.1f} ms/call)")
 
 
+def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80):
+    """Wall seconds of a whole SCF run to dE < etol Ha/atom (the run's own
+    set-up on the device included: tables, initial density and GS)."""
+    from rsdft_develop_amd import gpu, scf, systems
+
+    cell = systems.silicon_supercell(nrep, displacement=0.05, seed=0)
+    ctx = gpu.Context(0)
+    ctx.set_grid(cell.ngrid, cell.aa, cell.md)
+    ctx.build_gth_projectors(cell)
+    ctx.autotune_hpsi(64, cell.nband, reps=2)
+    ctx.autotune_hpsi(32, cell.nband, reps=2)
+    t0 = time.perf_counter()
+    run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=etol, max_iter=max_iter, mixed=mixed), ctx=ctx)
+    res = run.run()
+    wall = time.perf_counter() - t0
+    its = res.iterations
+    out = dict(seconds=round(wall, 3), iterations=len(its), converged=bool(res.converged), etot=res.etot,
+               fp32_iterations=sum(1 for r in its if r.precision == 32),
+               per_phase_s={k: round(sum(r.times.get(k, 0.0) for r in its), 3)
+                            for k in ("sd", "cg", "gs", "esp", "eto", "pot")},
+               esp=np.sort(res.esp).tolist())
+    ctx.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--scf", type=int, default=3, help="supercell repeat for the SCF-time leg (3 = Si216); 0 skips")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path is not sharded in this build); got --gpus {args.gpus}",
@@ -113,6 +139,18 @@ def main() -> int:
     r64 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float64)
     r32 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float32)
     cpu = None if args.no_cpu else cpu_leg(cell, vloc)
+    scf_out = None
+    if args.scf > 0:
+        plain = scf_leg(args.scf, mixed=False)
+        mixed = scf_leg(args.scf, mixed=True)
+        de = float(np.abs(np.array(mixed.pop("esp")) - np.array(plain.pop("esp"))).max())
+        natom = 8 * args.scf ** 3
+        scf_out = {"workload": f"Si{natom} {36 * args.scf}^3 grid, {natom * 5 // 2} bands, EKBT 0.01 Ha, "
+                               f"to dE < 1e-6 Ha/atom",
+                   "fp64": plain, "mixed": mixed,
+                   "speedup_mixed_vs_fp64": round(plain["seconds"] / mixed["seconds"], 3),
+                   "mixed_max_eig_diff_vs_fp64_Ha": de,
+                   "mixed_etot_diff_vs_fp64_Ha_per_atom": abs(mixed["etot"] - plain["etot"]) / natom}
     out = {
         "metric": METRIC,
         "value": round(r64["gbs"], 1),
@@ -136,6 +174,7 @@ def main() -> int:
         "f32": {"value": round(r32["gbs"], 1), "ms_per_step": round(r32["ms"], 4),
                 "frac": round(r32["gbs"] / HBM_PEAK_GBS, 4), "hpsi_shape": r32["shape"]},
         "cpu_baseline": cpu,
+        "scf": scf_out,
     }
     print(json.dumps(out))
     return 0

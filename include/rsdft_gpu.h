@@ -149,6 +149,14 @@ int rsg_subspace_diag(rsg_ctx *ctx, double *esp);
  * mb). */
 int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop,
                            double *esp, double *res);
+/* pc_kind 5 = the learned preconditioner declared below. */
+/* The same band update with every CG and preconditioner vector, hpsi
+ * included, in FP32 storage with FP32 arithmetic; dot products accumulate
+ * in FP64, the 2x2 Ritz step is FP64 and the accepted update lands in the
+ * FP64 store.  No reference counterpart (mixed-precision addition for the
+ * early SCF iterations). */
+int rsg_conjugate_gradient_f32(rsg_ctx *ctx, int ncg, int pc_kind, int mloop,
+                               double *esp, double *res);
 
 /* e_n = dV <psi_n|H psi_n>.  Replaces d_esp_calc (esp_calc_module.f90:17-68). */
 int rsg_esp_calc(rsg_ctx *ctx, double *esp);
@@ -200,6 +208,41 @@ int rsg_density_integrals(rsg_ctx *ctx, int field, double *e_loc,
  * the next input density into RHO. */
 int rsg_mix_init(rsg_ctx *ctx, int mmix, double beta);
 int rsg_mix_pulay(rsg_ctx *ctx);
+
+/* ---- learned CG preconditioner (pc_kind = 5 in rsg_conjugate_gradient*).
+ * No reference counterpart: it replaces the kinetic smoother
+ * precond_hprsdft (cgpc_hprsdft_module.f90:133-361) with a 3-layer 3x3x3
+ * convolution net on the periodic grid, P g = g + s * net(g/s, v), s =
+ * rms(g), v the cell's normalised Vloc; bfloat16 weights and activations,
+ * FP32 accumulation on MFMA.  A block whose <Pg, g> is not positive falls
+ * back to precond_hprsdft.  Parameters (FP32, this order):
+ * W1[16][2][27] b1[16] W2[16][16][27] b2[16] W3[1][16][27] b3[1], W[co][ci]
+ * [tap], tap = d3*9 + d2*3 + d1 (offsets d - 1 along z, y, x). */
+int rsg_lpc_param_count(void);
+int rsg_lpc_set_params(rsg_ctx *ctx, const float *params, int n);
+int rsg_lpc_get_params(rsg_ctx *ctx, float *params, int n);
+/* normalised potential input; vloc NULL takes the context's current Vloc */
+int rsg_lpc_set_potential(rsg_ctx *ctx, const double *vloc);
+int rsg_lpc_apply_f64(rsg_ctx *ctx, const double *d_g, double *d_pg,
+                      int64_t ld, int nb);
+int rsg_lpc_apply_f32(rsg_ctx *ctx, const float *d_g, float *d_pg,
+                      int64_t ld, int nb);
+/* Training targets t = A^-1 g, A = -1/2 lap_7 + (Vloc - min Vloc + shift),
+ * by `steps` CG steps (FP32, device, leading dimension N). */
+int rsg_lpc_target_f32(rsg_ctx *ctx, const float *d_g, float *d_t, int nb,
+                       int steps, double shift);
+/* One Adam step (lr > 0) or a gradient evaluation only (lr = 0) on nsamp
+ * residual/target pairs (device FP32, leading dimension N); loss = mean
+ * ((P g - t)/s)^2; grads (host, may be NULL) in the parameter order. */
+int rsg_lpc_train_step(rsg_ctx *ctx, const float *d_g, const float *d_t,
+                       int nsamp, float lr, double *loss, float *grads);
+int rsg_lpc_reset_optimizer(rsg_ctx *ctx);
+/* CG gradients g_n = -2 (H u_n - e_n u_n) of every band in the store,
+ * written FP32 to d_g (N x mb, device): the first gradient of a CG call
+ * (cg_module.f90:207), exported as training residuals. */
+int rsg_band_residuals_f32(rsg_ctx *ctx, float *d_g);
+/* CG band blocks that fell back to precond_hprsdft since creation */
+int rsg_lpc_fallbacks(rsg_ctx *ctx, int64_t *count);
 
 #ifdef __cplusplus
 }

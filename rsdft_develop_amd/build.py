@@ -28,7 +28,7 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", INCLUDE, "-I", CSRC, "-Wall", "-Wno-unused-result",
 ]
-SOURCES = ["capi.cpp", "tables.cpp", "hpsi_f64.hip", "hpsi_f32.hip", "solver.hip", "scf.hip"]
+SOURCES = ["capi.cpp", "tables.cpp", "hpsi_f64.hip", "hpsi_f32.hip", "solver.hip", "scf.hip", "lpc.hip"]
 
 
 def _compile(src: str) -> str:

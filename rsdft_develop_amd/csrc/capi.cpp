@@ -4,6 +4,7 @@
 #include <new>
 
 #include "context.hpp"
+#include "lpc.hpp"
 #include "solver.hpp"
 
 using rsg::DevBuf;
@@ -358,6 +359,12 @@ int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double
   return rsg::conjugate_gradient(ctx, ncg, pc_kind, mloop, esp, res);
 }
 
+int rsg_conjugate_gradient_f32(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res) {
+  if (!ctx || !esp || !res) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::conjugate_gradient_f32(ctx, ncg, pc_kind, mloop, esp, res);
+}
+
 int rsg_esp_calc(rsg_ctx *ctx, double *esp) {
   if (!ctx || !esp) return RSG_ERR_ARG;
   RSG_HIP(hipSetDevice(ctx->device));
@@ -436,6 +443,76 @@ int rsg_mix_pulay(rsg_ctx *ctx) {
   if (!ctx) return RSG_ERR_ARG;
   RSG_HIP(hipSetDevice(ctx->device));
   return rsg::scf_mix_pulay(ctx);
+}
+
+int rsg_lpc_param_count(void) { return rsg::lpc_param_count(); }
+
+int rsg_lpc_set_params(rsg_ctx *ctx, const float *params, int n) {
+  if (!ctx || !params) return RSG_ERR_ARG;
+  if (n != rsg::lpc_param_count()) return ctx->fail(RSG_ERR_ARG, "lpc_set_params: wrong parameter count");
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::lpc_set_params(ctx, params);
+}
+
+int rsg_lpc_get_params(rsg_ctx *ctx, float *params, int n) {
+  if (!ctx || !params) return RSG_ERR_ARG;
+  if (n != rsg::lpc_param_count()) return ctx->fail(RSG_ERR_ARG, "lpc_get_params: wrong parameter count");
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::lpc_get_params(ctx, params);
+}
+
+int rsg_lpc_set_potential(rsg_ctx *ctx, const double *vloc) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "lpc_set_potential: grid not set");
+  RSG_HIP(hipSetDevice(ctx->device));
+  if (vloc) return rsg::lpc_set_potential(ctx, vloc);
+  std::vector<double> v(static_cast<size_t>(ctx->N));
+  RSG_HIP(hipMemcpyAsync(v.data(), ctx->vloc64.p, size_t(ctx->N) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return rsg::lpc_set_potential(ctx, v.data());
+}
+
+int rsg_lpc_apply_f64(rsg_ctx *ctx, const double *g, double *pg, int64_t ld, int nb) {
+  if (!ctx || !g || !pg) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::lpc_apply(ctx, g, pg, ld, nb, 64);
+}
+
+int rsg_lpc_apply_f32(rsg_ctx *ctx, const float *g, float *pg, int64_t ld, int nb) {
+  if (!ctx || !g || !pg) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::lpc_apply(ctx, g, pg, ld, nb, 32);
+}
+
+int rsg_lpc_target_f32(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double shift) {
+  if (!ctx || !g || !t) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::lpc_target(ctx, g, t, nb, steps, shift);
+}
+
+int rsg_lpc_train_step(rsg_ctx *ctx, const float *g, const float *t, int nsamp, float lr, double *loss,
+                       float *grads) {
+  if (!ctx || !g || !t || nsamp < 1) return RSG_ERR_ARG;
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "lpc_train_step: grid not set");
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::lpc_train_step(ctx, g, t, nsamp, lr, loss, grads);
+}
+
+int rsg_lpc_reset_optimizer(rsg_ctx *ctx) {
+  if (!ctx) return RSG_ERR_ARG;
+  return rsg::lpc_reset_optimizer(ctx);
+}
+
+int rsg_band_residuals_f32(rsg_ctx *ctx, float *d_g) {
+  if (!ctx || !d_g) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::band_residuals_f32(ctx, d_g);
+}
+
+int rsg_lpc_fallbacks(rsg_ctx *ctx, int64_t *count) {
+  if (!ctx || !count) return RSG_ERR_ARG;
+  *count = ctx->lpc.fallbacks;
+  return RSG_OK;
 }
 
 }  // extern "C"

@@ -98,6 +98,23 @@ struct HpsiShape {
   int bt = 8;     // bands per scatter thread
 };
 
+// Learned preconditioner state: bf16 weight matrices for the kernels, the
+// FP32 master parameters with Adam moments, the normalised potential and
+// activation buffers.
+struct LpcState {
+  DevBuf wbf, bias, vnorm, x0, a1, a2, y, dyb, dz1, dz2, sval, gg, part, grad, master, mom1, mom2, lsum;
+  DevBuf diag, tbuf;  // training-target solve
+  bool have_params = false, have_v = false;
+  int64_t vN = 0;
+  int adam_t = 0;
+  int64_t fallbacks = 0;  // CG blocks that fell back to hprsdft
+  LpcState() = default;
+  LpcState(const LpcState &) = delete;
+  LpcState &operator=(const LpcState &) = delete;
+  ~LpcState();
+  void release();
+};
+
 struct HostTables {
   int nzlma = 0, maxmjj = 0;
   std::vector<int> mjj, jjp, iuv, amap;  // jjp/uvk column-major (maxmjj,nzlma)
@@ -153,6 +170,7 @@ struct rsg_ctx {
   rsg::DevBuf dotpart;             // multidot partial sums
   rsg::DevBuf dead;                // store target of padding lanes (64 doubles)
   rsg::ScfState *scf = nullptr;    // density/potential state of the SCF step
+  rsg::LpcState lpc;               // learned CG preconditioner
 
   int fail(int code, const std::string &msg) {
     err = msg;

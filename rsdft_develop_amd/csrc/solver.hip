@@ -11,6 +11,7 @@
 #include <cmath>
 #include <vector>
 
+#include "lpc.hpp"
 #include "solver.hpp"
 
 namespace rsg {
@@ -19,13 +20,14 @@ namespace {
 constexpr int kThreads = 256;
 
 // ---------------------------------------------------------------- reductions
-struct DotArgs {
-  const double *x[6];
-  const double *y[6];
+template <typename W> struct DotArgs {
+  const W *x[6];
+  const W *y[6];
 };
 
-template <int NP>
-__global__ __launch_bounds__(kThreads) void k_multidot(DotArgs a, int64_t N, int nb, int nseg,
+// products and sums in FP64 for either storage type
+template <int NP, typename W>
+__global__ __launch_bounds__(kThreads) void k_multidot(DotArgs<W> a, int64_t N, int nb, int nseg,
                                                        int64_t seglen, double *partial) {
   __shared__ double red[kThreads / 64][NP];
   const int s = blockIdx.x, b = blockIdx.y;
@@ -36,7 +38,7 @@ __global__ __launch_bounds__(kThreads) void k_multidot(DotArgs a, int64_t N, int
   const int64_t off = int64_t(b) * N;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
 #pragma unroll
-    for (int q = 0; q < NP; ++q) acc[q] += a.x[q][off + i] * a.y[q][off + i];
+    for (int q = 0; q < NP; ++q) acc[q] += double(a.x[q][off + i]) * double(a.y[q][off + i]);
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -66,24 +68,26 @@ __global__ void k_reduce_partials(const double *__restrict__ partial, int nseg, 
 
 // --------------------------------------------------------- band-wise updates
 // gk = -2 (hx - E x)          (cg_module.f90:207)
-__global__ void k_cg_grad(const double *__restrict__ hx, const double *__restrict__ x,
-                          const double *__restrict__ E, double *__restrict__ g, int64_t N) {
+template <typename W>
+__global__ void k_cg_grad(const W *__restrict__ hx, const W *__restrict__ x, const double *__restrict__ E,
+                          W *__restrict__ g, int64_t N) {
   const int b = blockIdx.y;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int64_t k = int64_t(b) * N + i;
-  g[k] = -2.0 * (hx[k] - E[b] * x[k]);
+  const W e = W(E[b]);
+  g[k] = W(-2) * (hx[k] - e * x[k]);
 }
 
 // y = x (first step) or y = x + (rb/gkgk) y   (cg_module.f90:266-279)
-__global__ void k_cg_dir(const double *__restrict__ pg, double *__restrict__ pk,
-                         const double *__restrict__ rb, const double *__restrict__ gkgk, int first,
-                         int64_t N) {
+template <typename W>
+__global__ void k_cg_dir(const W *__restrict__ pg, W *__restrict__ pk, const double *__restrict__ rb,
+                         const double *__restrict__ gkgk, int first, int64_t N) {
   const int b = blockIdx.y;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int64_t k = int64_t(b) * N + i;
-  pk[k] = first ? pg[k] : pg[k] + (rb[b] / gkgk[b]) * pk[k];
+  pk[k] = first ? pg[k] : pg[k] + W(rb[b] / gkgk[b]) * pk[k];
 }
 
 // 2x2 generalised Ritz step per band (cg_module.f90:305-334): DSYGV on
@@ -136,18 +140,18 @@ __global__ void k_cg_ritz(const double *__restrict__ d6, int nb, double *__restr
 }
 
 // hx <- u1 hx + u2 hp ; g <- -2 (hx - W (u1 x + u2 p))   (cg_module.f90:336-352)
-__global__ void k_cg_update(double *__restrict__ hx, const double *__restrict__ hp,
-                            const double *__restrict__ x, const double *__restrict__ p,
-                            double *__restrict__ g, const double *__restrict__ uc,
+template <typename W>
+__global__ void k_cg_update(W *__restrict__ hx, const W *__restrict__ hp, const W *__restrict__ x,
+                            const W *__restrict__ p, W *__restrict__ g, const double *__restrict__ uc,
                             const double *__restrict__ E, int64_t N) {
   const int b = blockIdx.y;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int64_t k = int64_t(b) * N + i;
-  const double u1 = uc[2 * b], u2 = uc[2 * b + 1], W = E[b];
-  const double h = u1 * hx[k] + u2 * hp[k];
+  const W u1 = W(uc[2 * b]), u2 = W(uc[2 * b + 1]), ev = W(E[b]);
+  const W h = u1 * hx[k] + u2 * hp[k];
   hx[k] = h;
-  g[k] = -2.0 * (h - W * (u1 * x[k] + u2 * p[k]));
+  g[k] = W(-2) * (h - ev * (u1 * x[k] + u2 * p[k]));
 }
 
 // divergence guard (cg_module.f90:358-364): reject when rb/res > 1e8
@@ -162,15 +166,25 @@ __global__ void k_cg_guard(const double *__restrict__ rb, const double *__restri
 }
 
 // x <- u1 x + u2 p for accepted bands (cg_module.f90:366)
-__global__ void k_cg_accept(double *__restrict__ x, const double *__restrict__ p,
-                            const double *__restrict__ uc, const int *__restrict__ accept,
-                            int64_t N) {
+// (mixed precision: x is the FP64 store, x32 its FP32 copy for the block)
+template <typename W>
+__global__ void k_cg_accept(double *__restrict__ x, W *__restrict__ x32, const W *__restrict__ p,
+                            const double *__restrict__ uc, const int *__restrict__ accept, int64_t N) {
   const int b = blockIdx.y;
   if (!accept[b]) return;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int64_t k = int64_t(b) * N + i;
-  x[k] = uc[2 * b] * x[k] + uc[2 * b + 1] * p[k];
+  const double v = uc[2 * b] * x[k] + uc[2 * b + 1] * double(p[k]);
+  x[k] = v;
+  if (x32) x32[k] = W(v);
+}
+
+// x32 = (W) x for an nn-band block
+template <typename W>
+__global__ void k_to_work(const double *__restrict__ x, W *__restrict__ x32, int64_t total) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < total) x32[k] = W(x[k]);
 }
 
 __global__ void k_scale_copy(const double *__restrict__ a, double s, double *__restrict__ b, int n) {
@@ -186,31 +200,33 @@ __global__ void k_fill(double *__restrict__ a, double v, int n) {
 // ------------------------------------------------ hprsdft preconditioner
 // f = A g with A the 7-point -1/2 Laplacian (E0 = 0): precond_cg_mat2_omp,
 // cgpc_hprsdft_module.f90:288-361, periodic 1-deep halo (bcset_1).
-__global__ __launch_bounds__(kThreads) void k_pc_mat(const double *__restrict__ g,
-                                                     double *__restrict__ f, int n1, int n2,
-                                                     int n3, double c0, double c1, double c2,
-                                                     double c3) {
+template <typename W>
+__global__ __launch_bounds__(kThreads) void k_pc_mat(const W *__restrict__ g, W *__restrict__ f, int n1,
+                                                     int n2, int n3, W c0, W c1, W c2, W c3,
+                                                     const float *__restrict__ diag) {
   const int64_t n12 = int64_t(n1) * n2, N = n12 * n3;
   const int b = blockIdx.y;
   const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= N) return;
   const int i1 = int(j % n1), i2 = int((j / n1) % n2), i3 = int(j / n12);
-  const double *w = g + int64_t(b) * N;
+  const W *w = g + int64_t(b) * N;
   const int xp = i1 + 1 == n1 ? 0 : i1 + 1, xm = i1 == 0 ? n1 - 1 : i1 - 1;
   const int yp = i2 + 1 == n2 ? 0 : i2 + 1, ym = i2 == 0 ? n2 - 1 : i2 - 1;
   const int zp = i3 + 1 == n3 ? 0 : i3 + 1, zm = i3 == 0 ? n3 - 1 : i3 - 1;
   const int64_t row = int64_t(i2) * n1 + i3 * n12;
-  f[int64_t(b) * N + j] = c0 * w[j] + c1 * (w[xp + row] + w[xm + row]) +
+  const W d = diag ? W(diag[j]) : W(0);
+  f[int64_t(b) * N + j] = (c0 + d) * w[j] + c1 * (w[xp + row] + w[xm + row]) +
                           c2 * (w[i1 + int64_t(yp) * n1 + i3 * n12] + w[i1 + int64_t(ym) * n1 + i3 * n12]) +
                           c3 * (w[i1 + int64_t(i2) * n1 + zp * n12] + w[i1 + int64_t(i2) * n1 + zm * n12]);
 }
 
 // r = g - f ; p = r
-__global__ void k_pc_init(const double *__restrict__ g, const double *__restrict__ f,
-                          double *__restrict__ r, double *__restrict__ p, int64_t total) {
+template <typename W>
+__global__ void k_pc_init(const W *__restrict__ g, const W *__restrict__ f, W *__restrict__ r,
+                          W *__restrict__ p, int64_t total) {
   const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= total) return;
-  const double v = g[k] - f[k];
+  const W v = g[k] - f[k];
   r[k] = v;
   p[k] = v;
 }
@@ -225,27 +241,27 @@ __global__ void k_pc_alpha(const double *__restrict__ rr1, const double *__restr
 }
 
 // x += a p ; optionally r -= a f
-__global__ void k_pc_step(double *__restrict__ x, const double *__restrict__ p,
-                          double *__restrict__ r, const double *__restrict__ f,
-                          const double *__restrict__ a, int update_r, int64_t N) {
+template <typename W>
+__global__ void k_pc_step(W *__restrict__ x, const W *__restrict__ p, W *__restrict__ r,
+                          const W *__restrict__ f, const double *__restrict__ a, int update_r, int64_t N) {
   const int b = blockIdx.y;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int64_t k = int64_t(b) * N + i;
-  const double s = a[b];
+  const W s = W(a[b]);
   x[k] += s * p[k];
   if (update_r) r[k] -= s * f[k];
 }
 
 // p = r + (rr1/rr0) p
-__global__ void k_pc_dir(const double *__restrict__ r, double *__restrict__ p,
-                         const double *__restrict__ rr1, const double *__restrict__ rr0,
-                         int64_t N) {
+template <typename W>
+__global__ void k_pc_dir(const W *__restrict__ r, W *__restrict__ p, const double *__restrict__ rr1,
+                         const double *__restrict__ rr0, int64_t N) {
   const int b = blockIdx.y;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int64_t k = int64_t(b) * N + i;
-  p[k] = r[k] + (rr1[b] / rr0[b]) * p[k];
+  p[k] = r[k] + W(rr1[b] / rr0[b]) * p[k];
 }
 
 inline dim3 grid_nb(int64_t N, int nb) { return dim3(unsigned((N + kThreads - 1) / kThreads), nb); }
@@ -286,8 +302,10 @@ int seg_count(int64_t N, int nb) {
 
 }  // namespace
 
-int multidot(rsg_ctx *ctx, int npair, const double *const *x, const double *const *y, int nb,
-             double alpha, double *out) {
+namespace {
+
+template <typename W>
+int multidot_t(rsg_ctx *ctx, int npair, const W *const *x, const W *const *y, int nb, double alpha, double *out) {
   if (npair < 1 || npair > 6) return ctx->fail(RSG_ERR_ARG, "multidot: 1..6 pairs");
   if (nb == 0) return RSG_OK;
   const int64_t N = ctx->N;
@@ -296,17 +314,17 @@ int multidot(rsg_ctx *ctx, int npair, const double *const *x, const double *cons
   if (ctx->dotpart.ensure(size_t(npair) * nb * nseg * sizeof(double)) != hipSuccess)
     return ctx->fail(RSG_ERR_HIP, "multidot: scratch allocation failed");
   double *partial = ctx->dotpart.as<double>();
-  DotArgs a{};
+  DotArgs<W> a{};
   for (int q = 0; q < npair; ++q) {
     a.x[q] = x[q];
     a.y[q] = y[q];
   }
   dim3 grid(nseg, nb);
   switch (npair) {
-#define RSG_DOT_CASE(P)                                                                           \
-  case P:                                                                                         \
-    hipLaunchKernelGGL(k_multidot<P>, grid, dim3(kThreads), 0, ctx->stream, a, N, nb, nseg, seglen, \
-                       partial);                                                                  \
+#define RSG_DOT_CASE(P)                                                                                  \
+  case P:                                                                                                \
+    hipLaunchKernelGGL((k_multidot<P, W>), grid, dim3(kThreads), 0, ctx->stream, a, N, nb, nseg, seglen, \
+                       partial);                                                                         \
     break;
     RSG_DOT_CASE(1)
     RSG_DOT_CASE(2)
@@ -320,6 +338,18 @@ int multidot(rsg_ctx *ctx, int npair, const double *const *x, const double *cons
   hipLaunchKernelGGL(k_reduce_partials, dim3((count + 255) / 256), dim3(256), 0, ctx->stream, partial,
                      nseg, count, alpha, out);
   return RSG_OK;
+}
+
+}  // namespace
+
+int multidot(rsg_ctx *ctx, int npair, const double *const *x, const double *const *y, int nb,
+             double alpha, double *out) {
+  return multidot_t<double>(ctx, npair, x, y, nb, alpha, out);
+}
+
+int multidot(rsg_ctx *ctx, int npair, const float *const *x, const float *const *y, int nb, double alpha,
+             double *out) {
+  return multidot_t<float>(ctx, npair, x, y, nb, alpha, out);
 }
 
 namespace {
@@ -338,8 +368,12 @@ int read_info(rsg_ctx *ctx, int *host) {
 
 // hprsdft preconditioner on an nn-band block (cgpc_hprsdft_module.f90:133-270):
 // mloop CG steps on A x = g from x0 = pg (= g on entry).
-int precond_hprsdft(rsg_ctx *ctx, int mloop, const double *g, double *pg, int nn, double *r,
-                    double *p, double *f, Scratch &s) {
+// With diag non-null the operator gains a diagonal term, A = -1/2 lap_7 +
+// diag (the shifted, potential-aware solve that trains the learned
+// preconditioner).
+template <typename W>
+int precond_hprsdft(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W *r, W *p, W *f, Scratch &s,
+                    const float *diag = nullptr) {
   if (mloop <= 0) return RSG_OK;
   const int64_t N = ctx->N;
   const double c0 = ctx->ggg[0] / (ctx->hgrid[0] * ctx->hgrid[0]) +
@@ -350,10 +384,12 @@ int precond_hprsdft(rsg_ctx *ctx, int mloop, const double *g, double *pg, int nn
   const double c3 = -0.5 / (ctx->hgrid[2] * ctx->hgrid[2]) * ctx->ggg[2];
   const dim3 gN = grid_nb(N, nn), blk(kThreads);
   const int64_t tot = N * nn;
-  hipLaunchKernelGGL(k_pc_mat, gN, blk, 0, ctx->stream, g, f, ctx->n[0], ctx->n[1], ctx->n[2], c0, c1, c2, c3);
-  hipLaunchKernelGGL(k_pc_init, dim3(unsigned((tot + kThreads - 1) / kThreads)), blk, 0, ctx->stream, g, f, r, p, tot);
+  hipLaunchKernelGGL((k_pc_mat<W>), gN, blk, 0, ctx->stream, pg, f, ctx->n[0], ctx->n[1], ctx->n[2], W(c0), W(c1),
+                     W(c2), W(c3), diag);
+  hipLaunchKernelGGL((k_pc_init<W>), dim3(unsigned((tot + kThreads - 1) / kThreads)), blk, 0, ctx->stream, g, f, r,
+                     p, tot);
   {
-    const double *xs[1] = {r}, *ys[1] = {r};
+    const W *xs[1] = {r}, *ys[1] = {r};
     int rc = multidot(ctx, 1, xs, ys, nn, 1.0, s.rr1);
     if (rc) return rc;
   }
@@ -365,18 +401,19 @@ int precond_hprsdft(rsg_ctx *ctx, int mloop, const double *g, double *pg, int nn
   if (small) return RSG_OK;
   const dim3 gb((nn + 63) / 64), b64(64);
   for (int it = 1; it <= mloop; ++it) {
-    hipLaunchKernelGGL(k_pc_mat, gN, blk, 0, ctx->stream, p, f, ctx->n[0], ctx->n[1], ctx->n[2], c0, c1, c2, c3);
-    const double *xs[1] = {p}, *ys[1] = {f};
+    hipLaunchKernelGGL((k_pc_mat<W>), gN, blk, 0, ctx->stream, p, f, ctx->n[0], ctx->n[1], ctx->n[2], W(c0),
+                       W(c1), W(c2), W(c3), diag);
+    const W *xs[1] = {p}, *ys[1] = {f};
     int rc = multidot(ctx, 1, xs, ys, nn, 1.0, s.pap);
     if (rc) return rc;
     hipLaunchKernelGGL(k_pc_alpha, gb, b64, 0, ctx->stream, s.rr1, s.pap, s.rr0, s.a, nn);
     const int last = it == mloop;
-    hipLaunchKernelGGL(k_pc_step, gN, blk, 0, ctx->stream, pg, p, r, f, s.a, last ? 0 : 1, N);
+    hipLaunchKernelGGL((k_pc_step<W>), gN, blk, 0, ctx->stream, pg, p, r, f, s.a, last ? 0 : 1, N);
     if (last) break;
-    const double *xr[1] = {r}, *yr[1] = {r};
+    const W *xr[1] = {r}, *yr[1] = {r};
     rc = multidot(ctx, 1, xr, yr, nn, 1.0, s.rr1);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pc_dir, gN, blk, 0, ctx->stream, r, p, s.rr1, s.rr0, N);
+    hipLaunchKernelGGL((k_pc_dir<W>), gN, blk, 0, ctx->stream, r, p, s.rr1, s.rr0, N);
   }
   RSG_HIP(hipGetLastError());
   return RSG_OK;
@@ -463,46 +500,67 @@ int esp_calc(rsg_ctx *ctx, double *esp) {
   return RSG_OK;
 }
 
+namespace {
+
 // d_conjugate_gradient_1 (cg_module.f90:112-431) with the band block MB_d
-// as large as the CG work arrays allow.
-int conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res) {
+// as large as the CG work arrays allow.  W = double is the plain FP64
+// method on the store itself.  W = float keeps every CG and preconditioner
+// vector, and an FP32 copy of the block's bands, in FP32 storage with FP32
+// arithmetic (hpsi included); dot products accumulate in FP64, the 2x2
+// Ritz step runs in FP64 and the accepted update is applied to the FP64
+// store.
+template <typename W>
+int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res) {
   int rc = check_store(ctx, "conjugate_gradient");
   if (rc) return rc;
-  if (ncg < 0 || (pc_kind != 0 && pc_kind != 4)) return ctx->fail(RSG_ERR_ARG, "conjugate_gradient: ncg < 0 or pc_kind not 0/4");
+  if (ncg < 0 || (pc_kind != 0 && pc_kind != 4 && pc_kind != 5))
+    return ctx->fail(RSG_ERR_ARG, "conjugate_gradient: ncg < 0 or pc_kind not 0/4/5");
+  constexpr bool kMixed = sizeof(W) == 4;
   const int mb = ctx->mb;
   const int64_t N = ctx->N;
   const double dv = ctx->dv;
-  // 8 work vectors per band: hxk hpk gk Pgk pk + preconditioner r p f
+  // work vectors per band: hxk hpk gk Pgk pk + preconditioner r p f (+ x32)
+  constexpr int kVec = kMixed ? 9 : 8;
   const size_t budget = size_t(64) << 30;
-  int mbd = int(std::min<size_t>(size_t(mb), std::max<size_t>(1, budget / (size_t(N) * 8 * 8))));
-  if (ctx->cgbuf.ensure(size_t(N) * mbd * 8 * 8) != hipSuccess)
+  int mbd = int(std::min<size_t>(size_t(mb), std::max<size_t>(1, budget / (size_t(N) * sizeof(W) * kVec))));
+  if (ctx->cgbuf.ensure(size_t(N) * mbd * sizeof(W) * kVec) != hipSuccess)
     return ctx->fail(RSG_ERR_HIP, "conjugate_gradient: work allocation failed");
   Scratch s;
   rc = ensure_scalars(ctx, mbd, 1, s);
   if (rc) return rc;
-  double *base = ctx->cgbuf.as<double>();
+  W *base = ctx->cgbuf.as<W>();
   const size_t blk = size_t(N) * mbd;
-  double *hxk = base, *hpk = base + blk, *gk = base + 2 * blk, *pgk = base + 3 * blk, *pk = base + 4 * blk;
-  double *pr = base + 5 * blk, *pp = base + 6 * blk, *pf = base + 7 * blk;
+  W *hxk = base, *hpk = base + blk, *gk = base + 2 * blk, *pgk = base + 3 * blk, *pk = base + 4 * blk;
+  W *pr = base + 5 * blk, *pp = base + 6 * blk, *pf = base + 7 * blk;
+  W *x32 = kMixed ? base + 8 * blk : nullptr;
   std::vector<double> hE(static_cast<size_t>(mbd)), hE1(static_cast<size_t>(mbd));
   for (int ns = 0; ns < mb; ns += mbd) {
     const int nn = std::min(mbd, mb - ns);
-    double *x = ctx->unk.as<double>() + size_t(ns) * N;
+    double *xs64 = ctx->unk.as<double>() + size_t(ns) * N;
     const dim3 gN = grid_nb(N, nn), blkd(kThreads), gb((nn + 63) / 64), b64(64);
+    W *x;
+    if constexpr (kMixed) {
+      const int64_t tot = N * nn;
+      hipLaunchKernelGGL((k_to_work<W>), dim3(unsigned((tot + kThreads - 1) / kThreads)), blkd, 0, ctx->stream, xs64,
+                         x32, tot);
+      x = x32;
+    } else {
+      x = reinterpret_cast<W *>(xs64);
+    }
     hipLaunchKernelGGL(k_fill, gb, b64, 0, ctx->stream, s.E1, 1.0e10, nn);
-    rc = launch_hpsi<double>(ctx, x, hxk, N, nn);
+    rc = launch_hpsi<W>(ctx, x, hxk, N, nn);
     if (rc) return rc;
     {
-      const double *xs[1] = {x}, *ys[1] = {hxk};
+      const W *xs[1] = {x}, *ys[1] = {hxk};
       if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.E))) return rc;
     }
-    hipLaunchKernelGGL(k_cg_grad, gN, blkd, 0, ctx->stream, hxk, x, s.E, gk, N);
+    hipLaunchKernelGGL((k_cg_grad<W>), gN, blkd, 0, ctx->stream, hxk, x, s.E, gk, N);
     {
-      const double *xs[1] = {gk}, *ys[1] = {gk};
+      const W *xs[1] = {gk}, *ys[1] = {gk};
       if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
     }
     for (int icg = 1; icg <= ncg + 1; ++icg) {
-      RSG_HIP(hipMemcpyAsync(pgk, gk, size_t(N) * nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+      RSG_HIP(hipMemcpyAsync(pgk, gk, size_t(N) * nn * sizeof(W), hipMemcpyDeviceToDevice, ctx->stream));
       hipLaunchKernelGGL(k_scale_copy, gb, b64, 0, ctx->stream, s.rb, 0.25, s.res, nn);
       RSG_HIP(hipMemcpyAsync(hE.data(), s.E, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
       RSG_HIP(hipMemcpyAsync(hE1.data(), s.E1, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -510,32 +568,117 @@ int conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *es
       bool conv = true;
       for (int b = 0; b < nn; ++b) conv = conv && (std::fabs(hE[b] - hE1[b]) < 1e-15);
       if (conv || icg == ncg + 1) break;
-      if (pc_kind == 4 && (rc = precond_hprsdft(ctx, mloop, gk, pgk, nn, pr, pp, pf, s))) return rc;
+      if (pc_kind == 4 && (rc = precond_hprsdft<W>(ctx, mloop, gk, pgk, nn, pr, pp, pf, s))) return rc;
+      if (pc_kind == 5 && (rc = lpc_apply(ctx, gk, pgk, N, nn, kMixed ? 32 : 64))) return rc;
       {
-        const double *xs[1] = {pgk}, *ys[1] = {gk};
+        const W *xs[1] = {pgk}, *ys[1] = {gk};
         if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
       }
-      hipLaunchKernelGGL(k_cg_dir, gN, blkd, 0, ctx->stream, pgk, pk, s.rb, s.gkgk, icg == 1 ? 1 : 0, N);
+      if (pc_kind == 5) {
+        // descent check: <Pg, g> must be positive and finite for every band,
+        // else the block falls back to the hprsdft smoother
+        std::vector<double> rbh(static_cast<size_t>(nn));
+        RSG_HIP(hipMemcpyAsync(rbh.data(), s.rb, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
+        RSG_HIP(hipStreamSynchronize(ctx->stream));
+        bool good = true;
+        for (double v : rbh) good = good && std::isfinite(v) && v > 0.0;
+        if (!good) {
+          ++ctx->lpc.fallbacks;
+          RSG_HIP(hipMemcpyAsync(pgk, gk, size_t(N) * nn * sizeof(W), hipMemcpyDeviceToDevice, ctx->stream));
+          if ((rc = precond_hprsdft<W>(ctx, mloop, gk, pgk, nn, pr, pp, pf, s))) return rc;
+          const W *xs[1] = {pgk}, *ys[1] = {gk};
+          if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
+        }
+      }
+      hipLaunchKernelGGL((k_cg_dir<W>), gN, blkd, 0, ctx->stream, pgk, pk, s.rb, s.gkgk, icg == 1 ? 1 : 0, N);
       RSG_HIP(hipMemcpyAsync(s.gkgk, s.rb, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
-      if ((rc = launch_hpsi<double>(ctx, pk, hpk, N, nn))) return rc;
+      if ((rc = launch_hpsi<W>(ctx, pk, hpk, N, nn))) return rc;
       {
-        const double *xs[6] = {x, pk, pk, x, pk, pk}, *ys[6] = {x, x, pk, hxk, hxk, hpk};
+        const W *xs[6] = {x, pk, pk, x, pk, pk}, *ys[6] = {x, x, pk, hxk, hxk, hpk};
         if ((rc = multidot(ctx, 6, xs, ys, nn, dv, s.d6))) return rc;
       }
       hipLaunchKernelGGL(k_cg_ritz, gb, b64, 0, ctx->stream, s.d6, nn, s.E, s.E1, s.uc);
-      hipLaunchKernelGGL(k_cg_update, gN, blkd, 0, ctx->stream, hxk, hpk, x, pk, gk, s.uc, s.E, N);
+      hipLaunchKernelGGL((k_cg_update<W>), gN, blkd, 0, ctx->stream, hxk, hpk, x, pk, gk, s.uc, s.E, N);
       {
-        const double *xs[1] = {gk}, *ys[1] = {gk};
+        const W *xs[1] = {gk}, *ys[1] = {gk};
         if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
       }
       hipLaunchKernelGGL(k_cg_guard, gb, b64, 0, ctx->stream, s.rb, s.res, s.E, s.E1, s.accept, nn);
-      hipLaunchKernelGGL(k_cg_accept, gN, blkd, 0, ctx->stream, x, pk, s.uc, s.accept, N);
+      hipLaunchKernelGGL((k_cg_accept<W>), gN, blkd, 0, ctx->stream, xs64, x32, pk, s.uc, s.accept, N);
       RSG_HIP(hipGetLastError());
     }
     RSG_HIP(hipMemcpyAsync(esp + ns, s.E, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
     RSG_HIP(hipMemcpyAsync(res + ns, s.res, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
     RSG_HIP(hipStreamSynchronize(ctx->stream));
   }
+  return RSG_OK;
+}
+
+}  // namespace
+
+int conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res) {
+  return cg_impl<double>(ctx, ncg, pc_kind, mloop, esp, res);
+}
+
+int conjugate_gradient_f32(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res) {
+  return cg_impl<float>(ctx, ncg, pc_kind, mloop, esp, res);
+}
+
+namespace {
+__global__ void k_resid32(const double *__restrict__ hx, const double *__restrict__ x, const double *__restrict__ E,
+                          float *__restrict__ g, int64_t N) {
+  const int b = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t k = int64_t(b) * N + i;
+  g[k] = float(-2.0 * (hx[k] - E[b] * x[k]));
+}
+}  // namespace
+
+// CG gradients g_n = -2 (H u_n - e_n u_n), e_n = dV <u_n|H u_n>, of the
+// whole store in FP32 (cg_module.f90:207, the first gradient of a CG call):
+// the residuals the learned preconditioner is trained on.
+int band_residuals_f32(rsg_ctx *ctx, float *g) {
+  int rc = check_store(ctx, "band_residuals");
+  if (rc) return rc;
+  const int mb = ctx->mb;
+  const int64_t N = ctx->N;
+  if (ctx->esp_dev.ensure(size_t(mb) * 16) != hipSuccess) return ctx->fail(RSG_ERR_HIP, "band_residuals: allocation failed");
+  double *U = ctx->unk.as<double>(), *HU = ctx->work.as<double>();
+  if ((rc = launch_hpsi<double>(ctx, U, HU, N, mb))) return rc;
+  const double *xs[1] = {U}, *ys[1] = {HU};
+  if ((rc = multidot(ctx, 1, xs, ys, mb, ctx->dv, ctx->esp_dev.as<double>()))) return rc;
+  hipLaunchKernelGGL(k_resid32, grid_nb(N, mb), dim3(kThreads), 0, ctx->stream, HU, U, ctx->esp_dev.as<double>(), g, N);
+  RSG_HIP(hipGetLastError());
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+// Training targets of the learned preconditioner: t = A^-1 g with
+// A = -1/2 lap_7 + (Vloc - min Vloc + shift), by `steps` CG steps from
+// t0 = g, for nb FP32 bands (device, leading dimension N).
+int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double shift) {
+  if (!ctx->have_grid || !ctx->have_vloc) return ctx->fail(RSG_ERR_STATE, "lpc_target: grid or Vloc not set");
+  if (nb < 1 || steps < 1 || !(shift > 0.0)) return ctx->fail(RSG_ERR_ARG, "lpc_target: nb, steps >= 1, shift > 0");
+  const int64_t N = ctx->N;
+  LpcState &L = ctx->lpc;
+  std::vector<double> v(static_cast<size_t>(N));
+  RSG_HIP(hipMemcpyAsync(v.data(), ctx->vloc64.p, size_t(N) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  const double vmin = *std::min_element(v.begin(), v.end());
+  std::vector<float> d(static_cast<size_t>(N));
+  for (int64_t i = 0; i < N; ++i) d[size_t(i)] = float(v[size_t(i)] - vmin + shift);
+  if (L.diag.ensure(size_t(N) * 4) != hipSuccess || L.tbuf.ensure(size_t(N) * nb * 4 * 3) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "lpc_target: allocation failed");
+  RSG_HIP(hipMemcpyAsync(L.diag.p, d.data(), size_t(N) * 4, hipMemcpyHostToDevice, ctx->stream));
+  Scratch s;
+  int rc = ensure_scalars(ctx, nb, 1, s);
+  if (rc) return rc;
+  float *r = L.tbuf.as<float>(), *p = r + size_t(N) * nb, *f = p + size_t(N) * nb;
+  RSG_HIP(hipMemcpyAsync(t, g, size_t(N) * nb * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  rc = precond_hprsdft<float>(ctx, steps, g, t, nb, r, p, f, s, L.diag.as<float>());
+  if (rc) return rc;
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
   return RSG_OK;
 }
 

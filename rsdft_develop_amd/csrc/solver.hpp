@@ -6,6 +6,9 @@ namespace rsg {
 int gram_schmidt(rsg_ctx *ctx, int n0, int n1);
 int subspace_diag(rsg_ctx *ctx, double *esp);
 int conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res);
+int conjugate_gradient_f32(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res);
+int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double shift);
+int band_residuals_f32(rsg_ctx *ctx, float *g);
 int esp_calc(rsg_ctx *ctx, double *esp);
 void solver_release(rsg_ctx *ctx);
 
@@ -14,6 +17,8 @@ void solver_release(rsg_ctx *ctx);
 // device.  Deterministic two-pass reduction.
 int multidot(rsg_ctx *ctx, int npair, const double *const *x, const double *const *y, int nb,
              double alpha, double *out);
+int multidot(rsg_ctx *ctx, int npair, const float *const *x, const float *const *y, int nb, double alpha,
+             double *out);
 // scf.hip
 int scf_set_local_gth(rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind, const double *zps,
                       const double *rloc, const double *cloc);

@@ -82,6 +82,7 @@ def load_library():
         "rsg_gram_schmidt": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_subspace_diag": (_c_int, [_vp, _vp]),
         "rsg_conjugate_gradient": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp]),
+        "rsg_conjugate_gradient_f32": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp]),
         "rsg_esp_calc": (_c_int, [_vp, _vp]),
         "rsg_set_local_gth": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _vp, _vp, _vp]),
         "rsg_get_scf_constants": (_c_int, [_vp, _P(_c_dbl), _P(_c_dbl)]),
@@ -95,6 +96,17 @@ def load_library():
         "rsg_density_integrals": (_c_int, [_vp, _c_int, _P(_c_dbl), _P(_c_dbl)]),
         "rsg_mix_init": (_c_int, [_vp, _c_int, _c_dbl]),
         "rsg_mix_pulay": (_c_int, [_vp]),
+        "rsg_lpc_param_count": (_c_int, []),
+        "rsg_lpc_set_params": (_c_int, [_vp, _vp, _c_int]),
+        "rsg_lpc_get_params": (_c_int, [_vp, _vp, _c_int]),
+        "rsg_lpc_set_potential": (_c_int, [_vp, _vp]),
+        "rsg_lpc_apply_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
+        "rsg_lpc_apply_f32": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
+        "rsg_lpc_target_f32": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_dbl]),
+        "rsg_lpc_train_step": (_c_int, [_vp, _vp, _vp, _c_int, ctypes.c_float, _P(_c_dbl), _vp]),
+        "rsg_lpc_reset_optimizer": (_c_int, [_vp]),
+        "rsg_band_residuals_f32": (_c_int, [_vp, _vp]),
+        "rsg_lpc_fallbacks": (_c_int, [_vp, _P(_c_i64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -292,11 +304,14 @@ class Context:
         self._check(self.L.rsg_subspace_diag(self.h, _ptr(esp)))
         return esp
 
-    def conjugate_gradient(self, ncg: int = 3, pc_kind: int = 4, mloop: int = 3, k: int = 1, s: int = 1):
+    def conjugate_gradient(self, ncg: int = 3, pc_kind: int = 4, mloop: int = 3, k: int = 1, s: int = 1,
+                           precision: int = 64):
+        """Band CG update; precision 32 runs the FP32-storage variant."""
         self._gamma_only(k, s)
         esp = np.zeros(self.mb)
         res = np.zeros(self.mb)
-        self._check(self.L.rsg_conjugate_gradient(self.h, int(ncg), int(pc_kind), int(mloop), _ptr(esp), _ptr(res)))
+        f = {64: self.L.rsg_conjugate_gradient, 32: self.L.rsg_conjugate_gradient_f32}[precision]
+        self._check(f(self.h, int(ncg), int(pc_kind), int(mloop), _ptr(esp), _ptr(res)))
         return esp, res
 
     def esp_calc(self, k: int = 1, s: int = 1) -> np.ndarray:
@@ -355,6 +370,52 @@ class Context:
 
     def mix_init(self, mmix: int = 4, beta: float = 0.3):
         self._check(self.L.rsg_mix_init(self.h, int(mmix), float(beta)))
+
+    # ---- learned CG preconditioner
+    def lpc_param_count(self) -> int:
+        return int(self.L.rsg_lpc_param_count())
+
+    def lpc_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.L.rsg_lpc_set_params(self.h, p.ctypes.data, int(p.size)))
+
+    def lpc_get_params(self) -> np.ndarray:
+        p = np.zeros(self.lpc_param_count(), dtype=np.float32)
+        self._check(self.L.rsg_lpc_get_params(self.h, p.ctypes.data, int(p.size)))
+        return p
+
+    def lpc_set_potential(self, vloc=None):
+        """Normalised potential input (None: the context's current Vloc)."""
+        if vloc is None:
+            self._check(self.L.rsg_lpc_set_potential(self.h, None))
+        else:
+            v = _f64(vloc)
+            self._check(self.L.rsg_lpc_set_potential(self.h, _ptr(v)))
+
+    def lpc_apply_device(self, d_g: int, d_pg: int, ld: int, nb: int, dtype=np.float64):
+        f = self.L.rsg_lpc_apply_f64 if dtype == np.float64 else self.L.rsg_lpc_apply_f32
+        self._check(f(self.h, d_g, d_pg, int(ld), int(nb)))
+
+    def lpc_target_device(self, d_g: int, d_t: int, nb: int, steps: int = 40, shift: float = 0.5):
+        self._check(self.L.rsg_lpc_target_f32(self.h, d_g, d_t, int(nb), int(steps), float(shift)))
+
+    def lpc_train_step(self, d_g: int, d_t: int, nsamp: int, lr: float = 0.0, want_grads: bool = False):
+        loss = _c_dbl()
+        grads = np.zeros(self.lpc_param_count(), dtype=np.float32) if want_grads else None
+        self._check(self.L.rsg_lpc_train_step(self.h, d_g, d_t, int(nsamp), float(lr), ctypes.byref(loss),
+                                              grads.ctypes.data if grads is not None else None))
+        return loss.value, grads
+
+    def lpc_reset_optimizer(self):
+        self._check(self.L.rsg_lpc_reset_optimizer(self.h))
+
+    def band_residuals_device(self, d_g: int):
+        self._check(self.L.rsg_band_residuals_f32(self.h, d_g))
+
+    def lpc_fallbacks(self) -> int:
+        c = _c_i64()
+        self._check(self.L.rsg_lpc_fallbacks(self.h, ctypes.byref(c)))
+        return c.value
 
     def mix_pulay(self):
         self._check(self.L.rsg_mix_pulay(self.h))

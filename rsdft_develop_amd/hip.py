@@ -14,7 +14,7 @@ import numpy as np
 from . import gpu
 
 _rt = None
-_H2D, _D2H = 1, 2
+_H2D, _D2H, _D2D = 1, 2, 3
 
 
 def runtime():
@@ -43,6 +43,11 @@ def _check(rc: int, what: str):
     if rc != 0:
         msg = runtime().hipGetErrorString(rc)
         raise RuntimeError(f"{what}: {msg.decode() if msg else rc}")
+
+
+def copy_d2d(dst: int, src: int, nbytes: int):
+    """Synchronous device-to-device copy between raw device addresses."""
+    _check(runtime().hipMemcpy(dst, src, int(nbytes), _D2D), "hipMemcpy D2D")
 
 
 class DeviceArray:

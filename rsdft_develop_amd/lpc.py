@@ -1,0 +1,153 @@
+"""Learned CG preconditioner: parameters, training data and training loop.
+
+The network and its kernels live in the library (csrc/lpc.hip); this module
+drives them through the C ABI:
+
+- `init_params` / `pack` / `unpack`: the flat FP32 parameter vector in the
+  library's order W1[16][2][27] b1[16] W2[16][16][27] b2[16] W3[1][16][27]
+  b3[1] (W[co][ci][tap], tap = d3*9 + d2*3 + d1);
+- `make_dataset`: residuals g = -2 (H u - e u) of the plain method's own
+  bands, taken from a few plain SCF iterations on perturbed cells, paired
+  with targets t = A^-1 g, A = -1/2 lap_7 + (Vloc - min Vloc + shift) (a
+  potential-aware, shifted smoother solved to convergence);
+- `train`: Adam over mini-batches, all on the GPU.
+
+No counterpart in the reference: it replaces precond_hprsdft
+(cgpc_hprsdft_module.f90:133-361) when CG runs with pc_kind = 5.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+
+import numpy as np
+
+CH = 16
+SHAPES = [("W1", (CH, 2, 27)), ("b1", (CH,)), ("W2", (CH, CH, 27)), ("b2", (CH,)), ("W3", (1, CH, 27)), ("b3", (1,))]
+NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-scaled conv weights, zero biases, a small last layer (the net
+    starts close to P g = g)."""
+    rng = np.random.default_rng(seed)
+    d = {
+        "W1": rng.standard_normal((CH, 2, 27)) * np.sqrt(2.0 / (2 * 27)),
+        "b1": np.zeros(CH),
+        "W2": rng.standard_normal((CH, CH, 27)) * np.sqrt(2.0 / (CH * 27)),
+        "b2": np.zeros(CH),
+        "W3": rng.standard_normal((1, CH, 27)) * 1e-2,
+        "b3": np.zeros(1),
+    }
+    return pack(d)
+
+
+def pack(d: dict) -> np.ndarray:
+    return np.concatenate([np.asarray(d[k], dtype=np.float64).reshape(-1) for k, _ in SHAPES]).astype(np.float32)
+
+
+def unpack(p: np.ndarray) -> dict:
+    out, o = {}, 0
+    for k, s in SHAPES:
+        n = int(np.prod(s))
+        out[k] = np.asarray(p[o:o + n], dtype=np.float64).reshape(s)
+        o += n
+    return out
+
+
+def to_oracle(p: np.ndarray) -> dict:
+    """Parameters in the (co, ci, 3, 3, 3) layout of the CPU restatement."""
+    d = unpack(p)
+    return {k: (v.reshape(v.shape[0], v.shape[1], 3, 3, 3) if k.startswith("W") else v) for k, v in d.items()}
+
+
+@dataclasses.dataclass
+class Dataset:
+    g: object       # hip.DeviceArray (N, nsamp) FP32 residuals
+    t: object       # hip.DeviceArray (N, nsamp) FP32 targets
+    nsamp: int
+    npoints: int
+    cells: int
+    seconds: float
+
+
+def make_dataset(ctx_factory, cells, scf_iters: int = 2, target_steps: int = 40, shift: float = 0.5,
+                 max_samples: int | None = None, log=None) -> Dataset:
+    """Residual/target pairs from plain SCF runs on `cells` (all on one grid
+    shape).  ctx_factory() returns a fresh gpu.Context.  Each cell gives
+    nband samples per SCF iteration."""
+    from . import hip, scf
+
+    N = cells[0].npoints
+    per_cell = cells[0].nband * scf_iters
+    total = per_cell * len(cells)
+    if max_samples:
+        total = min(total, max_samples)
+    g_all = hip.DeviceArray((N, total), dtype=np.float32)
+    t_all = hip.DeviceArray((N, total), dtype=np.float32)
+    t0 = time.perf_counter()
+    filled = 0
+    for ic, cell in enumerate(cells):
+        if filled >= total:
+            break
+        assert cell.npoints == N
+        ctx = ctx_factory()
+        run = scf.SCF(cell, scf.ScfParams(ekbt=0.01), ctx=ctx)
+        e_prev = None
+        for it in range(scf_iters):
+            rec = run.step(it + 1, e_prev)
+            e_prev = rec.etot
+            run.update_potential()
+            nb = min(cell.nband, total - filled)
+            if nb <= 0:
+                break
+            # residuals of the current bands (FP32) into a staging block, then
+            # targets with the current potential
+            stage = hip.DeviceArray((N, cell.nband), dtype=np.float32)
+            ctx.band_residuals_device(stage.ptr)
+            g_ptr = g_all.ptr + filled * N * 4
+            t_ptr = t_all.ptr + filled * N * 4
+            hip.copy_d2d(g_ptr, stage.ptr, nb * N * 4)
+            ctx.lpc_target_device(g_ptr, t_ptr, nb, steps=target_steps, shift=shift)
+            stage.free()
+            filled += nb
+        ctx.close()
+        if log:
+            log(f"dataset: cell {ic + 1}/{len(cells)}, {filled} samples, {time.perf_counter() - t0:.1f} s")
+    return Dataset(g_all, t_all, filled, N, len(cells), time.perf_counter() - t0)
+
+
+def train(ctx, data: Dataset, params: np.ndarray, epochs: int = 4, batch: int = 16, lr: float = 2e-3,
+          seed: int = 0, log=None) -> tuple:
+    """Adam over shuffled mini-batches of consecutive samples.  Returns the
+    trained parameters and the per-epoch mean loss."""
+    ctx.lpc_set_params(params)
+    ctx.lpc_reset_optimizer()
+    rng = np.random.default_rng(seed)
+    N = data.npoints
+    nbatch = data.nsamp // batch
+    hist = []
+    for ep in range(epochs):
+        order = rng.permutation(nbatch)
+        tot = 0.0
+        t0 = time.perf_counter()
+        for b in order:
+            off = int(b) * batch * N * 4
+            loss, _ = ctx.lpc_train_step(data.g.ptr + off, data.t.ptr + off, batch, lr=lr)
+            tot += loss
+        hist.append(tot / max(nbatch, 1))
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: loss {hist[-1]:.4e} ({time.perf_counter() - t0:.1f} s)")
+    return ctx.lpc_get_params(), hist
+
+
+def save(path: str, params: np.ndarray, meta: dict | None = None):
+    np.savez(path, params=np.asarray(params, dtype=np.float32), **{k: np.asarray(v) for k, v in (meta or {}).items()})
+
+
+def load(path: str) -> np.ndarray:
+    with np.load(path, allow_pickle=False) as z:
+        p = np.asarray(z["params"], dtype=np.float32)
+    if p.size != NPARAM:
+        raise ValueError(f"{path}: {p.size} parameters, expected {NPARAM}")
+    return p

@@ -30,6 +30,8 @@ class ScfParams:
     max_iter: int = 200     # DITER
     init_width: float = 1.0  # width (bohr) of the initial atomic Gaussians
     seed: int = 0           # N(0,1) initial bands
+    mixed: bool = False     # FP32 CG/hpsi while dE/atom >= fp64_below, FP64 after
+    fp64_below: float = 1e-4
 
 
 @dataclasses.dataclass
@@ -42,6 +44,7 @@ class ScfIteration:
     dharris_per_atom: float  # |Etot - Eharris| / natom (scf_module.f90:335-339)
     esp: np.ndarray
     times: dict
+    precision: int = 64     # CG/hpsi precision of this iteration
 
 
 @dataclasses.dataclass
@@ -120,14 +123,14 @@ class SCF:
         c.gram_schmidt(1, cell.nband)   # main.f90:392
         c.mix_init(self.p.mmix, self.p.beta)
 
-    def step(self, it: int, e_prev: float | None) -> ScfIteration:
+    def step(self, it: int, e_prev: float | None, precision: int = 64) -> ScfIteration:
         c, p = self.ctx, self.p
         t = {}
         t0 = time.perf_counter()
         c.subspace_diag()
         t["sd"] = time.perf_counter() - t0
         t0 = time.perf_counter()
-        c.conjugate_gradient(p.ncg, p.pc_kind, p.mloop)
+        c.conjugate_gradient(p.ncg, p.pc_kind, p.mloop, precision=precision)
         t["cg"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         c.gram_schmidt()
@@ -149,7 +152,7 @@ class SCF:
         dh = abs(etot - eharris) / self.cell.natom
         fene = etot - entropy_fd(occ, p.ekbt)
         self.occ = occ
-        return ScfIteration(it, etot, eharris, fene, de, dh, esp, t)
+        return ScfIteration(it, etot, eharris, fene, de, dh, esp, t, precision)
 
     def update_potential(self) -> float:
         t0 = time.perf_counter()
@@ -165,9 +168,13 @@ class SCF:
         e_prev = None
         conv = False
         w0 = time.perf_counter()
+        prec = 32 if self.p.mixed else 64
         for it in range(1, self.p.max_iter + 1):
-            rec = self.step(it, e_prev)
-            conv = rec.de_per_atom < self.p.etol_per_atom
+            rec = self.step(it, e_prev, prec)
+            # FP32 is never trusted for the stop test: switch first
+            if prec == 32 and rec.de_per_atom < self.p.fp64_below:
+                prec = 64
+            conv = rec.precision == 64 and rec.de_per_atom < self.p.etol_per_atom
             if not conv and it < self.p.max_iter:
                 rec.times["pot"] = self.update_potential()
             its.append(rec)

@@ -117,13 +117,19 @@ def test_field_errors(ctx):
         ctx.mix_init(0, 0.3)
 
 
-def test_si8_scf_matches_oracle():
-    from rsdft_develop_amd import scf
-
+@pytest.fixture(scope="module")
+def si8_ref():
     cell = systems.silicon_supercell(1, ngrid_per_cell=36)
-    print("case: Si8 36^3 SCF, EKBT 0.01, etol 1e-10 Ha/atom", flush=True)
     ref = so.run_scf(cell, etol_per_atom=1e-10, ekbt=0.01, max_iter=80)
     assert ref["converged"]
+    return cell, ref
+
+
+def test_si8_scf_matches_oracle(si8_ref):
+    from rsdft_develop_amd import scf
+
+    cell, ref = si8_ref
+    print("case: Si8 36^3 SCF, EKBT 0.01, etol 1e-10 Ha/atom", flush=True)
     run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=1e-10, max_iter=80))
     res = run.run()
     assert res.converged
@@ -134,3 +140,17 @@ def test_si8_scf_matches_oracle():
     assert abs(res.etot - ref["etot"]) / cell.natom < 1e-7
     assert abs(res.fene - ref["fene"]) / cell.natom < 1e-7
     np.testing.assert_allclose(np.sort(res.esp), np.sort(ref["esp"]), rtol=0, atol=1e-8)
+
+
+def test_si8_mixed_precision_scf_reaches_fp64_result(si8_ref):
+    """FP32 CG/hpsi until dE/atom < 1e-4, then FP64: same converged state."""
+    from rsdft_develop_amd import scf
+
+    cell, ref = si8_ref
+    print("case: Si8 36^3 mixed-precision SCF", flush=True)
+    res = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=1e-10, max_iter=80, mixed=True)).run()
+    assert res.converged
+    precs = [r.precision for r in res.iterations]
+    assert precs[0] == 32 and precs[-1] == 64 and precs == sorted(precs)
+    assert abs(res.etot - ref["etot"]) / cell.natom < 1e-7
+    np.testing.assert_allclose(np.sort(res.esp), np.sort(ref["esp"]), rtol=0, atol=1e-6)

@@ -116,3 +116,36 @@ def test_inner_loop_iteration_matches_oracle(setup):
         ctx.gram_schmidt()
         u = ham.gram_schmidt(u, "lusl")
     np.testing.assert_allclose(ctx.esp_calc(), ham.esp_calc(u), atol=1e-8)
+
+
+def test_fp32_conjugate_gradient_tracks_fp64(setup):
+    """The FP32-storage CG follows the FP64 one to FP32 accuracy in one call,
+    and FP32 iterations followed by FP64 ones reach the FP64 fixed point."""
+    cell, ham, ctx, u0 = setup
+    q = ham.gram_schmidt(u0, "lusl")
+    u1, _ = ham.subspace_diag(q)
+    _load(ctx, u1)
+    e32, r32 = ctx.conjugate_gradient(ncg=4, pc_kind=4, mloop=3, precision=32)
+    _, e64, r64 = ham.conjugate_gradient(u1, ncg=4, pc_kind=4)
+    scale = np.abs(e64).max()
+    assert np.abs(e32 - e64).max() < 5e-5 * scale
+    np.testing.assert_allclose(r32, r64, rtol=5e-2, atol=1e-6)
+    # inner loop at fixed potential: 8 FP32 rounds, then FP64 rounds
+    _load(ctx, q)
+    for _ in range(8):
+        ctx.subspace_diag()
+        ctx.conjugate_gradient(ncg=4, precision=32)
+        ctx.gram_schmidt()
+    for _ in range(12):
+        ctx.subspace_diag()
+        ctx.conjugate_gradient(ncg=4)
+        ctx.gram_schmidt()
+    e_mixed = np.sort(ctx.esp_calc())
+    u = q
+    for _ in range(20):
+        u, _ = ham.subspace_diag(u)
+        u, _, _ = ham.conjugate_gradient(u, ncg=4)
+        u = ham.gram_schmidt(u, "lusl")
+    e_ref = np.sort(ham.esp_calc(u))
+    # lowest 12 states: converged in both runs
+    np.testing.assert_allclose(e_mixed[:12], e_ref[:12], atol=1e-8)
