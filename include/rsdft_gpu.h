@@ -123,7 +123,8 @@ int rsg_autotune_hpsi(rsg_ctx *ctx, int precision, int nb, int reps,
                       float *best_ms);
 
 /* Device-resident wavefunction store, the counterpart of
- * wf_module::unk(ML_0:ML_1, MB, k, s) (wf_module.f90:131). */
+ * wf_module::unk(ML_0:ML_1, MB, k, s) (wf_module.f90:131).  Upload and
+ * download move nb bands starting at store band n0 (0-based). */
 int rsg_alloc_bands(rsg_ctx *ctx, int mb);
 int rsg_upload_bands(rsg_ctx *ctx, const double *u, int64_t ld, int n0,
                      int nb);
@@ -207,6 +208,12 @@ int rsg_density_integrals(rsg_ctx *ctx, int field, double *e_loc,
  * mix_pulay call takes RHO_OUT as the output of the latest input and writes
  * the next input density into RHO. */
 int rsg_mix_init(rsg_ctx *ctx, int mmix, double beta);
+/* The same with the reference's IMIX choice: 10 = Pulay on the density in
+ * real space (pulay_mixing, the default), 30 = Pulay on its Fourier
+ * coefficients with Kerker weights (pulay_g_mixing,
+ * mixing_pulay_module.f90:124-337; mmix <= 4), which damps the
+ * long-wavelength charge sloshing of large cells. */
+int rsg_mix_init_imix(rsg_ctx *ctx, int mmix, double beta, int imix);
 int rsg_mix_pulay(rsg_ctx *ctx);
 
 /* ---- learned CG preconditioner (pc_kind = 5 in rsg_conjugate_gradient*).

@@ -9,6 +9,7 @@ Restated routines (reference file:line):
   density                    density_module.f90:120-151
   Fermi-Dirac occupations    fermi_module.f90 (mp_version = -1)
   Pulay mixing               mixing/mixing_pulay_module.f90:29-121, mixing_module.f90:145-297
+  Pulay mixing in G space    mixing/mixing_pulay_module.f90:124-227 (IMIX 30), Kerker weights :275-337
   energies                   total_energy_module.f90:320-352, 395-428
   SCF order                  scf_module.f90:183-481
 The Ewald sum is the standard erfc/Gaussian split (the value ewald_module.f90
@@ -227,8 +228,62 @@ class Pulay:
         return self.xin[m - 1].copy()
 
 
+def kerker_weights(cell, bmix=None):
+    """(wq, wq2) on the FFT grid (numpy order): wq = (G^2 + b1^2)/G^2 weighs
+    residuals, wq2 = G^2/(G^2 + b2^2) damps long-wavelength updates; both 1
+    at G = 0 (get_Kerker_weights, mixing_pulay_module.f90:275-337)."""
+    bb = reciprocal(cell.aa)
+    ms = [np.fft.fftfreq(n, 1.0 / n) for n in cell.ngrid]
+    M1, M2, M3 = np.meshgrid(*ms, indexing="ij")
+    G = M1[..., None] * bb[:, 0] + M2[..., None] * bb[:, 1] + M3[..., None] * bb[:, 2]
+    gg = np.sum(G * G, axis=-1)
+    nz = gg > 0
+    gmin, gmax = gg[nz].min(), gg[nz].max()
+    b1 = 19.0 * gmin * gmax / (gmax - 20.0 * gmin)
+    b2 = ((1.5 * 0.529177) ** 2 if bmix is None else bmix) ** 2
+    wq = np.ones_like(gg)
+    wq2 = np.ones_like(gg)
+    wq[nz] = (gg[nz] + b1) / gg[nz]
+    wq2[nz] = gg[nz] / (gg[nz] + b2)
+    return wq, wq2
+
+
+class PulayG:
+    """pulay_g_mixing (mixing_pulay_module.f90:124-227): Pulay on the Fourier
+    coefficients of the density, residual metric wq, update damped by wq2."""
+
+    def __init__(self, cell, rho_in, mmix, beta):
+        self.cell, self.m, self.beta, self.count = cell, mmix, beta, 0
+        self.wq, self.wq2 = kerker_weights(cell)
+        shp = (mmix,) + tuple(cell.ngrid)
+        self.xin = np.zeros(shp, dtype=np.complex128)
+        self.xou = np.zeros(shp, dtype=np.complex128)
+        self.xin[mmix - 1] = np.fft.fftn(to_3d(rho_in, cell.ngrid))
+
+    def mix(self, f):
+        m, b = self.m, self.beta
+        self.xou[m - 1] = np.fft.fftn(to_3d(f, self.cell.ngrid))
+        self.count += 1
+        m0 = min(self.count, m)
+        if m == 1 or m0 < m:
+            self.xin[:-1] = self.xin[1:].copy()
+            self.xou[:-1] = self.xou[1:].copy()
+            self.xin[m - 1] = self.xin[m - 1] + b * self.wq2 * (self.xou[m - 1] - self.xin[m - 1])
+        else:
+            R = (self.xou[m - m0:] - self.xin[m - m0:]).reshape(m0, -1)
+            A = np.real(np.conj(R) @ (R * self.wq.reshape(1, -1)).T)
+            c = np.linalg.solve(A, np.ones(m0))
+            c /= c.sum()
+            X = np.tensordot(c, self.xin[m - m0:], axes=1)
+            Y = np.tensordot(c, self.xou[m - m0:], axes=1)
+            self.xin[:-1] = self.xin[1:].copy()
+            self.xou[:-1] = self.xou[1:].copy()
+            self.xin[m - 1] = X + b * self.wq2 * (Y - X)
+        return to_flat(np.real(np.fft.ifftn(self.xin[m - 1])))
+
+
 def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-8, max_iter=200,
-            init_width=1.0, seed=0, gs_variant="t", bands=None, callback=None):
+            init_width=1.0, seed=0, gs_variant="t", bands=None, callback=None, imix=10):
     """Plain FP64 SCF in the reference's order; returns a dict of results.
 
     Stops when the Kohn-Sham energy of the output density changes by less
@@ -249,7 +304,7 @@ def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-
     ham = pyoracle.Ham(cell, vloc=vloc)
     u = bands if bands is not None else systems.random_bands(cell.npoints, cell.nband, seed=seed)
     u = ham.gram_schmidt(u, gs_variant)
-    mixer = Pulay(rho, mmix, beta)
+    mixer = PulayG(cell, rho, mmix, beta) if imix == 30 else Pulay(rho, mmix, beta)
     hist = []
     e_prev = None
     for it in range(1, max_iter + 1):

@@ -71,16 +71,18 @@ def build_fortran(verbose: bool = False) -> str | None:
     moddir = os.path.join(BUILD, "mod")
     os.makedirs(moddir, exist_ok=True)
     steps = [
-        [fc, "-O2", "-fPIC", "-shared", "-module-dir", moddir, "-o", out + ".tmp", src,
-         f"-L{LIBDIR}", "-lrsdft_gpu", f"-Wl,-rpath,{LIBDIR}"],
-        [fc, "-O2", "-I", moddir, "-o", exe + ".tmp", prog, out + ".tmp",
-         f"-L{LIBDIR}", "-lrsdft_gpu", f"-Wl,-rpath,{LIBDIR}"],
+        [fc, "-O2", "-fPIC", "-shared", "-Wl,-soname,librsdft_gpu_f.so", "-module-dir", moddir, "-o", out + ".tmp",
+         src, f"-L{LIBDIR}", "-lrsdft_gpu", "-Wl,-rpath,$ORIGIN"],
+        [fc, "-O2", "-I", moddir, "-o", exe + ".tmp", prog, f"-L{LIBDIR}", "-lrsdft_gpu_f", "-lrsdft_gpu",
+         "-Wl,-rpath,$ORIGIN"],
     ]
-    for cmd in steps:
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(out + ".tmp", out)
+    r = subprocess.run(steps[0], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran build failed: {' '.join(steps[0])}\n{r.stdout}\n{r.stderr}")
+    os.replace(out + ".tmp", out)  # the program links the module library by its final name
+    r = subprocess.run(steps[1], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran build failed: {' '.join(steps[1])}\n{r.stdout}\n{r.stderr}")
     os.replace(exe + ".tmp", exe)
     if verbose:
         print(f"built {out} and {exe}")

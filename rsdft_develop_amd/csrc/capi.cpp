@@ -436,7 +436,13 @@ int rsg_density_integrals(rsg_ctx *ctx, int field, double *e_loc, double *e_ion)
 int rsg_mix_init(rsg_ctx *ctx, int mmix, double beta) {
   if (!ctx) return RSG_ERR_ARG;
   RSG_HIP(hipSetDevice(ctx->device));
-  return rsg::scf_mix_init(ctx, mmix, beta);
+  return rsg::scf_mix_init(ctx, mmix, beta, 10);
+}
+
+int rsg_mix_init_imix(rsg_ctx *ctx, int mmix, double beta, int imix) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::scf_mix_init(ctx, mmix, beta, imix);
 }
 
 int rsg_mix_pulay(rsg_ctx *ctx) {

@@ -26,6 +26,11 @@ struct ScfState {
   double beta = 0.3;
   std::vector<DevBuf> xin, xou;
   DevBuf res;  // mmix residuals Xou_i - Xin_i, N doubles each
+  // IMIX 30: Pulay on Fourier coefficients (complex histories), Kerker
+  // weights wq (residual metric) and wq2 (update damping)
+  int imix = 10;
+  std::vector<DevBuf> gxin, gxou;
+  DevBuf wq, wq2, gpart;
   // GTH local parameters per species
   std::vector<double> zps, rloc, cloc;
   std::vector<double> frac;
@@ -106,6 +111,12 @@ __global__ void k_vion_g(GridG g, int natom, const double *__restrict__ frac, co
 __global__ void k_real_part(const double2 *__restrict__ z, double *__restrict__ out, int64_t N) {
   const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j < N) out[j] = z[j].x;
+}
+
+// dst = s * src (complex)
+__global__ void k_zscale(const double2 *__restrict__ src, double2 *__restrict__ dst, double sc, int64_t N) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < N) dst[j] = make_double2(sc * src[j].x, sc * src[j].y);
 }
 
 __global__ void k_to_complex(const double *__restrict__ r, double2 *__restrict__ z, int64_t N) {
@@ -217,6 +228,78 @@ __global__ void k_gauss_rho(GridG g, const double *__restrict__ aa, int natom, c
 __global__ void k_sub(double *__restrict__ y, const double *__restrict__ a, const double *__restrict__ b, int64_t N) {
   const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j < N) y[j] = a[j] - b[j];
+}
+
+// x += beta * w2 * (y - x) on complex arrays
+__global__ void k_gmix_step(double2 *__restrict__ x, const double2 *__restrict__ y, const double *__restrict__ w2,
+                            double beta, int64_t N) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const double f = beta * w2[j];
+  x[j].x += f * (y[j].x - x[j].x);
+  x[j].y += f * (y[j].y - x[j].y);
+}
+
+struct GSlots {
+  const double2 *xin[4];
+  const double2 *xou[4];
+};
+
+// per-block partial sums of Re(conj(R_i) R_j) wq over the m0 residuals
+// R_i = xou_i - xin_i, pairs (i, j <= i) in row order
+template <int M0>
+__global__ __launch_bounds__(256) void k_gram_g(GSlots sl, const double *__restrict__ wq, int64_t N,
+                                                double *__restrict__ part) {
+  constexpr int NP = M0 * (M0 + 1) / 2;
+  __shared__ double red[4][NP];
+  double acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = 0.0;
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < N; j += int64_t(gridDim.x) * blockDim.x) {
+    double rr[M0], ri[M0];
+#pragma unroll
+    for (int i = 0; i < M0; ++i) {
+      rr[i] = sl.xou[i][j].x - sl.xin[i][j].x;
+      ri[i] = sl.xou[i][j].y - sl.xin[i][j].y;
+    }
+    const double w = wq[j];
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < M0; ++i)
+#pragma unroll
+      for (int k = 0; k <= i; ++k) acc[q++] += w * (rr[i] * rr[k] + ri[i] * ri[k]);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    double v = acc[q];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wid][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NP) part[int64_t(blockIdx.x) * NP + threadIdx.x] =
+      red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+struct GCoef {
+  double c[4];
+};
+
+// out = X + beta w2 (Y - X), X = sum c_i xin_i, Y = sum c_i xou_i
+__global__ void k_gmix_combine(GSlots sl, GCoef cf, int m0, const double *__restrict__ w2, double beta, int64_t N,
+                               double2 *__restrict__ out) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  double xr = 0, xi = 0, yr = 0, yi = 0;
+  for (int i = 0; i < m0; ++i) {
+    xr += cf.c[i] * sl.xin[i][j].x;
+    xi += cf.c[i] * sl.xin[i][j].y;
+    yr += cf.c[i] * sl.xou[i][j].x;
+    yi += cf.c[i] * sl.xou[i][j].y;
+  }
+  const double f = beta * w2[j];
+  out[j] = make_double2(xr + f * (yr - xr), xi + f * (yi - xi));
 }
 
 __global__ void k_fill_value(double *__restrict__ y, double v, int64_t N) {
@@ -578,30 +661,182 @@ int scf_density_integrals(rsg_ctx *ctx, int field, double *eloc, double *eion) {
   return RSG_OK;
 }
 
+namespace {
+
+// Fourier coefficients FFT(r) / N into a complex history slot
+int to_g(rsg_ctx *ctx, const double *r, double2 *slot) {
+  ScfState *st = ctx->scf;
+  const int64_t N = ctx->N;
+  hipLaunchKernelGGL(k_to_complex, dim3(nblk(N)), dim3(kT), 0, ctx->stream, r, st->zbuf.as<double2>(), N);
+  int rc = fft(ctx, HIPFFT_FORWARD);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_zscale, dim3(nblk(N)), dim3(kT), 0, ctx->stream, st->zbuf.as<double2>(), slot,
+                     1.0 / double(N), N);
+  return RSG_OK;
+}
+
+// Kerker weights of get_Kerker_weights (mixing_pulay_module.f90:275-337)
+int kerker_weights(rsg_ctx *ctx) {
+  ScfState *st = ctx->scf;
+  const int64_t N = ctx->N;
+  const int n1 = ctx->n[0], n2 = ctx->n[1], n3 = ctx->n[2];
+  std::vector<double> gg(static_cast<size_t>(N));
+  double gmin = 1e300, gmax = -1e300;
+  auto fr = [](int i, int n) {
+    int j = i - n;
+    return j < -(n - 1) / 2 ? i : j;
+  };
+  for (int i3 = 0; i3 < n3; ++i3)
+    for (int i2 = 0; i2 < n2; ++i2)
+      for (int i1 = 0; i1 < n1; ++i1) {
+        const int j1 = fr(i1, n1), j2 = fr(i2, n2), j3 = fr(i3, n3);
+        const double gx = j1 * st->bb[0] + j2 * st->bb[3] + j3 * st->bb[6];
+        const double gy = j1 * st->bb[1] + j2 * st->bb[4] + j3 * st->bb[7];
+        const double gz = j1 * st->bb[2] + j2 * st->bb[5] + j3 * st->bb[8];
+        const double g2 = gx * gx + gy * gy + gz * gz;
+        gg[size_t(i1 + int64_t(n1) * (i2 + int64_t(n2) * i3))] = g2;
+        if (j1 == 0 && j2 == 0 && j3 == 0) continue;
+        gmin = std::min(gmin, g2);
+        gmax = std::max(gmax, g2);
+      }
+  const double b1 = 19.0 * gmin * gmax / (gmax - 20.0 * gmin);
+  const double b2 = std::pow((1.5 * 0.529177) * (1.5 * 0.529177), 2.0);
+  std::vector<double> w(static_cast<size_t>(N)), w2(static_cast<size_t>(N));
+  for (int64_t i = 0; i < N; ++i) {
+    const double g2 = gg[size_t(i)];
+    w[size_t(i)] = g2 == 0.0 ? 1.0 : (g2 + b1) / g2;
+    w2[size_t(i)] = g2 == 0.0 ? 1.0 : g2 / (g2 + b2);
+  }
+  if (st->wq.ensure(size_t(N) * 8) || st->wq2.ensure(size_t(N) * 8))
+    return ctx->fail(RSG_ERR_HIP, "mix_init: weight allocation failed");
+  RSG_HIP(hipMemcpyAsync(st->wq.p, w.data(), size_t(N) * 8, hipMemcpyHostToDevice, ctx->stream));
+  RSG_HIP(hipMemcpyAsync(st->wq2.p, w2.data(), size_t(N) * 8, hipMemcpyHostToDevice, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+}  // namespace
+
 // init_mixing (mixing_module.f90:120-126): history slot mmix holds rho_in
-int scf_mix_init(rsg_ctx *ctx, int mmix, double beta) {
+// (IMIX 10, real space) or its Fourier coefficients (IMIX 30).
+int scf_mix_init(rsg_ctx *ctx, int mmix, double beta, int imix) {
   int rc = ensure_state(ctx);
   if (rc) return rc;
   if (mmix < 1 || mmix > 32 || !(beta > 0.0 && beta <= 1.0))
     return ctx->fail(RSG_ERR_ARG, "mix_init: mmix in 1..32 and 0 < beta <= 1");
+  if (imix != 10 && imix != 30) return ctx->fail(RSG_ERR_ARG, "mix_init: imix must be 10 or 30");
+  if (imix == 30 && mmix > 4) return ctx->fail(RSG_ERR_ARG, "mix_init: imix 30 takes mmix <= 4");
   ScfState *st = ctx->scf;
   for (auto &b : st->xin) b.release();
   for (auto &b : st->xou) b.release();
-  st->xin.assign(size_t(mmix), DevBuf());
-  st->xou.assign(size_t(mmix), DevBuf());
+  for (auto &b : st->gxin) b.release();
+  for (auto &b : st->gxou) b.release();
+  st->xin.clear();
+  st->xou.clear();
+  st->gxin.clear();
+  st->gxou.clear();
+  st->imix = imix;
   const size_t n8 = size_t(ctx->N) * 8;
-  for (int k = 0; k < mmix; ++k)
-    if (st->xin[size_t(k)].ensure(n8) || st->xou[size_t(k)].ensure(n8))
-      return ctx->fail(RSG_ERR_HIP, "mix_init: history allocation failed");
-  if (st->res.ensure(n8 * size_t(mmix)) || st->occ.ensure(size_t(mmix) * mmix * 8 + 64 * 8))
-    return ctx->fail(RSG_ERR_HIP, "mix_init: residual allocation failed");
+  if (imix == 10) {
+    st->xin.assign(size_t(mmix), DevBuf());
+    st->xou.assign(size_t(mmix), DevBuf());
+    for (int k = 0; k < mmix; ++k)
+      if (st->xin[size_t(k)].ensure(n8) || st->xou[size_t(k)].ensure(n8))
+        return ctx->fail(RSG_ERR_HIP, "mix_init: history allocation failed");
+    if (st->res.ensure(n8 * size_t(mmix)) || st->occ.ensure(size_t(mmix) * mmix * 8 + 64 * 8))
+      return ctx->fail(RSG_ERR_HIP, "mix_init: residual allocation failed");
+  } else {
+    st->gxin.assign(size_t(mmix), DevBuf());
+    st->gxou.assign(size_t(mmix), DevBuf());
+    for (int k = 0; k < mmix; ++k)
+      if (st->gxin[size_t(k)].ensure(2 * n8) || st->gxou[size_t(k)].ensure(2 * n8))
+        return ctx->fail(RSG_ERR_HIP, "mix_init: history allocation failed");
+    if (st->gpart.ensure(size_t(1024) * 10 * 8)) return ctx->fail(RSG_ERR_HIP, "mix_init: allocation failed");
+    if ((rc = kerker_weights(ctx))) return rc;
+  }
   st->mmix = mmix;
   st->beta = beta;
   st->count = 0;
-  RSG_HIP(hipMemcpyAsync(st->xin[size_t(mmix - 1)].p, st->rho.p, n8, hipMemcpyDeviceToDevice, ctx->stream));
+  if (imix == 10) {
+    RSG_HIP(hipMemcpyAsync(st->xin[size_t(mmix - 1)].p, st->rho.p, n8, hipMemcpyDeviceToDevice, ctx->stream));
+  } else if ((rc = to_g(ctx, st->rho.as<double>(), st->gxin[size_t(mmix - 1)].as<double2>()))) {
+    return rc;
+  }
   RSG_HIP(hipStreamSynchronize(ctx->stream));
   return RSG_OK;
 }
+
+namespace {
+
+// pulay_g_mixing (mixing_pulay_module.f90:124-227) on f = rho_out
+int mix_pulay_g(rsg_ctx *ctx) {
+  ScfState *st = ctx->scf;
+  const int mm = st->mmix;
+  const int64_t N = ctx->N;
+  const size_t n16 = size_t(N) * 16;
+  int rc = to_g(ctx, st->rho_out.as<double>(), st->gxou[size_t(mm - 1)].as<double2>());
+  if (rc) return rc;
+  st->count += 1;
+  const int m0 = std::min(st->count, mm);
+  auto rotate = [&](std::vector<DevBuf> &h) -> int {
+    DevBuf first = h[0];
+    for (int i = 1; i < mm; ++i) h[size_t(i - 1)] = h[size_t(i)];
+    h[size_t(mm - 1)] = first;
+    if (mm > 1 && hipMemcpyAsync(h[size_t(mm - 1)].p, h[size_t(mm - 2)].p, n16, hipMemcpyDeviceToDevice, ctx->stream))
+      return RSG_ERR_HIP;
+    return RSG_OK;
+  };
+  if (mm == 1 || m0 < mm) {
+    if (rotate(st->gxin) || rotate(st->gxou)) return ctx->fail(RSG_ERR_HIP, "mix_pulay: history copy failed");
+    hipLaunchKernelGGL(k_gmix_step, dim3(nblk(N)), dim3(kT), 0, ctx->stream, st->gxin[size_t(mm - 1)].as<double2>(),
+                       st->gxou[size_t(mm - 1)].as<double2>(), st->wq2.as<double>(), st->beta, N);
+  } else {
+    const int base = mm - m0;
+    GSlots sl{};
+    for (int i = 0; i < m0; ++i) {
+      sl.xin[i] = st->gxin[size_t(base + i)].as<double2>();
+      sl.xou[i] = st->gxou[size_t(base + i)].as<double2>();
+    }
+    const int nblock = 1024, np = m0 * (m0 + 1) / 2;
+    double *part = st->gpart.as<double>();
+    switch (m0) {
+      case 1: hipLaunchKernelGGL(k_gram_g<1>, dim3(nblock), dim3(256), 0, ctx->stream, sl, st->wq.as<double>(), N, part); break;
+      case 2: hipLaunchKernelGGL(k_gram_g<2>, dim3(nblock), dim3(256), 0, ctx->stream, sl, st->wq.as<double>(), N, part); break;
+      case 3: hipLaunchKernelGGL(k_gram_g<3>, dim3(nblock), dim3(256), 0, ctx->stream, sl, st->wq.as<double>(), N, part); break;
+      default: hipLaunchKernelGGL(k_gram_g<4>, dim3(nblock), dim3(256), 0, ctx->stream, sl, st->wq.as<double>(), N, part); break;
+    }
+    std::vector<double> ph(size_t(nblock) * np);
+    RSG_HIP(hipMemcpyAsync(ph.data(), part, ph.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    RSG_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<double> A(size_t(m0) * m0), b(size_t(m0), 1.0);
+    int q = 0;
+    for (int i = 0; i < m0; ++i)
+      for (int k = 0; k <= i; ++k, ++q) {
+        double v = 0.0;
+        for (int blk = 0; blk < nblock; ++blk) v += ph[size_t(blk) * np + q];
+        A[size_t(i) + size_t(m0) * k] = A[size_t(k) + size_t(m0) * i] = v;
+      }
+    if (!solve_small(m0, A, b)) return ctx->fail(RSG_ERR_SOLVER, "mix_pulay: singular residual matrix");
+    double sb = 0.0;
+    for (double v : b) sb += v;
+    GCoef cf{};
+    for (int i = 0; i < m0; ++i) cf.c[i] = b[size_t(i)] / sb;
+    // the combination goes to zbuf, then becomes the newest input slot
+    hipLaunchKernelGGL(k_gmix_combine, dim3(nblk(N)), dim3(kT), 0, ctx->stream, sl, cf, m0, st->wq2.as<double>(),
+                       st->beta, N, st->zbuf.as<double2>());
+    if (rotate(st->gxin) || rotate(st->gxou)) return ctx->fail(RSG_ERR_HIP, "mix_pulay: history copy failed");
+    RSG_HIP(hipMemcpyAsync(st->gxin[size_t(mm - 1)].p, st->zbuf.p, n16, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  // rho = Re IFFT(Xin(mmix)) (coefficients carry the 1/N)
+  RSG_HIP(hipMemcpyAsync(st->zbuf.p, st->gxin[size_t(mm - 1)].p, n16, hipMemcpyDeviceToDevice, ctx->stream));
+  if ((rc = fft(ctx, HIPFFT_BACKWARD))) return rc;
+  hipLaunchKernelGGL(k_real_part, dim3(nblk(N)), dim3(kT), 0, ctx->stream, st->zbuf.as<double2>(),
+                     st->rho.as<double>(), N);
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+
+}  // namespace
 
 // pulay_mixing (mixing_pulay_module.f90:29-121) on f = rho_out; the mixed
 // input density lands in history slot mmix and in rho.
@@ -609,6 +844,7 @@ int scf_mix_pulay(rsg_ctx *ctx) {
   int rc = ensure_state(ctx);
   if (rc) return rc;
   ScfState *st = ctx->scf;
+  if (st->mmix >= 1 && st->imix == 30) return mix_pulay_g(ctx);
   const int mm = st->mmix;
   if (mm < 1) return ctx->fail(RSG_ERR_STATE, "mix_pulay: mixing not initialised");
   const int64_t N = ctx->N;
@@ -697,7 +933,9 @@ void scf_release(rsg_ctx *ctx) {
   for (DevBuf *b : {&st->rho, &st->rho_out, &st->vion, &st->vh, &st->vxc, &st->zbuf, &st->work, &st->occ}) b->release();
   for (auto &b : st->xin) b.release();
   for (auto &b : st->xou) b.release();
-  st->res.release();
+  for (auto &b : st->gxin) b.release();
+  for (auto &b : st->gxou) b.release();
+  for (DevBuf *b : {&st->res, &st->wq, &st->wq2, &st->gpart}) b->release();
   delete st;
   ctx->scf = nullptr;
 }

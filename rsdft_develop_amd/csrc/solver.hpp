@@ -30,7 +30,7 @@ int scf_calc_density(rsg_ctx *ctx, const double *occ);
 int scf_hartree_xc(rsg_ctx *ctx, int field, double *eh, double *exc);
 int scf_update_vloc(rsg_ctx *ctx);
 int scf_density_integrals(rsg_ctx *ctx, int field, double *eloc, double *eion);
-int scf_mix_init(rsg_ctx *ctx, int mmix, double beta);
+int scf_mix_init(rsg_ctx *ctx, int mmix, double beta, int imix);
 int scf_mix_pulay(rsg_ctx *ctx);
 void scf_release(rsg_ctx *ctx);
 }  // namespace rsg

@@ -304,12 +304,13 @@ contains
   subroutine rsdft_gpu_upload_unk(unk)
     real(8), intent(in) :: unk(:, :)
     if (mb_store /= size(unk, 2)) call rsdft_gpu_alloc_unk(size(unk, 2))
-    call check(rsg_upload_bands(ctx, unk, int(size(unk, 1), c_int64_t), 1, size(unk, 2)), "rsdft_gpu_upload_unk")
+    ! the store's band offset n0 is 0-based
+    call check(rsg_upload_bands(ctx, unk, int(size(unk, 1), c_int64_t), 0, size(unk, 2)), "rsdft_gpu_upload_unk")
   end subroutine rsdft_gpu_upload_unk
 
   subroutine rsdft_gpu_download_unk(unk)
     real(8), intent(inout) :: unk(:, :)
-    call check(rsg_download_bands(ctx, unk, int(size(unk, 1), c_int64_t), 1, size(unk, 2)), &
+    call check(rsg_download_bands(ctx, unk, int(size(unk, 1), c_int64_t), 0, size(unk, 2)), &
                "rsdft_gpu_download_unk")
   end subroutine rsdft_gpu_download_unk
 

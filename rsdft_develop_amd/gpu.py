@@ -95,6 +95,7 @@ def load_library():
         "rsg_update_vloc": (_c_int, [_vp]),
         "rsg_density_integrals": (_c_int, [_vp, _c_int, _P(_c_dbl), _P(_c_dbl)]),
         "rsg_mix_init": (_c_int, [_vp, _c_int, _c_dbl]),
+        "rsg_mix_init_imix": (_c_int, [_vp, _c_int, _c_dbl, _c_int]),
         "rsg_mix_pulay": (_c_int, [_vp]),
         "rsg_lpc_param_count": (_c_int, []),
         "rsg_lpc_set_params": (_c_int, [_vp, _vp, _c_int]),
@@ -368,8 +369,10 @@ class Context:
         self._check(self.L.rsg_density_integrals(self.h, int(field), ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
-    def mix_init(self, mmix: int = 4, beta: float = 0.3):
-        self._check(self.L.rsg_mix_init(self.h, int(mmix), float(beta)))
+    def mix_init(self, mmix: int = 4, beta: float = 0.3, imix: int = 10):
+        """IMIX 10: Pulay on the density; 30: Pulay on its Fourier
+        coefficients with Kerker weights."""
+        self._check(self.L.rsg_mix_init_imix(self.h, int(mmix), float(beta), int(imix)))
 
     # ---- learned CG preconditioner
     def lpc_param_count(self) -> int:

@@ -21,9 +21,10 @@ from . import gpu, systems
 @dataclasses.dataclass
 class ScfParams:
     ncg: int = 4            # NCG
-    pc_kind: int = 4        # IPC: 4 = hprsdft, 0 = none
+    pc_kind: int = 4        # IPC: 4 = hprsdft, 0 = none, 5 = learned (parameters set on the context)
     mloop: int = 3          # MLOOP
     mmix: int = 4           # MMIX
+    imix: int = 10          # IMIX: 10 Pulay on rho, 30 Pulay in G space with Kerker weights
     beta: float = 0.3       # BETA
     ekbt: float = 1e-5      # EKBT (Fermi-Dirac temperature, Ha)
     etol_per_atom: float = 1e-8   # stop when |Etot_k - Etot_(k-1)| / natom < etol
@@ -117,11 +118,13 @@ class SCF:
         c.init_density(self.p.init_width)
         self.eh_in, self.exc_in = c.hartree_xc(gpu.FIELD_RHO)
         c.update_vloc()
+        if self.p.pc_kind == 5:
+            c.lpc_set_potential()  # the network's potential input follows Vloc
         c.alloc_bands(cell.nband)
         u0 = bands if bands is not None else systems.random_bands(cell.npoints, cell.nband, seed=self.p.seed)
         c.upload_bands(u0)
         c.gram_schmidt(1, cell.nband)   # main.f90:392
-        c.mix_init(self.p.mmix, self.p.beta)
+        c.mix_init(self.p.mmix, self.p.beta, self.p.imix)
 
     def step(self, it: int, e_prev: float | None, precision: int = 64) -> ScfIteration:
         c, p = self.ctx, self.p
@@ -161,6 +164,8 @@ class SCF:
         c.normalize_density()
         self.eh_in, self.exc_in = c.hartree_xc(gpu.FIELD_RHO)
         c.update_vloc()
+        if self.p.pc_kind == 5:
+            c.lpc_set_potential()
         return time.perf_counter() - t0
 
     def run(self, callback=None) -> ScfResult:

@@ -5,7 +5,11 @@ the module and driver are built when AMD flang is installed.  GPU: the
 Fortran-driven SCF of Si8 (36^3, EKBT 0.01 Ha) converges to the oracle's
 fixed point.  The two runs start from different random bands, so they are
 compared at convergence (|dE| < 1e-10 Ha/atom per step): total energy within
-1e-7 Ha/atom, eigenvalues within 1e-6 Ha.
+1e-7 Ha/atom; the mean of each degenerate shell of levels within 3e-7 Ha,
+since levels are first order in the density left unconverged at the stop
+(two oracle runs from different seeds differ by 4e-8 Ha, this driver's own
+start by 1.3e-7 Ha); single levels within 5e-6 Ha (members of a degenerate
+shell stay split by ~1e-6 Ha at this convergence, depending on the start).
 """
 import os
 import re
@@ -68,4 +72,6 @@ def test_fortran_scf_matches_oracle():
     assert ref["converged"]
     assert abs(etot - ref["etot"]) / cell.natom < 1e-7
     assert abs(fene - ref["fene"]) / cell.natom < 1e-7
-    np.testing.assert_allclose(esp, np.sort(ref["esp"]), rtol=0, atol=1e-6)
+    from levels import assert_levels_match
+
+    assert_levels_match(esp, ref["esp"], shell_tol=3e-7, level_tol=5e-6)

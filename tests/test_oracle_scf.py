@@ -147,3 +147,37 @@ def test_si8_scf_converges(si8_scf):
     esp = np.sort(out["esp"])
     assert esp[1] - esp[0] > 0.05
     np.testing.assert_allclose(esp[1:7], esp[1], atol=1e-4)
+
+
+def test_kerker_weights_closed_form():
+    cell = systems.silicon_supercell(1, ngrid_per_cell=12)
+    wq, wq2 = so.kerker_weights(cell)
+    g1 = (2 * np.pi / cell.aa[0, 0]) ** 2   # |G|^2 of (1,0,0)
+    b2 = ((1.5 * 0.529177) ** 2) ** 2
+    assert wq2[0, 0, 0] == 1.0 and wq[0, 0, 0] == 1.0
+    assert wq2[1, 0, 0] == pytest.approx(g1 / (g1 + b2), rel=1e-14)
+    gmax = g1 * 3 * 6 ** 2  # corner (6,6,6) on a 12^3 grid
+    b1 = 19 * g1 * gmax / (gmax - 20 * g1)
+    assert wq[0, 0, 1] == pytest.approx((g1 + b1) / g1, rel=1e-14)
+
+
+def test_pulay_g_reduces_to_weighted_linear_mixing_and_fixed_point():
+    cell = systems.silicon_supercell(1, ngrid_per_cell=10)
+    rng = np.random.default_rng(3)
+    n = cell.npoints
+    x0 = 0.03 + 0.01 * rng.random(n)
+    mix = so.PulayG(cell, x0, 4, 0.3)
+    f = 0.02 + 0.01 * rng.random(n)
+    x1 = mix.mix(f)
+    _, wq2 = so.kerker_weights(cell)
+    want = so.to_flat(np.real(np.fft.ifftn(np.fft.fftn(so.to_3d(x0, cell.ngrid))
+                                           + 0.3 * wq2 * np.fft.fftn(so.to_3d(f - x0, cell.ngrid)))))
+    np.testing.assert_allclose(x1, want, atol=1e-14)
+    # a contraction x -> M x + c converges to its fixed point
+    c = 0.01 * rng.random(n)
+    Mop = lambda x: 0.3 * x + 0.2 * np.roll(x, 1) + c
+    x = x1
+    for _ in range(60):
+        x = mix.mix(Mop(x))
+    fixed = np.linalg.solve(np.eye(n) - 0.3 * np.eye(n) - 0.2 * np.roll(np.eye(n), 1, axis=0), c)
+    np.testing.assert_allclose(x, fixed, atol=1e-9)
