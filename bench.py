@@ -95,20 +95,33 @@ def cpu_leg(cell, vloc, target_s: float = 12.0):
                        f"scaled linearly to {cell.nband} bands ({full * 1e3:.1f} ms/call)")
 
 
-def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80):
+NBANDS = {1: 20, 2: 160, 3: 480, 4: 1100}
+
+
+def progress(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_kind: int = 4, params=None):
     """Wall seconds of a whole SCF run to dE < etol Ha/atom (the run's own
-    set-up on the device included: tables, initial density and GS)."""
+    set-up on the device included: tables, initial density and GS).  Pulay
+    mixing in G space with Kerker weights (IMIX 30), EKBT 0.01 Ha."""
     from rsdft_develop_amd import gpu, scf, systems
 
-    cell = systems.silicon_supercell(nrep, displacement=0.05, seed=0)
+    cell = systems.silicon_supercell(nrep, nband=NBANDS[nrep], displacement=0.05, seed=0)
     ctx = gpu.Context(0)
     ctx.set_grid(cell.ngrid, cell.aa, cell.md)
     ctx.build_gth_projectors(cell)
     ctx.autotune_hpsi(64, cell.nband, reps=2)
     ctx.autotune_hpsi(32, cell.nband, reps=2)
+    if params is not None:
+        ctx.lpc_set_params(params)
+    tag = f"scf Si{cell.natom} {'mixed' if mixed else 'fp64'} pc{pc_kind}"
+    progress(f"{tag}: start")
     t0 = time.perf_counter()
-    run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=etol, max_iter=max_iter, mixed=mixed), ctx=ctx)
-    res = run.run()
+    run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=etol, max_iter=max_iter, mixed=mixed, imix=30,
+                                      pc_kind=pc_kind), ctx=ctx)
+    res = run.run(callback=lambda r: progress(f"{tag}: iter {r.iter} prec {r.precision} dE/atom {r.de_per_atom:.2e}"))
     wall = time.perf_counter() - t0
     its = res.iterations
     out = dict(seconds=round(wall, 3), iterations=len(its), converged=bool(res.converged), etot=res.etot,
@@ -136,8 +149,11 @@ def main() -> int:
 
     cell = systems.silicon_supercell(2, nband=160, displacement=0.05, seed=0)
     vloc = systems.model_vloc(cell)
+    progress("hpsi FP64")
     r64 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float64)
+    progress("hpsi FP32")
     r32 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float32)
+    progress("CPU baseline")
     cpu = None if args.no_cpu else cpu_leg(cell, vloc)
     scf_out = None
     if args.scf > 0:
@@ -145,8 +161,8 @@ def main() -> int:
         mixed = scf_leg(args.scf, mixed=True)
         de = float(np.abs(np.array(mixed.pop("esp")) - np.array(plain.pop("esp"))).max())
         natom = 8 * args.scf ** 3
-        scf_out = {"workload": f"Si{natom} {36 * args.scf}^3 grid, {natom * 5 // 2} bands, EKBT 0.01 Ha, "
-                               f"to dE < 1e-6 Ha/atom",
+        scf_out = {"workload": f"Si{natom} {36 * args.scf}^3 grid, {NBANDS[args.scf]} bands, EKBT 0.01 Ha, "
+                               f"IMIX 30, to dE < 1e-6 Ha/atom",
                    "fp64": plain, "mixed": mixed,
                    "speedup_mixed_vs_fp64": round(plain["seconds"] / mixed["seconds"], 3),
                    "mixed_max_eig_diff_vs_fp64_Ha": de,

@@ -114,6 +114,11 @@ int rsg_hpsi_f64_host(rsg_ctx *ctx, const double *psi, double *hpsi,
  * counterpart (launch tuning of this implementation). */
 int rsg_set_hpsi_shape(rsg_ctx *ctx, int precision, int ry, int ty, int zlen,
                        int chunk, int bt);
+/* Order of the two passes over H psi with analytic projectors: 0 = the
+ * nonlocal kernel writes H psi and the stencil adds to it, 1 = the stencil
+ * writes and the nonlocal kernel adds.  No reference counterpart. */
+int rsg_set_hpsi_order(rsg_ctx *ctx, int precision, int order);
+int rsg_get_hpsi_order(rsg_ctx *ctx, int precision, int *order);
 /* The shape an nb-band call would use. */
 int rsg_get_hpsi_shape(rsg_ctx *ctx, int precision, int nb, int *ry, int *ty,
                        int *zlen, int *chunk, int *bt);
@@ -219,7 +224,7 @@ int rsg_mix_pulay(rsg_ctx *ctx);
 /* ---- learned CG preconditioner (pc_kind = 5 in rsg_conjugate_gradient*).
  * No reference counterpart: it replaces the kinetic smoother
  * precond_hprsdft (cgpc_hprsdft_module.f90:133-361) with a 3-layer 3x3x3
- * convolution net on the periodic grid, P g = g + s * net(g/s, v), s =
+ * convolution net on the periodic grid, P g = s * net(g/s, v), s =
  * rms(g), v the cell's normalised Vloc; bfloat16 weights and activations,
  * FP32 accumulation on MFMA.  A block whose <Pg, g> is not positive falls
  * back to precond_hprsdft.  Parameters (FP32, this order):
@@ -234,15 +239,19 @@ int rsg_lpc_apply_f64(rsg_ctx *ctx, const double *d_g, double *d_pg,
                       int64_t ld, int nb);
 int rsg_lpc_apply_f32(rsg_ctx *ctx, const float *d_g, float *d_pg,
                       int64_t ld, int nb);
-/* Training targets t = A^-1 g, A = -1/2 lap_7 + (Vloc - min Vloc + shift),
- * by `steps` CG steps (FP32, device, leading dimension N). */
+/* Training targets t = A^-1 g, A = -1/2 lap_7 + vweight (Vloc - min Vloc)
+ * + shift, by `steps` CG steps from t = g (FP32, device, leading
+ * dimension N): with vweight = 0 the converged form of the kinetic solve
+ * that precond_hprsdft truncates after mloop steps. */
 int rsg_lpc_target_f32(rsg_ctx *ctx, const float *d_g, float *d_t, int nb,
-                       int steps, double shift);
+                       int steps, double shift, double vweight);
 /* One Adam step (lr > 0) or a gradient evaluation only (lr = 0) on nsamp
  * residual/target pairs (device FP32, leading dimension N); loss = mean
- * ((P g - t)/s)^2; grads (host, may be NULL) in the parameter order. */
+ * ((P g - kappa t)/s)^2, kappa > 0 a fixed scale of the targets; grads
+ * (host, may be NULL) in the parameter order. */
 int rsg_lpc_train_step(rsg_ctx *ctx, const float *d_g, const float *d_t,
-                       int nsamp, float lr, double *loss, float *grads);
+                       int nsamp, float kappa, float lr, double *loss,
+                       float *grads);
 int rsg_lpc_reset_optimizer(rsg_ctx *ctx);
 /* CG gradients g_n = -2 (H u_n - e_n u_n) of every band in the store,
  * written FP32 to d_g (N x mb, device): the first gradient of a CG call

@@ -11,7 +11,7 @@ smoother precond_hprsdft, cgpc_hprsdft_module.f90:133-361):
     a1  = relu(conv(W1, [x0, x1]) + b1)    2 -> 16 channels
     a2  = relu(conv(W2, a1) + b2)          16 -> 16
     y   = conv(W3, a2) + b3                16 -> 1
-    P g = s * (x0 + y) = g + s * y
+    P g = s * y
 
 conv is a 3x3x3 cross-correlation with periodic wrap on the grid
 (j = i1 + n1 i2 + n1 n2 i3, x fastest), W[co, ci, d3, d2, d1] with
@@ -19,7 +19,8 @@ d = offset + 1.  Fields are arrays f[c, i3, i2, i1] (C order), so a flat
 field f.reshape(C, -1) uses the grid index j.
 
 The GPU stores x0, x1, a1 and a2 and the weights in bfloat16 and
-accumulates in FP32; `forward(..., bf16=True)` reproduces those roundings.
+accumulates in FP32; `forward(..., bf16_storage=True)` reproduces those
+roundings.
 """
 from __future__ import annotations
 
@@ -97,16 +98,17 @@ def forward(p, g3, v3, bf16_storage=False):
     z2 = conv(W2, a1) + p["b2"][:, None, None, None]
     a2 = q(np.maximum(z2, 0.0))
     y = conv(W3, a2)[0] + p["b3"][0]
-    out = s * (x[0] + y)
+    out = s * y
     cache = dict(s=s, x=x, z1=z1, a1=a1, z2=z2, a2=a2, y=y, W1=W1, W2=W2, W3=W3)
     return out, cache
 
 
-def loss_and_grads(p, g3, v3, t3, bf16_storage=False):
-    """L = mean((P g - t)^2) / s^2 = mean((x0 + y - t/s)^2) and its gradients."""
+def loss_and_grads(p, g3, v3, t3, bf16_storage=False, kappa=1.0):
+    """L = mean((P g - kappa t)^2) / s^2 = mean((y - kappa t/s)^2) and its
+    gradients."""
     out, c = forward(p, g3, v3, bf16_storage)
     s = c["s"]
-    r = (out - t3) / s
+    r = c["y"] - kappa * t3 / s
     npts = r.size
     L = float(np.mean(r * r))
     dy = (2.0 / npts) * r

@@ -276,6 +276,21 @@ int rsg_set_hpsi_shape(rsg_ctx *ctx, int precision, int ry, int ty, int zlen, in
   return RSG_OK;
 }
 
+int rsg_set_hpsi_order(rsg_ctx *ctx, int precision, int order) {
+  if (!ctx) return RSG_ERR_ARG;
+  if ((precision != 64 && precision != 32) || (order != 0 && order != 1))
+    return ctx->fail(RSG_ERR_ARG, "set_hpsi_order: precision 64/32 and order 0/1");
+  (precision == 64 ? ctx->shape64 : ctx->shape32).order = order;
+  return RSG_OK;
+}
+
+int rsg_get_hpsi_order(rsg_ctx *ctx, int precision, int *order) {
+  if (!ctx || !order) return RSG_ERR_ARG;
+  if (precision != 64 && precision != 32) return ctx->fail(RSG_ERR_ARG, "get_hpsi_order: precision 64/32");
+  *order = (precision == 64 ? ctx->shape64 : ctx->shape32).order;
+  return RSG_OK;
+}
+
 int rsg_get_hpsi_shape(rsg_ctx *ctx, int precision, int nb, int *ry, int *ty, int *zlen, int *chunk, int *bt) {
   if (!ctx) return RSG_ERR_ARG;
   rsg::HpsiShape sh;
@@ -490,18 +505,18 @@ int rsg_lpc_apply_f32(rsg_ctx *ctx, const float *g, float *pg, int64_t ld, int n
   return rsg::lpc_apply(ctx, g, pg, ld, nb, 32);
 }
 
-int rsg_lpc_target_f32(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double shift) {
+int rsg_lpc_target_f32(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double shift, double vweight) {
   if (!ctx || !g || !t) return RSG_ERR_ARG;
   RSG_HIP(hipSetDevice(ctx->device));
-  return rsg::lpc_target(ctx, g, t, nb, steps, shift);
+  return rsg::lpc_target(ctx, g, t, nb, steps, shift, vweight);
 }
 
-int rsg_lpc_train_step(rsg_ctx *ctx, const float *g, const float *t, int nsamp, float lr, double *loss,
+int rsg_lpc_train_step(rsg_ctx *ctx, const float *g, const float *t, int nsamp, float kappa, float lr, double *loss,
                        float *grads) {
-  if (!ctx || !g || !t || nsamp < 1) return RSG_ERR_ARG;
+  if (!ctx || !g || !t || nsamp < 1 || !(kappa > 0.f)) return RSG_ERR_ARG;
   if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "lpc_train_step: grid not set");
   RSG_HIP(hipSetDevice(ctx->device));
-  return rsg::lpc_train_step(ctx, g, t, nsamp, lr, loss, grads);
+  return rsg::lpc_train_step(ctx, g, t, nsamp, kappa, lr, loss, grads);
 }
 
 int rsg_lpc_reset_optimizer(rsg_ctx *ctx) {

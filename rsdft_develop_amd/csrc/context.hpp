@@ -96,6 +96,7 @@ struct HpsiShape {
   int zlen = 0;   // z-planes per workgroup
   int chunk = 0;  // bands per chunk (gather -> scatter -> stencil)
   int bt = 8;     // bands per scatter thread
+  int order = 0;  // analytic projectors: 0 = nonlocal writes, stencil adds; 1 = stencil writes, nonlocal adds
 };
 
 // Learned preconditioner state: bf16 weight matrices for the kernels, the

@@ -550,15 +550,21 @@ HpsiShape resolve_shape(const rsg_ctx *ctx, const HpsiShape &in, size_t s, int n
   }
   sh.zlen = std::max(1, std::min(sh.zlen, n3));
   if (sh.bt != 4 && sh.bt != 8 && sh.bt != 16) sh.bt = 8;
+  if (sh.order != 0 && sh.order != 1) sh.order = 0;
   return sh;
 }
 
 template <typename T, int BT>
-void launch_apply_t(rsg_ctx *ctx, AnalyticDev<T> &an, T *hpsi, int64_t ld, int nbc, const T *cproj) {
+void launch_apply_t(rsg_ctx *ctx, AnalyticDev<T> &an, T *hpsi, int64_t ld, int nbc, const T *cproj, bool acc) {
   dim3 grid(unsigned((ctx->N + 255) / 256), (nbc + BT - 1) / BT);
-  hipLaunchKernelGGL((k_nl_apply<T, BT, false>), grid, dim3(256), 0, ctx->stream, hpsi, ld, nbc, ctx->N, ctx->n[0],
-                     ctx->n[1], an.hits.template as<uint64_t>(), an.es.template as<T>(), an.ep.template as<T>(),
-                     an.dd.template as<T>(), an.amp.template as<T>(), an.S, cproj);
+  if (acc)
+    hipLaunchKernelGGL((k_nl_apply<T, BT, true>), grid, dim3(256), 0, ctx->stream, hpsi, ld, nbc, ctx->N, ctx->n[0],
+                       ctx->n[1], an.hits.template as<uint64_t>(), an.es.template as<T>(), an.ep.template as<T>(),
+                       an.dd.template as<T>(), an.amp.template as<T>(), an.S, cproj);
+  else
+    hipLaunchKernelGGL((k_nl_apply<T, BT, false>), grid, dim3(256), 0, ctx->stream, hpsi, ld, nbc, ctx->N, ctx->n[0],
+                       ctx->n[1], an.hits.template as<uint64_t>(), an.es.template as<T>(), an.ep.template as<T>(),
+                       an.dd.template as<T>(), an.amp.template as<T>(), an.S, cproj);
 }
 
 template <typename T, int BT>
@@ -601,13 +607,20 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb) {
       const int64_t nred = int64_t(an.ngroup) * 4 * nbc;
       hipLaunchKernelGGL((k_proj_reduce<T>), dim3(unsigned((nred + 255) / 256)), dim3(256), 0, ctx->stream, part,
                          nbc, an.ngroup, an.grp_seg_off.template as<int>(), an.scale.template as<T>(), cp);
-      switch (sh.bt) {
-        case 4: launch_apply_t<T, 4>(ctx, an, hc, ld, nbc, cp); break;
-        case 16: launch_apply_t<T, 16>(ctx, an, hc, ld, nbc, cp); break;
-        default: launch_apply_t<T, 8>(ctx, an, hc, ld, nbc, cp); break;
+      const bool nl_after = sh.order == 1;
+      if (nl_after) {
+        int rc = launch_stencil<T>(ctx, pc, hc, ld, nbc, sh, 0);
+        if (rc) return rc;
       }
-      int rc = launch_stencil<T>(ctx, pc, hc, ld, nbc, sh, 1);
-      if (rc) return rc;
+      switch (sh.bt) {
+        case 4: launch_apply_t<T, 4>(ctx, an, hc, ld, nbc, cp, nl_after); break;
+        case 16: launch_apply_t<T, 16>(ctx, an, hc, ld, nbc, cp, nl_after); break;
+        default: launch_apply_t<T, 8>(ctx, an, hc, ld, nbc, cp, nl_after); break;
+      }
+      if (!nl_after) {
+        int rc = launch_stencil<T>(ctx, pc, hc, ld, nbc, sh, 1);
+        if (rc) return rc;
+      }
     }
   } else {
     if (ctx->nl_mode == 2) return ctx->fail(RSG_ERR_STATE, "hpsi: analytic projectors requested but not available");
@@ -733,6 +746,17 @@ template <typename T> int autotune_t(rsg_ctx *ctx, int nb, int reps, float *best
       HpsiShape c = best;
       c.bt = bt;
       consider(c);
+    }
+    if (ctx->have_ana && ctx->nl_mode != 1) {
+      HpsiShape c = best;
+      c.order = 1 - best.order;
+      consider(c);
+      for (int f : {4, 8}) {  // smaller chunks keep psi and H psi in the Infinity Cache
+        HpsiShape d = best;
+        d.chunk = std::max(1, nb / f);
+        d.zlen = 0;
+        consider(d);
+      }
     }
   }
   for (int zf : {2, 4}) {

@@ -8,7 +8,7 @@
 //   a1 = relu(W1 * x + b1)   (2 -> 16)
 //   a2 = relu(W2 * a1 + b2)  (16 -> 16)
 //   y  = W3 * a2 + b3        (16 -> 1)
-//   P g = g + s y
+//   P g = s y                (scale-equivariant: P(c g) = c P g)
 //
 // Storage: weights and activations in bfloat16, channel-last
 // (act[sample][point][channel]); every product on MFMA
@@ -102,7 +102,7 @@ enum ConvMode { kReluOut = 0, kFinalOut = 1, kMaskedGrad = 2 };
 // One convolution layer for nsamp samples.  Modes:
 //  kReluOut:   out_bf[p][co] = bf16(relu(D + bias))            (16 outputs)
 //  kFinalOut:  y = D[0] + bias[0]; yout[p] = y (FP32 if non-null);
-//              pg[p] = g[p] + s y  (the preconditioned residual, type T)
+//              pg[p] = s y  (the preconditioned residual, type T)
 //  kMaskedGrad: out_bf[p][co] = bf16(D * (mask[p][co] > 0))    (16 outputs)
 template <int CIN, int KP, int MODE, typename T>
 __global__ __launch_bounds__(64 * kWaves) void k_conv(Grid3 g, int64_t N, const uint16_t *__restrict__ in,
@@ -165,10 +165,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_conv(Grid3 g, int64_t N, const 
       if (lq == 0 && ok) {
         const float y = acc[0] + bias[0];
         if (yout) yout[int64_t(smp) * N + p] = y;
-        if (pg) {
-          const int64_t o = int64_t(smp) * ldg + p;
-          pg[o] = T(double(gsrc[o]) + double(sval[smp]) * double(y));
-        }
+        if (pg) pg[int64_t(smp) * ldg + p] = T(double(sval[smp]) * double(y));
       }
     }
   }
@@ -193,11 +190,11 @@ __global__ void k_rms(const double *__restrict__ gg, int nsamp, int64_t N, float
   if (i < nsamp) s[i] = float(fmax(sqrt(gg[i] / double(N)), 1e-30));
 }
 
-// Loss of a batch: r = (g - t)/s + y, L += sum r^2 / (N nsamp);
+// Loss of a batch: r = y - kappa t / s, L += sum r^2 / (N nsamp);
 // dy = 2 r / (N nsamp) stored as bf16 for the backward convolutions.
-__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ gsrc, const float *__restrict__ tgt,
-                                              const float *__restrict__ y, const float *__restrict__ sval,
-                                              int64_t N, double inv_count, uint16_t *__restrict__ dy,
+__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ tgt, const float *__restrict__ y,
+                                              const float *__restrict__ sval, float kappa, int64_t N,
+                                              double inv_count, uint16_t *__restrict__ dy,
                                               double *__restrict__ lsum) {
   __shared__ double red[4];
   const int smp = blockIdx.y;
@@ -205,8 +202,7 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ gsrc, co
   double r2 = 0.0;
   if (p < N) {
     const int64_t o = int64_t(smp) * N + p;
-    const float s = sval[smp];
-    const float r = (gsrc[o] - tgt[o]) / s + y[o];
+    const float r = y[o] - kappa * tgt[o] / sval[smp];
     r2 = double(r) * r;
     dy[o] = f2bf(float(2.0 * inv_count * r));
   }
@@ -518,9 +514,11 @@ int lpc_apply(rsg_ctx *ctx, const void *g, void *pg, int64_t ld, int nb, int pre
 }
 
 // One training step on nsamp samples (FP32 residuals g and targets t, each
-// N x nsamp column-major): forward, loss, backward, optional Adam update.
-// grads (host, may be NULL) receives the gradient in the parameter layout.
-int lpc_train_step(rsg_ctx *ctx, const float *g, const float *t, int nsamp, float lr, double *loss, float *grads) {
+// N x nsamp column-major): forward, loss against kappa t, backward,
+// optional Adam update.  grads (host, may be NULL) receives the gradient in
+// the parameter layout.
+int lpc_train_step(rsg_ctx *ctx, const float *g, const float *t, int nsamp, float kappa, float lr, double *loss,
+                   float *grads) {
   LpcState &L = ctx->lpc;
   int rc = forward_t<float>(ctx, g, ctx->N, nsamp, nullptr, true);
   if (rc) return rc;
@@ -530,8 +528,8 @@ int lpc_train_step(rsg_ctx *ctx, const float *g, const float *t, int nsamp, floa
     return ctx->fail(RSG_ERR_HIP, "lpc: allocation failed");
   RSG_HIP(hipMemsetAsync(L.lsum.p, 0, 64, ctx->stream));
   const double inv = 1.0 / (double(N) * nsamp);
-  hipLaunchKernelGGL(k_loss, dim3(unsigned((N + 255) / 256), nsamp), dim3(256), 0, ctx->stream, g, t, L.y.as<float>(),
-                     L.sval.as<float>(), N, inv, L.dyb.as<uint16_t>(), L.lsum.as<double>());
+  hipLaunchKernelGGL(k_loss, dim3(unsigned((N + 255) / 256), nsamp), dim3(256), 0, ctx->stream, t, L.y.as<float>(),
+                     L.sval.as<float>(), kappa, N, inv, L.dyb.as<uint16_t>(), L.lsum.as<double>());
   const uint16_t *w1 = L.wbf.as<uint16_t>(), *w2 = w1 + 16 * KP1, *w3 = w2 + 16 * KP2, *w2t = w3 + 16 * KP3,
                  *w3t = w2t + 16 * KP2;
   // dz2 = (W3' * dy) masked by a2 > 0 ; dz1 = (W2' * dz2) masked by a1 > 0

@@ -10,6 +10,7 @@ int lpc_set_potential(rsg_ctx *ctx, const double *vloc_host);
 // P g for nb bands g[b*ld + p] (device), precision 64 or 32; pg may equal
 // neither g nor overlap it
 int lpc_apply(rsg_ctx *ctx, const void *g, void *pg, int64_t ld, int nb, int precision);
-int lpc_train_step(rsg_ctx *ctx, const float *g, const float *t, int nsamp, float lr, double *loss, float *grads);
+int lpc_train_step(rsg_ctx *ctx, const float *g, const float *t, int nsamp, float kappa, float lr, double *loss,
+                   float *grads);
 int lpc_reset_optimizer(rsg_ctx *ctx);
 }  // namespace rsg

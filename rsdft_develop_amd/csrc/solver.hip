@@ -264,6 +264,15 @@ __global__ void k_pc_dir(const W *__restrict__ r, W *__restrict__ p, const doubl
   p[k] = r[k] + W(rr1[b] / rr0[b]) * p[k];
 }
 
+// A <- s (A + A^T) in place (both triangles), for an n x n matrix
+__global__ void k_symmetrize(double *__restrict__ A, int n, double s) {
+  const int i = blockIdx.x * 16 + threadIdx.x, j = blockIdx.y * 16 + threadIdx.y;
+  if (i >= n || j >= n || i > j) return;
+  const double v = s * (A[i + int64_t(n) * j] + A[j + int64_t(n) * i]);
+  A[i + int64_t(n) * j] = v;
+  A[j + int64_t(n) * i] = v;
+}
+
 inline dim3 grid_nb(int64_t N, int nb) { return dim3(unsigned((N + kThreads - 1) / kThreads), nb); }
 
 struct Scratch {
@@ -432,9 +441,10 @@ int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
     return ctx->fail(RSG_ERR_HIP, "gram_schmidt: allocation failed");
   double *S = ctx->hsub.as<double>();
   const double dv = ctx->dv, zero = 0.0, one = 1.0;
-  // S = dV U^T U (upper), gram_schmidt_lusl_module.f90:268 (calc_dsyrk3)
-  RSG_BLAS(rocblas_dsyrk(ctx->blas, rocblas_fill_upper, rocblas_operation_transpose, nb, int(N), &dv, U,
-                         int(N), &zero, S, nb));
+  // S = dV U^T U, gram_schmidt_lusl_module.f90:268 (calc_dsyrk3), as a full
+  // GEMM: rocBLAS's SYRK runs this K = N shape far below the MFMA GEMM
+  RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, int(N), &dv, U,
+                         int(N), U, int(N), &zero, S, nb));
   // S = R^T R (:286, PDPOTRF)
   RSG_BLAS(rocsolver_dpotrf(ctx->blas, rocblas_fill_upper, nb, S, nb, ctx->info.as<rocblas_int>()));
   int info = 0;
@@ -463,10 +473,14 @@ int subspace_diag(rsg_ctx *ctx, double *esp) {
   // psit = H U for all bands in one band-blocked call (subspace_diag_la_module.f90:79)
   rc = launch_hpsi<double>(ctx, U, HU, N, mb);
   if (rc) return rc;
-  // Hsub = 1/2 dV (U^T HU + HU^T U) (:86, DSYR2K 'U','T')
-  const double zz = 0.5 * ctx->dv, zero = 0.0, one = 1.0;
-  RSG_BLAS(rocblas_dsyr2k(ctx->blas, rocblas_fill_upper, rocblas_operation_transpose, mb, int(N), &zz, U,
-                          int(N), HU, int(N), &zero, Hs, mb));
+  // Hsub = 1/2 dV (U^T HU + HU^T U) (:86, DSYR2K 'U','T') as A = U^T HU by
+  // one GEMM, then Hsub = dV (A + A^T) / 2 (the SYR2K path of rocBLAS runs
+  // this K = N shape far below the MFMA GEMM)
+  const double zero = 0.0, one = 1.0;
+  RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, mb, mb, int(N), &one, U,
+                         int(N), HU, int(N), &zero, Hs, mb));
+  hipLaunchKernelGGL(k_symmetrize, dim3((mb + 15) / 16, (mb + 15) / 16), dim3(16, 16), 0, ctx->stream, Hs, mb,
+                     0.5 * ctx->dv);
   // eigen-decomposition, ascending (:155, DSYEV 'V','U')
   RSG_BLAS(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, mb, Hs, mb, D, Ework,
                             ctx->info.as<rocblas_int>()));
@@ -655,11 +669,12 @@ int band_residuals_f32(rsg_ctx *ctx, float *g) {
 }
 
 // Training targets of the learned preconditioner: t = A^-1 g with
-// A = -1/2 lap_7 + (Vloc - min Vloc + shift), by `steps` CG steps from
-// t0 = g, for nb FP32 bands (device, leading dimension N).
-int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double shift) {
+// A = -1/2 lap_7 + vweight (Vloc - min Vloc) + shift, by `steps` CG steps
+// from t0 = g, for nb FP32 bands (device, leading dimension N).
+int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double shift, double vweight) {
   if (!ctx->have_grid || !ctx->have_vloc) return ctx->fail(RSG_ERR_STATE, "lpc_target: grid or Vloc not set");
-  if (nb < 1 || steps < 1 || !(shift > 0.0)) return ctx->fail(RSG_ERR_ARG, "lpc_target: nb, steps >= 1, shift > 0");
+  if (nb < 1 || steps < 1 || !(shift > 0.0) || vweight < 0.0)
+    return ctx->fail(RSG_ERR_ARG, "lpc_target: nb, steps >= 1, shift > 0, vweight >= 0");
   const int64_t N = ctx->N;
   LpcState &L = ctx->lpc;
   std::vector<double> v(static_cast<size_t>(N));
@@ -667,7 +682,7 @@ int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double
   RSG_HIP(hipStreamSynchronize(ctx->stream));
   const double vmin = *std::min_element(v.begin(), v.end());
   std::vector<float> d(static_cast<size_t>(N));
-  for (int64_t i = 0; i < N; ++i) d[size_t(i)] = float(v[size_t(i)] - vmin + shift);
+  for (int64_t i = 0; i < N; ++i) d[size_t(i)] = float(vweight * (v[size_t(i)] - vmin) + shift);
   if (L.diag.ensure(size_t(N) * 4) != hipSuccess || L.tbuf.ensure(size_t(N) * nb * 4 * 3) != hipSuccess)
     return ctx->fail(RSG_ERR_HIP, "lpc_target: allocation failed");
   RSG_HIP(hipMemcpyAsync(L.diag.p, d.data(), size_t(N) * 4, hipMemcpyHostToDevice, ctx->stream));

@@ -75,6 +75,8 @@ def load_library():
         "rsg_set_hpsi_shape": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
         "rsg_get_hpsi_shape": (_c_int, [_vp, _c_int, _c_int, _P(_c_int), _P(_c_int), _P(_c_int), _P(_c_int), _P(_c_int)]),
         "rsg_autotune_hpsi": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(ctypes.c_float)]),
+        "rsg_set_hpsi_order": (_c_int, [_vp, _c_int, _c_int]),
+        "rsg_get_hpsi_order": (_c_int, [_vp, _c_int, _P(_c_int)]),
         "rsg_alloc_bands": (_c_int, [_vp, _c_int]),
         "rsg_upload_bands": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int]),
         "rsg_download_bands": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int]),
@@ -103,8 +105,8 @@ def load_library():
         "rsg_lpc_set_potential": (_c_int, [_vp, _vp]),
         "rsg_lpc_apply_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
         "rsg_lpc_apply_f32": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
-        "rsg_lpc_target_f32": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_dbl]),
-        "rsg_lpc_train_step": (_c_int, [_vp, _vp, _vp, _c_int, ctypes.c_float, _P(_c_dbl), _vp]),
+        "rsg_lpc_target_f32": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_dbl, _c_dbl]),
+        "rsg_lpc_train_step": (_c_int, [_vp, _vp, _vp, _c_int, ctypes.c_float, ctypes.c_float, _P(_c_dbl), _vp]),
         "rsg_lpc_reset_optimizer": (_c_int, [_vp]),
         "rsg_band_residuals_f32": (_c_int, [_vp, _vp]),
         "rsg_lpc_fallbacks": (_c_int, [_vp, _P(_c_i64)]),
@@ -257,10 +259,20 @@ class Context:
     def set_hpsi_shape(self, precision=64, ry=0, ty=0, zlen=0, chunk=0, bt=0):
         self._check(self.L.rsg_set_hpsi_shape(self.h, int(precision), int(ry), int(ty), int(zlen), int(chunk), int(bt)))
 
+    def set_hpsi_order(self, precision=64, order=0):
+        self._check(self.L.rsg_set_hpsi_order(self.h, int(precision), int(order)))
+
+    def hpsi_order(self, precision=64) -> int:
+        o = _c_int()
+        self._check(self.L.rsg_get_hpsi_order(self.h, int(precision), ctypes.byref(o)))
+        return o.value
+
     def hpsi_shape(self, precision=64, nb=1) -> dict:
         v = [_c_int() for _ in range(5)]
         self._check(self.L.rsg_get_hpsi_shape(self.h, int(precision), int(nb), *[ctypes.byref(x) for x in v]))
-        return dict(zip(("ry", "ty", "zlen", "chunk", "bt"), (x.value for x in v)))
+        d = dict(zip(("ry", "ty", "zlen", "chunk", "bt"), (x.value for x in v)))
+        d["order"] = self.hpsi_order(precision)
+        return d
 
     def autotune_hpsi(self, precision=64, nb=1, reps=5) -> float:
         ms = ctypes.c_float()
@@ -399,13 +411,15 @@ class Context:
         f = self.L.rsg_lpc_apply_f64 if dtype == np.float64 else self.L.rsg_lpc_apply_f32
         self._check(f(self.h, d_g, d_pg, int(ld), int(nb)))
 
-    def lpc_target_device(self, d_g: int, d_t: int, nb: int, steps: int = 40, shift: float = 0.5):
-        self._check(self.L.rsg_lpc_target_f32(self.h, d_g, d_t, int(nb), int(steps), float(shift)))
+    def lpc_target_device(self, d_g: int, d_t: int, nb: int, steps: int = 40, shift: float = 0.5,
+                          vweight: float = 1.0):
+        self._check(self.L.rsg_lpc_target_f32(self.h, d_g, d_t, int(nb), int(steps), float(shift), float(vweight)))
 
-    def lpc_train_step(self, d_g: int, d_t: int, nsamp: int, lr: float = 0.0, want_grads: bool = False):
+    def lpc_train_step(self, d_g: int, d_t: int, nsamp: int, lr: float = 0.0, want_grads: bool = False,
+                       kappa: float = 1.0):
         loss = _c_dbl()
         grads = np.zeros(self.lpc_param_count(), dtype=np.float32) if want_grads else None
-        self._check(self.L.rsg_lpc_train_step(self.h, d_g, d_t, int(nsamp), float(lr), ctypes.byref(loss),
+        self._check(self.L.rsg_lpc_train_step(self.h, d_g, d_t, int(nsamp), float(kappa), float(lr), ctypes.byref(loss),
                                               grads.ctypes.data if grads is not None else None))
         return loss.value, grads
 

@@ -8,8 +8,8 @@ drives them through the C ABI:
   b3[1] (W[co][ci][tap], tap = d3*9 + d2*3 + d1);
 - `make_dataset`: residuals g = -2 (H u - e u) of the plain method's own
   bands, taken from a few plain SCF iterations on perturbed cells, paired
-  with targets t = A^-1 g, A = -1/2 lap_7 + (Vloc - min Vloc + shift) (a
-  potential-aware, shifted smoother solved to convergence);
+  with targets t = A^-1 g, A = -1/2 lap_7 + vweight (Vloc - min Vloc) +
+  shift, solved far past the mloop steps of the hprsdft smoother;
 - `train`: Adam over mini-batches, all on the GPU.
 
 No counterpart in the reference: it replaces precond_hprsdft
@@ -28,8 +28,7 @@ NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)
 
 
 def init_params(seed: int = 0) -> np.ndarray:
-    """He-scaled conv weights, zero biases, a small last layer (the net
-    starts close to P g = g)."""
+    """He-scaled conv weights, zero biases, a small last layer."""
     rng = np.random.default_rng(seed)
     d = {
         "W1": rng.standard_normal((CH, 2, 27)) * np.sqrt(2.0 / (2 * 27)),
@@ -69,14 +68,15 @@ class Dataset:
     npoints: int
     cells: int
     seconds: float
+    blocks: list    # (first sample, count, Vloc of that cell and iteration)
 
 
 def make_dataset(ctx_factory, cells, scf_iters: int = 2, target_steps: int = 40, shift: float = 0.5,
-                 max_samples: int | None = None, log=None) -> Dataset:
+                 vweight: float = 1.0, max_samples: int | None = None, log=None) -> Dataset:
     """Residual/target pairs from plain SCF runs on `cells` (all on one grid
     shape).  ctx_factory() returns a fresh gpu.Context.  Each cell gives
     nband samples per SCF iteration."""
-    from . import hip, scf
+    from . import gpu, hip, scf
 
     N = cells[0].npoints
     per_cell = cells[0].nband * scf_iters
@@ -87,6 +87,7 @@ def make_dataset(ctx_factory, cells, scf_iters: int = 2, target_steps: int = 40,
     t_all = hip.DeviceArray((N, total), dtype=np.float32)
     t0 = time.perf_counter()
     filled = 0
+    blocks = []
     for ic, cell in enumerate(cells):
         if filled >= total:
             break
@@ -108,37 +109,66 @@ def make_dataset(ctx_factory, cells, scf_iters: int = 2, target_steps: int = 40,
             g_ptr = g_all.ptr + filled * N * 4
             t_ptr = t_all.ptr + filled * N * 4
             hip.copy_d2d(g_ptr, stage.ptr, nb * N * 4)
-            ctx.lpc_target_device(g_ptr, t_ptr, nb, steps=target_steps, shift=shift)
+            ctx.lpc_target_device(g_ptr, t_ptr, nb, steps=target_steps, shift=shift, vweight=vweight)
             stage.free()
+            blocks.append((filled, nb, ctx.get_field(gpu.FIELD_VLOC)))
             filled += nb
         ctx.close()
         if log:
             log(f"dataset: cell {ic + 1}/{len(cells)}, {filled} samples, {time.perf_counter() - t0:.1f} s")
-    return Dataset(g_all, t_all, filled, N, len(cells), time.perf_counter() - t0)
+    return Dataset(g_all, t_all, filled, N, len(cells), time.perf_counter() - t0, blocks)
 
 
 def train(ctx, data: Dataset, params: np.ndarray, epochs: int = 4, batch: int = 16, lr: float = 2e-3,
           seed: int = 0, log=None) -> tuple:
-    """Adam over shuffled mini-batches of consecutive samples.  Returns the
-    trained parameters and the per-epoch mean loss."""
+    """Adam over shuffled mini-batches; a batch never mixes cells, so the
+    network's potential input is that cell's Vloc.  Returns the trained
+    parameters and the per-epoch mean loss."""
     ctx.lpc_set_params(params)
     ctx.lpc_reset_optimizer()
     rng = np.random.default_rng(seed)
     N = data.npoints
-    nbatch = data.nsamp // batch
+    kappa = target_scale(data)
+    batches = [(ib, s) for ib, (first, count, _) in enumerate(data.blocks)
+               for s in range(first, first + count - batch + 1, batch)]
     hist = []
     for ep in range(epochs):
-        order = rng.permutation(nbatch)
+        order = rng.permutation(len(batches))
         tot = 0.0
         t0 = time.perf_counter()
-        for b in order:
-            off = int(b) * batch * N * 4
-            loss, _ = ctx.lpc_train_step(data.g.ptr + off, data.t.ptr + off, batch, lr=lr)
+        current = -1
+        for k in order:
+            ib, s = batches[int(k)]
+            if ib != current:
+                ctx.lpc_set_potential(data.blocks[ib][2])
+                current = ib
+            off = s * N * 4
+            loss, _ = ctx.lpc_train_step(data.g.ptr + off, data.t.ptr + off, batch, lr=lr, kappa=kappa)
             tot += loss
-        hist.append(tot / max(nbatch, 1))
+        hist.append(tot / max(len(batches), 1))
         if log:
             log(f"epoch {ep + 1}/{epochs}: loss {hist[-1]:.4e} ({time.perf_counter() - t0:.1f} s)")
     return ctx.lpc_get_params(), hist
+
+
+def target_scale(data: Dataset, nprobe: int = 8) -> float:
+    """kappa = 1 / mean over probe samples of rms(t)/rms(g): the network then
+    predicts corrections of unit size, and P g = s y stays one fixed linear
+    scale of the targets across bands and CG steps."""
+    import ctypes
+
+    from . import hip
+
+    N = data.npoints
+    idx = np.linspace(0, data.nsamp - 1, min(nprobe, data.nsamp)).astype(int)
+    ratios = []
+    for i in idx:
+        gh = np.empty(N, dtype=np.float32)
+        th = np.empty(N, dtype=np.float32)
+        hip.runtime().hipMemcpy(gh.ctypes.data, ctypes.c_void_p(data.g.ptr + int(i) * N * 4), N * 4, 2)
+        hip.runtime().hipMemcpy(th.ctypes.data, ctypes.c_void_p(data.t.ptr + int(i) * N * 4), N * 4, 2)
+        ratios.append(np.sqrt(np.mean(th.astype(np.float64) ** 2) / np.mean(gh.astype(np.float64) ** 2)))
+    return float(1.0 / np.mean(ratios))
 
 
 def save(path: str, params: np.ndarray, meta: dict | None = None):

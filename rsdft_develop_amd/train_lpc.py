@@ -81,7 +81,8 @@ def main(argv=None) -> int:
     ap.add_argument("--scf-iters", type=int, default=4)
     ap.add_argument("--max-samples", type=int, default=20480)
     ap.add_argument("--target-steps", type=int, default=40)
-    ap.add_argument("--shift", type=float, default=0.5)
+    ap.add_argument("--shift", type=float, default=0.1)
+    ap.add_argument("--vweight", type=float, default=0.0)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -94,11 +95,12 @@ def main(argv=None) -> int:
     cells = si64_cells(args.cells)
     t0 = time.perf_counter()
     data = lpc.make_dataset(lambda: gpu.Context(0), cells, scf_iters=args.scf_iters, target_steps=args.target_steps,
-                            shift=args.shift, max_samples=args.max_samples, log=log)
+                            shift=args.shift, vweight=args.vweight, max_samples=args.max_samples, log=log)
     log(f"dataset: {data.nsamp} samples of {data.npoints} points in {data.seconds:.1f} s")
     ctx = gpu.Context(0)
     c0 = cells[0]
     ctx.set_grid(c0.ngrid, c0.aa, c0.md)
+    log(f"training: {len(data.blocks)} blocks, batch {args.batch}, {args.epochs} epochs")
     p0 = lpc.init_params(args.seed)
     t1 = time.perf_counter()
     params, hist = lpc.train(ctx, data, p0, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed, log=log)
@@ -107,10 +109,11 @@ def main(argv=None) -> int:
     data.t.free()
     ctx.close()
     meta = dict(loss_history=np.array(hist), samples=data.nsamp, cells=args.cells, epochs=args.epochs,
-                shift=args.shift, target_steps=args.target_steps)
+                shift=args.shift, vweight=args.vweight, target_steps=args.target_steps)
     lpc.save(args.out, params, meta)
     summary = dict(samples=data.nsamp, dataset_s=round(data.seconds, 1), train_s=round(train_s, 1),
-                   loss_history=hist, out=args.out)
+                   loss_history=hist, out=args.out, shift=args.shift, vweight=args.vweight,
+                   target_steps=args.target_steps)
     if args.eval:
         from . import systems
 
