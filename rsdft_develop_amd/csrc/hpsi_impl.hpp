@@ -58,10 +58,9 @@ __host__ __device__ constexpr int ring_pf(int md) { return (md % 2 == 0) ? 3 : 5
 __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 template <typename T, int MD, int RY, int HM, bool ACC>
-__global__ __launch_bounds__(512) void k_stencil(const T *__restrict__ psi, T *__restrict__ hpsi,
-                                                 int64_t ld, const T *__restrict__ vloc,
+__global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t ld, const T *vloc,
                                                  StencilCoef<T> cf, int n1, int n2, int n3,
-                                                 int ty, int zlen, T *__restrict__ dead) {
+                                                 int ty, int zlen, T *dead) {
   constexpr int PF = ring_pf(MD);
   constexpr int R = 2 * MD + 1 + PF;
   static_assert(R % kHS == 0 && R % 2 == 0, "ring length must be a multiple of the rotations");
@@ -77,8 +76,12 @@ __global__ __launch_bounds__(512) void k_stencil(const T *__restrict__ psi, T *_
   const int y0 = blockIdx.x * ty;
   const int z0 = blockIdx.y * zlen, z1 = min(z0 + zlen, n3);
   const int64_t n12 = int64_t(n1) * n2;
-  const T *__restrict__ w = psi + int64_t(blockIdx.z) * ld;
-  T *__restrict__ o = hpsi + int64_t(blockIdx.z) * ld;
+  // No __restrict__ on the streams: with it the loads count as invariant
+  // and the register allocator re-issues them next to their uses instead of
+  // keeping the prefetched values, which turns every plane into a full
+  // memory round trip.  Without it no load moves across the H psi stores.
+  const T *w = psi + int64_t(blockIdx.z) * ld;
+  T *o = hpsi + int64_t(blockIdx.z) * ld;
 
   int64_t colofs[RY];
   bool own[RY];
