@@ -67,16 +67,26 @@ template <typename T> struct NonlocalDev {
 // leaves sum_m |p_m><p_m| unchanged.  Which points belong to which sphere
 // is decided on the host with the table builder's own arithmetic and
 // stored as flags.
+//
+// Work lists: a segment is one atom's sphere points on one grid plane (at
+// most kPlaneSegMax of them, in grid order); segments are numbered atom by
+// atom (grp_seg_off) and launched plane by plane (seg_order), so the atoms
+// that share a plane read it while it is cached.  A tile is 16 consecutive
+// grid points; tile_g lists the atoms whose spheres meet it.
 template <typename T> struct AnalyticDev {
-  int ngroup = 0, nseg = 0, S = 0;
+  int ngroup = 0, nseg = 0, S = 0, ntile = 0;
   DevBuf es, ep, dd;    // T[ngroup * S]: axis x at [0, n1), y at [n1, n1+n2), z after
   DevBuf amp, scale;    // T[ngroup * 4]: amplitudes (s, p, p, p), iuV * dV
   DevBuf pts;           // uint32 per group point: x | y << 10 | z << 20 | s << 30 | p << 31
   DevBuf seg_beg, seg_end, seg_grp;  // int[nseg]
+  DevBuf seg_order;     // int[nseg]      launch order (by plane, then atom)
   DevBuf grp_seg_off;   // int[ngroup + 1]
   DevBuf hits;          // uint64 per grid point: 4 x uint16 (group + 1 | s << 14 | p << 15)
+  DevBuf tile_off;      // int[ntile + 1]
+  DevBuf tile_g;        // int[tile_off[ntile]]
   void release_all() {
-    for (DevBuf *b : {&es, &ep, &dd, &amp, &scale, &pts, &seg_beg, &seg_end, &seg_grp, &grp_seg_off, &hits})
+    for (DevBuf *b : {&es, &ep, &dd, &amp, &scale, &pts, &seg_beg, &seg_end, &seg_grp, &seg_order, &grp_seg_off,
+                      &hits, &tile_off, &tile_g})
       b->release();
   }
 };
@@ -85,8 +95,9 @@ struct HostAnalytic {
   int ngroup = 0, S = 0;
   std::vector<double> es, ep, dd, amp, scale;
   std::vector<uint32_t> pts;
-  std::vector<int> seg_beg, seg_end, seg_grp, grp_seg_off{0};
+  std::vector<int> seg_beg, seg_end, seg_grp, seg_order, grp_seg_off{0};
   std::vector<uint64_t> hits;
+  std::vector<int> tile_off, tile_g;
 };
 
 // Launch shape of the hpsi kernels for one grid and precision.

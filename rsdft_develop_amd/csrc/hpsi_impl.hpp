@@ -1,18 +1,24 @@
 // hpsi.hip -- Hamiltonian application on gfx950:
 //   hpsi = (T + Vloc) psi + V_NL psi  over a block of bands.
 //
-// Bands are processed in chunks sized so one chunk of psi and H psi stays in
-// the 256 MB Infinity Cache between the launches that touch it:
+// Bands are processed in chunks (sized by the autotuner; one chunk of psi
+// and H psi can stay in the 256 MB Infinity Cache between launches).
+// Nonlocal part, analytic projectors (the default for l <= 1, n = 1 GTH):
+//  1. k_proj_m: partial projections sum_j beta_k(j) psi(j,b) per segment
+//     (one atom's sphere points on one plane) on the matrix cores
+//     (d_ps_nloc2_op_module.f90:182-191 with the projector values formed on
+//     the fly instead of read from uVk tables);
+//  2. k_proj_reduce: C(g,k,b) = iuV dV * sum over the atom's segments;
+//  3. k_apply_m: H psi(j,b) (=|+=) sum_{g,k} beta_k(j) C(g,k,b) per 16-point
+//     tile on the matrix cores (d_ps_nloc2_op_module.f90:274-281).
+// Nonlocal part, projector tables (any l; nonlocal mode 1):
 //  1. k_nl_gather: per (atom-group segment, band tile) partial projections
-//     sum_j uV(j,l) psi(j,b) for the <= 4 projectors of one atom, reading psi
-//     at each sphere point once (d_ps_nloc2_op_module.f90:182-191);
-//     wavefront-shuffle + LDS reduction inside the workgroup.
-//  2. k_nl_reduce: uVunk(lma,b) = iuV dV * sum over the group's segments.
-//  3. k_nl_scatter: H psi(j,b) = sum_lma uV(j,lma) uVunk(lma,b)
-//     (d_ps_nloc2_op_module.f90:274-281) from a per-point CSR, each thread
-//     one point and a tile of bands so the table is read once per tile; no
-//     atomics.
-//  4. k_stencil: H psi += (c + V) psi + 13-point-per-axis FD Laplacian
+//     from the uVk/JJP tables, reading psi at each sphere point once;
+//  2. k_nl_reduce: uVunk(lma,b) = iuV dV * sum over the group's segments;
+//  3. k_nl_scatter: H psi(j,b) = sum_lma uV(j,lma) uVunk(lma,b) from a
+//     per-point sliced-ELL table, no atomics.
+// Local part:
+//  4. k_stencil: H psi (+)= (c + V) psi + 13-point-per-axis FD Laplacian
 //     (d_kinetic_sol_module.f90:244-292).  A workgroup owns ty full x-rows
 //     of a z-run; each plane is staged in double-buffered LDS with its
 //     +-MD halo rows (periodic in x inside the row, in y by row wrap: the
@@ -57,6 +63,16 @@ __host__ __device__ constexpr int ring_pf(int md) { return (md % 2 == 0) ? 3 : 5
 
 __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// Workgroups of a launch are dealt to the 8 XCDs in turn (block b runs on
+// XCD b % 8), and each XCD has its own L2.  xcd_block maps the hardware
+// block index to a logical one such that every XCD works through one
+// contiguous range of logical blocks: neighbouring tiles, which share halo
+// rows or sphere rows, then meet in the same L2.  A bijection on [0, nblk).
+__device__ inline unsigned xcd_block(unsigned b, unsigned nblk) {
+  const unsigned q = nblk / 8, r = nblk % 8, x = b % 8;
+  return x * q + min(x, r) + b / 8;
+}
+
 template <typename T, int MD, int RY, int HM, bool ACC>
 __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t ld, const T *vloc,
                                                  StencilCoef<T> cf, int n1, int n2, int n3,
@@ -73,15 +89,20 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
   const bool active = tid < n1 * G;
   const int x = active ? tid % n1 : 0;
   const int g = active ? tid / n1 : 0;
-  const int y0 = blockIdx.x * ty;
-  const int z0 = blockIdx.y * zlen, z1 = min(z0 + zlen, n3);
+  // logical block: y-tile fastest, then z-run, then band
+  const int ntile_y = (n2 + ty - 1) / ty, nz = (n3 + zlen - 1) / zlen;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int by = int(lb % unsigned(ntile_y)), bz = int((lb / unsigned(ntile_y)) % unsigned(nz));
+  const int band = int(lb / unsigned(ntile_y * nz));
+  const int y0 = by * ty;
+  const int z0 = bz * zlen, z1 = min(z0 + zlen, n3);
   const int64_t n12 = int64_t(n1) * n2;
   // No __restrict__ on the streams: with it the loads count as invariant
   // and the register allocator re-issues them next to their uses instead of
   // keeping the prefetched values, which turns every plane into a full
   // memory round trip.  Without it no load moves across the H psi stores.
-  const T *w = psi + int64_t(blockIdx.z) * ld;
-  T *o = hpsi + int64_t(blockIdx.z) * ld;
+  const T *w = psi + int64_t(band) * ld;
+  T *o = hpsi + int64_t(band) * ld;
 
   int64_t colofs[RY];
   bool own[RY];
@@ -290,85 +311,6 @@ __global__ __launch_bounds__(256) void k_nl_scatter(T *__restrict__ hpsi, int64_
   }
 }
 
-// ---- analytic GTH projectors (l <= 1, n = 1), evaluated on the fly ----
-//
-// Projector values at a sphere point of group (atom) g with grid coordinates
-// (x, y, z): s = amp_s * es_x es_y es_z, p_m = amp_p * ep_x ep_y ep_z * d_m,
-// from per-atom 1-D tables (see AnalyticDev).  The sphere flags come from
-// the host.  Compared with the table path this streams 4 bytes per sphere
-// point instead of 36.
-
-template <typename T>
-__device__ inline void ana_values(const T *__restrict__ es, const T *__restrict__ ep, const T *__restrict__ dd,
-                                  int64_t gS, int oy, int oz, int x, int y, int z, bool fs, bool fp, T amp_s,
-                                  T amp_p, T &cs, T &cx, T &cy, T &cz) {
-  const int64_t ix = gS + x, iy = gS + oy + y, iz = gS + oz + z;
-  const T vs = fs ? amp_s * es[ix] * es[iy] * es[iz] : T(0);
-  const T vp = fp ? amp_p * ep[ix] * ep[iy] * ep[iz] : T(0);
-  cs = vs;
-  cx = vp * dd[ix];
-  cy = vp * dd[iy];
-  cz = vp * dd[iz];
-}
-
-template <typename T, int BB>
-__global__ __launch_bounds__(kGatherThreads) void k_proj_a(
-    const T *__restrict__ psi, int64_t ld, int nbc, int n1, int n2, const int *__restrict__ seg_beg,
-    const int *__restrict__ seg_end, const int *__restrict__ seg_grp, const uint32_t *__restrict__ pts,
-    const T *__restrict__ es, const T *__restrict__ ep, const T *__restrict__ dd, const T *__restrict__ amp,
-    int S, T *__restrict__ part) {
-  __shared__ T red[kGatherThreads / 64][4 * BB];
-  const int s = blockIdx.x;
-  const int b0 = blockIdx.y * BB;
-  const int nbb = min(BB, nbc - b0);
-  const int beg = seg_beg[s], end = seg_end[s], g = seg_grp[s];
-  const int64_t gS = int64_t(g) * S;
-  const T amp_s = amp[4 * g], amp_p = amp[4 * g + 1];
-  const int64_t n12 = int64_t(n1) * n2;
-  T acc[4][BB];
-#pragma unroll
-  for (int l = 0; l < 4; ++l)
-#pragma unroll
-    for (int b = 0; b < BB; ++b) acc[l][b] = T(0);
-  const T *__restrict__ w = psi + int64_t(b0) * ld;
-  for (int i = beg + threadIdx.x; i < end; i += kGatherThreads) {
-    const uint32_t wd = pts[i];
-    const int x = int(wd & 1023u), y = int((wd >> 10) & 1023u), z = int((wd >> 20) & 1023u);
-    T c0, c1, c2, c3;
-    ana_values<T>(es, ep, dd, gS, n1, n1 + n2, x, y, z, (wd >> 30) & 1u, wd >> 31, amp_s, amp_p, c0, c1, c2, c3);
-    const int64_t pt = x + int64_t(y) * n1 + int64_t(z) * n12;
-#pragma unroll
-    for (int b = 0; b < BB; ++b) {
-      if (b < nbb) {
-        const T v = w[pt + int64_t(b) * ld];
-        acc[0][b] += c0 * v;
-        acc[1][b] += c1 * v;
-        acc[2][b] += c2 * v;
-        acc[3][b] += c3 * v;
-      }
-    }
-  }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int l = 0; l < 4; ++l)
-#pragma unroll
-    for (int b = 0; b < BB; ++b) {
-      const T v = wave_sum(acc[l][b]);
-      if (lane == 0) red[wid][l * BB + b] = v;
-    }
-  __syncthreads();
-  const int t = threadIdx.x;
-  if (t < 4 * BB) {
-    const int l = t / BB, b = t % BB;
-    if (b < nbb) {
-      T v = T(0);
-#pragma unroll
-      for (int k = 0; k < kGatherThreads / 64; ++k) v += red[k][t];
-      part[(int64_t(s) * 4 + l) * nbc + b0 + b] = v;
-    }
-  }
-}
-
 // C[(g*4 + l)*nbc + b] = iuV dV * sum over the group's segments
 template <typename T>
 __global__ void k_proj_reduce(const T *__restrict__ part, int nbc, int ngroup, const int *__restrict__ grp_seg_off,
@@ -382,40 +324,177 @@ __global__ void k_proj_reduce(const T *__restrict__ part, int nbc, int ngroup, c
   cproj[t] = scale[gl] * v;
 }
 
-// H psi(j, b) (=|+=) sum over the spheres holding j of
-//   s(j) C(g,0,b) + p_x(j) C(g,1,b) + p_y(j) C(g,2,b) + p_z(j) C(g,3,b)
-// One thread per grid point and a tile of BT bands: the projector values of
-// a point are formed once per tile.
-template <typename T, int BT, bool ACC>
-__global__ __launch_bounds__(256) void k_nl_apply(T *__restrict__ hpsi, int64_t ld, int nbc, int64_t N, int n1,
-                                                  int n2, const uint64_t *__restrict__ hits,
-                                                  const T *__restrict__ es, const T *__restrict__ ep,
-                                                  const T *__restrict__ dd, const T *__restrict__ amp, int S,
-                                                  const T *__restrict__ cproj) {
-  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= N) return;
-  const int b0 = blockIdx.y * BT;
-  const int nbt = min(BT, nbc - b0);
-  const int x = int(j % n1), y = int((j / n1) % n2), z = int(j / (int64_t(n1) * n2));
-  T acc[BT];
+// ---- analytic projectors on the matrix cores ----
+//
+// Both products are tiny sparse ones (about 5.6 projector values per grid
+// point), so they run as dense 16x16x4 MFMA tiles over a few zero-padded
+// projectors: the point data is read once per tile and the band dimension
+// rides on the matrix-core columns.  Lane maps (verified on gfx950):
+//   A (16 x 4): lane l holds A[l % 16][l / 16]
+//   B (4 x 16): lane l holds B[l / 16][l % 16]
+//   D (16 x 16), register r of lane l:
+//     f32: D[4 (l / 16) + r][l % 16]      f64: D[4 r + l / 16][l % 16]
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef mf_d4 V;
+  __device__ static V mac(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int l, int r) { return 4 * r + (l >> 4); }
+};
+template <> struct Mfma<float> {
+  typedef mf_f4 V;
+  __device__ static V mac(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+// Projector k (0: s, 1..3: p_x, p_y, p_z) of group g at grid point (x, y, z)
+// with sphere flags fs, fp.
+template <typename T>
+__device__ inline T ana_value_k(const T *__restrict__ es, const T *__restrict__ ep, const T *__restrict__ dd,
+                                int64_t gS, int oy, int oz, int x, int y, int z, bool fs, bool fp, T amp_s,
+                                T amp_p, int k) {
+  const int64_t ix = gS + x, iy = gS + oy + y, iz = gS + oz + z;
+  if (k == 0) return fs ? amp_s * es[ix] * es[iy] * es[iz] : T(0);
+  const int64_t ia = k == 1 ? ix : (k == 2 ? iy : iz);
+  return fp ? amp_p * ep[ix] * ep[iy] * ep[iz] * dd[ia] : T(0);
+}
+
+// Partial projections of one segment (one atom, one plane run):
+//   part[(s*4 + k)*nbc + b] = sum_{j in segment} beta_k(j) psi(j, b)
+// D[band][k] += A[band][point] B[point][k]: one wave, NBT tiles of 16 bands.
+// The A operand puts 16 bands of one point on neighbouring lanes, so psi is
+// first staged through LDS in passes of 32 points, loaded with neighbouring
+// lanes on neighbouring points (rows padded by one element against bank
+// conflicts).
+template <typename T, int NBT>
+__global__ __launch_bounds__(64) void k_proj_m(
+    const T *__restrict__ psi, int64_t ld, int nbc, int n1, int n2, int nseg, const int *__restrict__ seg_order,
+    const int *__restrict__ seg_beg, const int *__restrict__ seg_end, const int *__restrict__ seg_grp,
+    const uint32_t *__restrict__ pts, const T *__restrict__ es, const T *__restrict__ ep,
+    const T *__restrict__ dd, const T *__restrict__ amp, int S, T *__restrict__ part) {
+  using M = Mfma<T>;
+  constexpr int P = 32;  // points per pass
+  __shared__ T tile[NBT][16][P + 1];
+  __shared__ T bsh[4][P + 1];  // projector values of the pass
+  // logical block: segment (in plane order) fastest, then band group
+  const unsigned lblk = xcd_block(blockIdx.x, gridDim.x);
+  const int s = seg_order[lblk % unsigned(nseg)];
+  const int l = threadIdx.x;
+  const int lo = l & 15, hi = l >> 4;
+  const int lp = l & (P - 1), lb = l / P;  // staging: point, band parity
+  const int b0 = int(lblk / unsigned(nseg)) * 16 * NBT;
+  const int beg = seg_beg[s], end = seg_end[s], g = seg_grp[s];
+  const int64_t gS = int64_t(g) * S;
+  const T amp_s = amp[4 * g], amp_p = amp[4 * g + 1];
+  const int64_t n12 = int64_t(n1) * n2;
+  typename M::V acc[NBT];
 #pragma unroll
-  for (int b = 0; b < BT; ++b) acc[b] = (ACC && b < nbt) ? hpsi[j + int64_t(b0 + b) * ld] : T(0);
-  uint64_t hw = hits[j];
-  while (hw) {
-    const uint32_t e = uint32_t(hw & 0xffffu);
-    hw >>= 16;
-    const int g = int(e & 0x3fffu) - 1;
-    T cs, cx, cy, cz;
-    ana_values<T>(es, ep, dd, int64_t(g) * S, n1, n1 + n2, x, y, z, (e >> 14) & 1u, (e >> 15) & 1u, amp[4 * g],
-                  amp[4 * g + 1], cs, cx, cy, cz);
-    const T *__restrict__ c = cproj + int64_t(g) * 4 * nbc + b0;
+  for (int t = 0; t < NBT; ++t) acc[t] = typename M::V{};
+  uint32_t wd_next = pts[beg + lp < end ? beg + lp : beg];
+  for (int i = beg; i < end; i += P) {
+    {
+      const int ip = i + lp;
+      const bool valid = ip < end;
+      const uint32_t wd = wd_next;
+      wd_next = pts[ip + P < end ? ip + P : beg];  // next pass's point, in flight during this one
+      const int x = int(wd & 1023u), y = int((wd >> 10) & 1023u), z = int((wd >> 20) & 1023u);
+      const int64_t lin = x + int64_t(y) * n1 + int64_t(z) * n12;
+      // this lane's two projector values: slots 2 lb, 2 lb + 1
+      const bool fs = valid && ((wd >> 30) & 1u), fp = valid && (wd >> 31);
+      const T pv0 = ana_value_k<T>(es, ep, dd, gS, n1, n1 + n2, x, y, z, fs, fp, amp_s, amp_p, 2 * lb);
+      const T pv1 = ana_value_k<T>(es, ep, dd, gS, n1, n1 + n2, x, y, z, fs, fp, amp_s, amp_p, 2 * lb + 1);
+      T v[NBT][8];
 #pragma unroll
-    for (int b = 0; b < BT; ++b)
-      if (b < nbt) acc[b] += cs * c[b] + cx * c[nbc + b] + cy * c[2 * nbc + b] + cz * c[3 * nbc + b];
+      for (int t = 0; t < NBT; ++t)
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          const int b = b0 + 16 * t + 2 * m + lb;
+          v[t][m] = (valid && b < nbc) ? psi[lin + int64_t(b) * ld] : T(0);
+        }
+      __syncthreads();  // previous pass done with the staging tile
+#pragma unroll
+      for (int t = 0; t < NBT; ++t)
+#pragma unroll
+        for (int m = 0; m < 8; ++m) tile[t][2 * m + lb][lp] = v[t][m];
+      bsh[2 * lb][lp] = pv0;
+      bsh[2 * lb + 1][lp] = pv1;
+      __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < P / 4; ++u) {
+      const int q = 4 * u + hi;  // K index: point of the pass
+      const T bv = lo < 4 ? bsh[lo][q] : T(0);
+#pragma unroll
+      for (int t = 0; t < NBT; ++t) acc[t] = M::mac(tile[t][lo][q], bv, acc[t]);
+    }
   }
+  // D[band][k]: keep k < 4
 #pragma unroll
-  for (int b = 0; b < BT; ++b)
-    if (b < nbt) hpsi[j + int64_t(b0 + b) * ld] = acc[b];
+  for (int t = 0; t < NBT; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int band = b0 + 16 * t + M::row(l, r);
+      if (lo < 4 && band < nbc) part[(int64_t(s) * 4 + lo) * nbc + band] = acc[t][r];
+    }
+}
+
+// H psi(j, b) (=|+=) sum over the atoms g meeting j's tile of
+//   sum_k beta_k(j) C(g, k, b)
+// One wave per 16-point tile: D[band][point] += A[band][k] B[k][point] per
+// atom, NBT tiles of 16 bands.
+template <typename T, int NBT, bool ACC>
+__global__ __launch_bounds__(256) void k_apply_m(
+    T *__restrict__ hpsi, int64_t ld, int nbc, int64_t N, int n1, int n2, int ntile,
+    const int *__restrict__ tile_off, const int *__restrict__ tile_g, const uint64_t *__restrict__ hits,
+    const T *__restrict__ es, const T *__restrict__ ep, const T *__restrict__ dd, const T *__restrict__ amp,
+    int S, const T *__restrict__ cproj) {
+  using M = Mfma<T>;
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= ntile) return;  // whole wave
+  const int l = threadIdx.x & 63;
+  const int lo = l & 15, hi = l >> 4;
+  const int b0 = blockIdx.y * 16 * NBT;
+  const int64_t j = int64_t(tile) * 16 + lo;
+  const bool inside = j < N;
+  const int64_t jj = inside ? j : 0;
+  const int x = int(jj % n1), y = int((jj / n1) % n2), z = int(jj / (int64_t(n1) * n2));
+  const uint64_t hw = inside ? hits[jj] : 0;
+  typename M::V acc[NBT];
+#pragma unroll
+  for (int t = 0; t < NBT; ++t) acc[t] = typename M::V{};
+  const int a0 = tile_off[tile], a1 = tile_off[tile + 1];
+  for (int a = a0; a < a1; ++a) {
+    const int g = tile_g[a];
+    bool fs = false, fp = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t e = uint32_t(hw >> (16 * q)) & 0xffffu;
+      const bool me = int(e & 0x3fffu) == g + 1;
+      fs |= me && ((e >> 14) & 1u);
+      fp |= me && ((e >> 15) & 1u);
+    }
+    const T bv = ana_value_k<T>(es, ep, dd, int64_t(g) * S, n1, n1 + n2, x, y, z, fs, fp, amp[4 * g],
+                                amp[4 * g + 1], hi);
+    const T *__restrict__ c = cproj + (int64_t(g) * 4 + hi) * nbc;
+#pragma unroll
+    for (int t = 0; t < NBT; ++t) {
+      const int b = b0 + 16 * t + lo;
+      const T av = b < nbc ? c[b] : T(0);
+      acc[t] = M::mac(av, bv, acc[t]);
+    }
+  }
+  if (!inside) return;
+#pragma unroll
+  for (int t = 0; t < NBT; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int band = b0 + 16 * t + M::row(l, r);
+      if (band < nbc) {
+        T *dst = hpsi + j + int64_t(band) * ld;
+        *dst = ACC ? *dst + acc[t][r] : acc[t][r];
+      }
+    }
 }
 
 template <typename T> StencilCoef<T> make_coef(const rsg_ctx *ctx) {
@@ -474,7 +553,9 @@ int launch_stencil_t(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, co
   const size_t lds = (2 * size_t(sh.ty + 2 * MD) * n1 + 1) * sizeof(T);
   if (threads > 512 || lds > kMaxDynLds || 2 * MD * n1 > HM * threads)
     return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: stencil tile does not fit");
-  dim3 grid((n2 + sh.ty - 1) / sh.ty, (n3 + sh.zlen - 1) / sh.zlen, nb);
+  const int64_t nblk = int64_t((n2 + sh.ty - 1) / sh.ty) * ((n3 + sh.zlen - 1) / sh.zlen) * nb;
+  if (nblk > 0x7fffffff) return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: too many stencil workgroups");
+  const dim3 grid{unsigned(nblk), 1u, 1u};
   if (accumulate)
     hipLaunchKernelGGL((k_stencil<T, MD, RY, HM, true>), grid, dim3(threads), lds, ctx->stream, psi, hpsi, ld,
                        vloc_of<T>(ctx), make_coef<T>(ctx), n1, n2, n3, sh.ty, sh.zlen, ctx->dead.template as<T>());
@@ -557,16 +638,19 @@ HpsiShape resolve_shape(const rsg_ctx *ctx, const HpsiShape &in, size_t s, int n
   return sh;
 }
 
-template <typename T, int BT>
-void launch_apply_t(rsg_ctx *ctx, AnalyticDev<T> &an, T *hpsi, int64_t ld, int nbc, const T *cproj, bool acc) {
-  dim3 grid(unsigned((ctx->N + 255) / 256), (nbc + BT - 1) / BT);
+// bands per wave = 16 NBT
+template <typename T, int NBT>
+void launch_apply_m(rsg_ctx *ctx, AnalyticDev<T> &an, T *hpsi, int64_t ld, int nbc, const T *cproj, bool acc) {
+  dim3 grid(unsigned((an.ntile + 3) / 4), (nbc + 16 * NBT - 1) / (16 * NBT));
   if (acc)
-    hipLaunchKernelGGL((k_nl_apply<T, BT, true>), grid, dim3(256), 0, ctx->stream, hpsi, ld, nbc, ctx->N, ctx->n[0],
-                       ctx->n[1], an.hits.template as<uint64_t>(), an.es.template as<T>(), an.ep.template as<T>(),
+    hipLaunchKernelGGL((k_apply_m<T, NBT, true>), grid, dim3(256), 0, ctx->stream, hpsi, ld, nbc, ctx->N, ctx->n[0],
+                       ctx->n[1], an.ntile, an.tile_off.template as<int>(), an.tile_g.template as<int>(),
+                       an.hits.template as<uint64_t>(), an.es.template as<T>(), an.ep.template as<T>(),
                        an.dd.template as<T>(), an.amp.template as<T>(), an.S, cproj);
   else
-    hipLaunchKernelGGL((k_nl_apply<T, BT, false>), grid, dim3(256), 0, ctx->stream, hpsi, ld, nbc, ctx->N, ctx->n[0],
-                       ctx->n[1], an.hits.template as<uint64_t>(), an.es.template as<T>(), an.ep.template as<T>(),
+    hipLaunchKernelGGL((k_apply_m<T, NBT, false>), grid, dim3(256), 0, ctx->stream, hpsi, ld, nbc, ctx->N,
+                       ctx->n[0], ctx->n[1], an.ntile, an.tile_off.template as<int>(), an.tile_g.template as<int>(),
+                       an.hits.template as<uint64_t>(), an.es.template as<T>(), an.ep.template as<T>(),
                        an.dd.template as<T>(), an.amp.template as<T>(), an.S, cproj);
 }
 
@@ -593,7 +677,6 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb) {
     if (rc) return rc;
   } else if (ctx->nl_mode != 1 && ctx->have_ana) {
     AnalyticDev<T> &an = ana_of<T>(ctx);
-    constexpr int BB = 8;
     if (ctx->cproj.ensure(size_t(an.ngroup) * 4 * sh.chunk * sizeof(T)) != hipSuccess ||
         ctx->uvpart.ensure(size_t(an.nseg) * 4 * sh.chunk * sizeof(T)) != hipSuccess)
       return ctx->fail(RSG_ERR_HIP, "hpsi: projection scratch allocation failed");
@@ -602,8 +685,9 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb) {
       const int nbc = std::min(sh.chunk, nb - c0);
       const T *pc = psi + int64_t(c0) * ld;
       T *hc = hpsi + int64_t(c0) * ld;
-      hipLaunchKernelGGL((k_proj_a<T, BB>), dim3(an.nseg, (nbc + BB - 1) / BB), dim3(kGatherThreads), 0,
-                         ctx->stream, pc, ld, nbc, ctx->n[0], ctx->n[1], an.seg_beg.template as<int>(),
+      hipLaunchKernelGGL((k_proj_m<T, 2>), dim3(unsigned(an.nseg * ((nbc + 31) / 32))), dim3(64), 0, ctx->stream, pc,
+                         ld, nbc, ctx->n[0], ctx->n[1], an.nseg, an.seg_order.template as<int>(),
+                         an.seg_beg.template as<int>(),
                          an.seg_end.template as<int>(), an.seg_grp.template as<int>(),
                          an.pts.template as<uint32_t>(), an.es.template as<T>(), an.ep.template as<T>(),
                          an.dd.template as<T>(), an.amp.template as<T>(), an.S, part);
@@ -616,9 +700,9 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb) {
         if (rc) return rc;
       }
       switch (sh.bt) {
-        case 4: launch_apply_t<T, 4>(ctx, an, hc, ld, nbc, cp, nl_after); break;
-        case 16: launch_apply_t<T, 16>(ctx, an, hc, ld, nbc, cp, nl_after); break;
-        default: launch_apply_t<T, 8>(ctx, an, hc, ld, nbc, cp, nl_after); break;
+        case 4: launch_apply_m<T, 1>(ctx, an, hc, ld, nbc, cp, nl_after); break;
+        case 16: launch_apply_m<T, 4>(ctx, an, hc, ld, nbc, cp, nl_after); break;
+        default: launch_apply_m<T, 2>(ctx, an, hc, ld, nbc, cp, nl_after); break;
       }
       if (!nl_after) {
         int rc = launch_stencil<T>(ctx, pc, hc, ld, nbc, sh, 1);

@@ -250,7 +250,8 @@ int build_gth_tables(const rsg_ctx *ctx, int natom, const double *frac,
 
 namespace {
 
-constexpr int kSegLen = 4096;  // points per gather segment
+constexpr int kSegLen = 4096;       // points per gather segment
+constexpr int kPlaneSegMax = 1024;  // points per analytic segment (one plane)
 
 struct HostLayouts {
   std::vector<int> grp_lma0, grp_nl, grp_seg_off{0}, seg_beg, seg_end, seg_grp, grp_pts, lma_grp;
@@ -498,16 +499,49 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
     std::sort(pts.begin(), pts.end());
     const int pbase = int(out.pts.size());
     for (auto &pw : pts) out.pts.push_back(pw.second);
-    for (int b = 0; b < int(pts.size()); b += kSegLen) {
+    // one segment per plane run (grid order is plane-major), capped in length
+    for (int b = 0; b < int(pts.size());) {
+      const int64_t plane = pts[size_t(b)].first / n12;
+      int e = b + 1;
+      while (e < int(pts.size()) && e - b < kPlaneSegMax && pts[size_t(e)].first / n12 == plane) ++e;
       out.seg_beg.push_back(pbase + b);
-      out.seg_end.push_back(pbase + std::min(int(pts.size()), b + kSegLen));
+      out.seg_end.push_back(pbase + e);
       out.seg_grp.push_back(g);
+      b = e;
     }
     out.grp_seg_off.push_back(int(out.seg_beg.size()));
     ++g;
   }
   out.ngroup = g;
-  return g > 0 ? RSG_OK : RSG_ERR_UNSUPPORTED;
+  if (g == 0) return RSG_ERR_UNSUPPORTED;
+  // launch order: by plane, then atom
+  out.seg_order.resize(out.seg_beg.size());
+  for (size_t i = 0; i < out.seg_order.size(); ++i) out.seg_order[i] = int(i);
+  std::stable_sort(out.seg_order.begin(), out.seg_order.end(), [&](int a, int b) {
+    return (out.pts[size_t(out.seg_beg[size_t(a)])] >> 20 & 1023u) < (out.pts[size_t(out.seg_beg[size_t(b)])] >> 20 & 1023u);
+  });
+  // atoms meeting each 16-point tile
+  const int64_t ntile = (ctx->N + 15) / 16;
+  out.tile_off.assign(size_t(ntile) + 1, 0);
+  out.tile_g.clear();
+  for (int64_t t = 0; t < ntile; ++t) {
+    int list[64];
+    int nl = 0;
+    for (int64_t j = t * 16; j < std::min<int64_t>(ctx->N, t * 16 + 16); ++j) {
+      uint64_t hw = out.hits[size_t(j)];
+      while (hw) {
+        const int gg = int(hw & 0x3fffu) - 1;
+        hw >>= 16;
+        bool seen = false;
+        for (int i = 0; i < nl; ++i) seen |= list[i] == gg;
+        if (!seen) list[nl++] = gg;
+      }
+    }
+    std::sort(list, list + nl);
+    out.tile_g.insert(out.tile_g.end(), list, list + nl);
+    out.tile_off[size_t(t) + 1] = int(out.tile_g.size());
+  }
+  return RSG_OK;
 }
 
 namespace {
@@ -534,7 +568,11 @@ template <typename T> int upload_analytic_t(rsg_ctx *ctx, AnalyticDev<T> &d, con
   rc |= up(d.seg_end, h.seg_end.data(), h.seg_end.size() * sizeof(int));
   rc |= up(d.seg_grp, h.seg_grp.data(), h.seg_grp.size() * sizeof(int));
   rc |= up(d.grp_seg_off, h.grp_seg_off.data(), h.grp_seg_off.size() * sizeof(int));
+  rc |= up(d.seg_order, h.seg_order.data(), h.seg_order.size() * sizeof(int));
   rc |= up(d.hits, h.hits.data(), h.hits.size() * sizeof(uint64_t));
+  rc |= up(d.tile_off, h.tile_off.data(), h.tile_off.size() * sizeof(int));
+  rc |= up(d.tile_g, h.tile_g.data(), h.tile_g.size() * sizeof(int));
+  d.ntile = int(h.tile_off.size()) - 1;
   return rc ? ctx->fail(RSG_ERR_HIP, "upload of analytic projector data failed") : RSG_OK;
 }
 
