@@ -3,14 +3,23 @@
 A step is one Hamiltonian application H psi over all 160 bands of the Si64
 cell (2x2x2 diamond supercell, a = 20.52 bohr, atoms displaced by N(0, 0.05
 bohr), 13-point-per-axis FD kinetic, GTH Si projectors), with psi and H psi
-resident in HBM.  `value` is algorithmic GB/s:
+resident in HBM.  `value` is algorithmic GB/s of the whole call:
 
     B = 2 s N nb  (psi read, H psi written)  + s N (Vloc)  + (s_uv + 4) E
 
 with E the projector-table entries (uVk value + JJP index), divided by the
 measured time per call (HIP events on the library's stream).  The FP32 twin
-is reported beside it.  The CPU baseline is the FP64 oracle hpsi on the
-host's cores, on a bounded band sample of the same cell.
+is reported beside it.
+
+`roofline` is for the dominant kernel of the call, the accumulating
+kinetic + Vloc stencil (k_stencil), timed live here through rsg_op_kinetic
+(the same kernel and launch shape hpsi uses) with HIP events on the
+library's stream; its algorithmic bytes are 2 s N nb + s N.  `traffic` is
+that kernel's HBM-side bytes per launch from the committed rocprofv3
+FETCH_SIZE / WRITE_SIZE summary (scaled by a device-to-device copy of known
+size), or null when the summary is absent.  The SCF leg times whole SCF runs
+of Si216 in FP64 and in mixed precision.  The CPU baseline is the FP64
+oracle hpsi on the host's cores, on a bounded band sample of the same cell.
 """
 from __future__ import annotations
 
@@ -33,36 +42,68 @@ def hpsi_bytes(N: int, nb: int, E: int, s: int) -> float:
     return 2.0 * s * N * nb + s * N + (s + 4) * E
 
 
-def gpu_leg(cell, vloc, steps: int, warmup: int, dtype):
+def gpu_leg(cell, vloc, steps: int, warmup: int, dtype, pin=None):
     from rsdft_develop_amd import gpu, hip, systems
 
     ctx = gpu.Context(0).setup(cell, vloc)
     prec = 64 if dtype == np.float64 else 32
-    ctx.autotune_hpsi(prec, cell.nband, reps=3)  # launch shape from the library's autotuner
+    if pin:  # launch shape given (ry, ty, zlen, chunk, bt, order): a counter run repeats a timed run's shape
+        ry, ty, zlen, chunk, bt, order = pin
+        ctx.set_hpsi_shape(prec, ry=ry, ty=ty, zlen=zlen, chunk=chunk, bt=bt)
+        ctx.set_hpsi_order(prec, order)
+    else:
+        ctx.autotune_hpsi(prec, cell.nband, reps=3)  # launch shape from the library's autotuner
     shape = ctx.hpsi_shape(prec, cell.nband)
     tabs = ctx.projector_tables()
     E = int(tabs["nentries"])
     N, nb = cell.npoints, cell.nband
     psi = hip.DeviceArray.from_numpy(systems.random_bands(N, nb, seed=2024), dtype=dtype)
     hpsi = hip.DeviceArray((N, nb), dtype=dtype)
-    for _ in range(warmup):
-        ctx.hpsi_device(psi.ptr, hpsi.ptr, N, nb, dtype=dtype)
-    ctx.synchronize()
-    t0, t1 = hip.Event(), hip.Event()
-    wall0 = time.perf_counter()
-    t0.record(ctx.stream)
-    for _ in range(steps):
-        ctx.hpsi_device(psi.ptr, hpsi.ptr, N, nb, dtype=dtype)
-    t1.record(ctx.stream)
-    t1.synchronize()
-    wall = time.perf_counter() - wall0
-    ms = t0.elapsed_ms(t1) / steps
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        ctx.synchronize()
+        t0, t1 = hip.Event(), hip.Event()
+        wall0 = time.perf_counter()
+        t0.record(ctx.stream)
+        for _ in range(steps):
+            fn()
+        t1.record(ctx.stream)
+        t1.synchronize()
+        return t0.elapsed_ms(t1) / steps, time.perf_counter() - wall0
+
+    ms, wall = timed(lambda: ctx.hpsi_device(psi.ptr, hpsi.ptr, N, nb, dtype=dtype))
+    # the dominant kernel alone: the accumulating stencil as hpsi launches it
+    ms_k, _ = timed(lambda: ctx.op_kinetic_device(psi.ptr, hpsi.ptr, N, nb, dtype=dtype, accumulate=True))
     s = 8 if dtype == np.float64 else 4
     B = hpsi_bytes(N, nb, E, s)
+    Bk = 2.0 * s * N * nb + s * N
+    # one device-to-device copy of s N nb bytes: the known-size reference a
+    # counter run of this script scales FETCH_SIZE / WRITE_SIZE by
+    hip.copy_d2d(hpsi.ptr, psi.ptr, s * N * nb)
     psi.free()
     hpsi.free()
     ctx.close()
-    return dict(ms=ms, gbs=B / (ms * 1e-3) / 1e9, bytes=B, E=E, wall=wall, shape=shape)
+    return dict(ms=ms, gbs=B / (ms * 1e-3) / 1e9, bytes=B, E=E, wall=wall, shape=shape,
+                kernel_us=ms_k * 1e3, kernel_bytes=Bk, kernel_gbs=Bk / (ms_k * 1e-3) / 1e9)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stencil_si64_72cubed_160bands_pmc.md")
+
+
+def kernel_traffic(tag: str):
+    """HBM-side bytes per launch of the stencil from the committed PMC
+    summary (line 'TRAFFIC_BYTES_PER_LAUNCH <tag> <bytes>'), else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "TRAFFIC_BYTES_PER_LAUNCH" and parts[1] == tag:
+                    return float(parts[2])
+    except OSError:
+        return None
+    return None
 
 
 def cpu_leg(cell, vloc, target_s: float = 12.0):
@@ -120,11 +161,13 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     progress(f"{tag}: start")
     t0 = time.perf_counter()
     run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=etol, max_iter=max_iter, mixed=mixed, imix=30,
-                                      pc_kind=pc_kind), ctx=ctx)
+                                      pc_kind=pc_kind, start="uniform"), ctx=ctx)
+    progress(f"{tag}: set-up {run.setup_s:.2f} s")
     res = run.run(callback=lambda r: progress(f"{tag}: iter {r.iter} prec {r.precision} dE/atom {r.de_per_atom:.2e}"))
     wall = time.perf_counter() - t0
     its = res.iterations
-    out = dict(seconds=round(wall, 3), iterations=len(its), converged=bool(res.converged), etot=res.etot,
+    out = dict(seconds=round(wall, 3), setup_s=round(res.setup, 3), iterations=len(its), converged=bool(res.converged),
+               etot=res.etot,
                fp32_iterations=sum(1 for r in its if r.precision == 32),
                per_phase_s={k: round(sum(r.times.get(k, 0.0) for r in its), 3)
                             for k in ("sd", "cg", "gs", "esp", "eto", "pot")},
@@ -140,6 +183,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--scf", type=int, default=3, help="supercell repeat for the SCF-time leg (3 = Si216); 0 skips")
+    ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order")
+    ap.add_argument("--shape32", default=None, help="pin the FP32 hpsi launch shape: ry,ty,zlen,chunk,bt,order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path is not sharded in this build); got --gpus {args.gpus}",
@@ -150,9 +195,10 @@ def main() -> int:
     cell = systems.silicon_supercell(2, nband=160, displacement=0.05, seed=0)
     vloc = systems.model_vloc(cell)
     progress("hpsi FP64")
-    r64 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float64)
+    pin = lambda t: [int(v) for v in t.split(",")] if t else None  # noqa: E731
+    r64 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float64, pin(args.shape64))
     progress("hpsi FP32")
-    r32 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float32)
+    r32 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float32, pin(args.shape32))
     progress("CPU baseline")
     cpu = None if args.no_cpu else cpu_leg(cell, vloc)
     scf_out = None
@@ -162,7 +208,7 @@ def main() -> int:
         de = float(np.abs(np.array(mixed.pop("esp")) - np.array(plain.pop("esp"))).max())
         natom = 8 * args.scf ** 3
         scf_out = {"workload": f"Si{natom} {36 * args.scf}^3 grid, {NBANDS[args.scf]} bands, EKBT 0.01 Ha, "
-                               f"IMIX 30, to dE < 1e-6 Ha/atom",
+                               f"IMIX 30, uniform random start bands drawn on the device, to dE < 1e-6 Ha/atom",
                    "fp64": plain, "mixed": mixed,
                    "speedup_mixed_vs_fp64": round(plain["seconds"] / mixed["seconds"], 3),
                    "mixed_max_eig_diff_vs_fp64_Ha": de,
@@ -182,13 +228,17 @@ def main() -> int:
         "data": "synthetic (seeded N(0,1) bands, model Vloc, GTH Si tables)",
         "config": {"workload": "hpsi Si64 2x2x2 a=20.52 bohr, 72^3 grid, 160 bands, Md=6",
                    "grid": [72, 72, 72], "bands": 160, "projector_entries": r64["E"]},
-        "roofline": {"bound": "hbm", "achieved": round(r64["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r64["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "hpsi call (k_nl_gather + k_kinetic_nl)",
-                     "algorithmic_bytes": r64["bytes"]},
+        "roofline": {"bound": "hbm", "achieved": round(r64["kernel_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r64["kernel_gbs"] / HBM_PEAK_GBS, 4), "traffic": kernel_traffic("f64"),
+                     "kernel": "k_stencil<double, 6, ry, hm, accumulate> (13-point FD kinetic + Vloc)",
+                     "kernel_avg_us": round(r64["kernel_us"], 1),
+                     "algorithmic_bytes": r64["kernel_bytes"]},
+        "hpsi_call": {"algorithmic_bytes": r64["bytes"], "frac": round(r64["gbs"] / HBM_PEAK_GBS, 4)},
         "hpsi_shape": r64["shape"],
         "f32": {"value": round(r32["gbs"], 1), "ms_per_step": round(r32["ms"], 4),
-                "frac": round(r32["gbs"] / HBM_PEAK_GBS, 4), "hpsi_shape": r32["shape"]},
+                "frac": round(r32["gbs"] / HBM_PEAK_GBS, 4), "hpsi_shape": r32["shape"],
+                "kernel_avg_us": round(r32["kernel_us"], 1), "kernel_gbs": round(r32["kernel_gbs"], 1),
+                "kernel_traffic": kernel_traffic("f32")},
         "cpu_baseline": cpu,
         "scf": scf_out,
     }

@@ -102,6 +102,15 @@ int rsg_hpsi_f64(rsg_ctx *ctx, const double *d_psi, double *d_hpsi,
  * to FP32).  No reference counterpart (mixed-precision addition). */
 int rsg_hpsi_f32(rsg_ctx *ctx, const float *d_psi, float *d_hpsi, int64_t ld,
                  int nb);
+/* Local part of hpsi alone: hpsi (+)= (T + Vloc) psi, accumulating when
+ * accumulate != 0.  Replaces d_op_kinetic with its Vloc argument
+ * (kinetic_module.f90:214-243), which hamiltonian0
+ * (hamiltonian_module.f90:62-116) calls before op_nonlocal.  Same kernel
+ * and launch shape as rsg_hpsi_*; asynchronous on the context stream. */
+int rsg_op_kinetic_f64(rsg_ctx *ctx, const double *d_psi, double *d_hpsi,
+                       int64_t ld, int nb, int accumulate);
+int rsg_op_kinetic_f32(rsg_ctx *ctx, const float *d_psi, float *d_hpsi,
+                       int64_t ld, int nb, int accumulate);
 /* hpsi on host arrays (copied through device scratch); the entry point the
  * Fortran `hamiltonian` shim binds.  Synchronous. */
 int rsg_hpsi_f64_host(rsg_ctx *ctx, const double *psi, double *hpsi,
@@ -126,6 +135,14 @@ int rsg_get_hpsi_shape(rsg_ctx *ctx, int precision, int nb, int *ry, int *ty,
  * bands) and keep the fastest for this context.  best_ms may be NULL. */
 int rsg_autotune_hpsi(rsg_ctx *ctx, int precision, int nb, int reps,
                       float *best_ms);
+
+/* Start bands on the device: columns n0 .. n0+nb-1 (0-based) of the band
+ * store get uniform [0, 1) values, one counter-based draw per point and band
+ * (SplitMix64 of seed + (b N + j + 1) * 0x9E3779B97F4A7C15, top 53 bits), so
+ * the result depends only on seed.  Replaces random_initial_wf
+ * (wf_module.f90:166-191), which draws uniform numbers per point and band
+ * with the Fortran generator.  Asynchronous on the context stream. */
+int rsg_random_bands(rsg_ctx *ctx, int n0, int nb, uint64_t seed);
 
 /* Device-resident wavefunction store, the counterpart of
  * wf_module::unk(ML_0:ML_1, MB, k, s) (wf_module.f90:131).  Upload and

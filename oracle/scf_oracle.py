@@ -283,7 +283,7 @@ class PulayG:
 
 
 def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-8, max_iter=200,
-            init_width=1.0, seed=0, gs_variant="t", bands=None, callback=None, imix=10):
+            init_width=1.0, seed=0, gs_variant="t", bands=None, callback=None, imix=10, start="normal"):
     """Plain FP64 SCF in the reference's order; returns a dict of results.
 
     Stops when the Kohn-Sham energy of the output density changes by less
@@ -302,7 +302,12 @@ def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-
     exc_in = cell.dv * ed.sum()
     vloc = vi + vh + vxc
     ham = pyoracle.Ham(cell, vloc=vloc)
-    u = bands if bands is not None else systems.random_bands(cell.npoints, cell.nband, seed=seed)
+    if bands is not None:
+        u = bands
+    elif start == "uniform":  # random_initial_wf (wf_module.f90:166-191), the library's counter draw
+        u = systems.counter_bands(cell.npoints, cell.nband, seed)
+    else:
+        u = systems.random_bands(cell.npoints, cell.nband, seed=seed)
     u = ham.gram_schmidt(u, gs_variant)
     mixer = PulayG(cell, rho, mmix, beta) if imix == 30 else Pulay(rho, mmix, beta)
     hist = []

@@ -245,6 +245,18 @@ int rsg_hpsi_f32(rsg_ctx *ctx, const float *psi, float *hpsi, int64_t ld, int nb
   return rsg::launch_hpsi<float>(ctx, psi, hpsi, ld, nb);
 }
 
+int rsg_op_kinetic_f64(rsg_ctx *ctx, const double *psi, double *hpsi, int64_t ld, int nb, int accumulate) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::launch_kinetic<double>(ctx, psi, hpsi, ld, nb, accumulate);
+}
+
+int rsg_op_kinetic_f32(rsg_ctx *ctx, const float *psi, float *hpsi, int64_t ld, int nb, int accumulate) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::launch_kinetic<float>(ctx, psi, hpsi, ld, nb, accumulate);
+}
+
 int rsg_hpsi_f64_host(rsg_ctx *ctx, const double *psi, double *hpsi, int64_t ld, int nb) {
   if (!ctx || !psi || !hpsi) return RSG_ERR_ARG;
   if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "hpsi: grid not set");
@@ -334,6 +346,12 @@ int rsg_upload_bands(rsg_ctx *ctx, const double *u, int64_t ld, int n0, int nb) 
   RSG_HIP(hipMemcpy2DAsync(d, ctx->N * 8, u, ld * 8, ctx->N * 8, nb, hipMemcpyHostToDevice, ctx->stream));
   RSG_HIP(hipStreamSynchronize(ctx->stream));
   return RSG_OK;
+}
+
+int rsg_random_bands(rsg_ctx *ctx, int n0, int nb, uint64_t seed) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::random_bands(ctx, n0, nb, seed);
 }
 
 int rsg_download_bands(rsg_ctx *ctx, double *u, int64_t ld, int n0, int nb) {

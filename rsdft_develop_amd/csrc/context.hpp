@@ -225,6 +225,8 @@ int upload_analytic(rsg_ctx *ctx, const HostAnalytic &h);
 namespace rsg {
 template <typename T>
 int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb);
+template <typename T>
+int launch_kinetic(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, int accumulate);
 int resolved_shape(rsg_ctx *ctx, int precision, int nb, HpsiShape *out);
 int autotune_hpsi(rsg_ctx *ctx, int precision, int nb, int reps, float *best_ms);
 }

@@ -664,6 +664,23 @@ void launch_scatter_t(rsg_ctx *ctx, NonlocalDev<T> &nl, T *hpsi, int64_t ld, int
 
 }  // namespace
 
+// Local part only: hpsi (+)= (T + Vloc) psi with the stencil kernel and the
+// launch shape hpsi uses.
+template <typename T>
+int launch_kinetic(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, int accumulate) {
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "op_kinetic: grid not set");
+  if (nb < 0 || ld < ctx->N) return ctx->fail(RSG_ERR_ARG, "op_kinetic: bad nb or ld");
+  if (nb == 0) return RSG_OK;
+  if (nb > 65535) return ctx->fail(RSG_ERR_ARG, "op_kinetic: nb above 65535");
+  if (!psi || !hpsi) return ctx->fail(RSG_ERR_ARG, "op_kinetic: null buffer");
+  const HpsiShape sh = resolve_shape(ctx, shape_of<T>(ctx), sizeof(T), nb, false);
+  int rc = launch_stencil<T>(ctx, psi, hpsi, ld, nb, sh, accumulate ? 1 : 0);
+  if (rc) return rc;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ctx->fail(RSG_ERR_HIP, std::string("op_kinetic launch: ") + hipGetErrorString(e));
+  return RSG_OK;
+}
+
 template <typename T>
 int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb) {
   if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "hpsi: grid not set");
@@ -878,5 +895,6 @@ int autotune_hpsi(rsg_ctx *ctx, int precision, int nb, int reps, float *best_ms)
 #endif  // RSG_HPSI_TYPE_IS_DOUBLE
 
 template int launch_hpsi<RSG_HPSI_TYPE>(rsg_ctx *, const RSG_HPSI_TYPE *, RSG_HPSI_TYPE *, int64_t, int);
+template int launch_kinetic<RSG_HPSI_TYPE>(rsg_ctx *, const RSG_HPSI_TYPE *, RSG_HPSI_TYPE *, int64_t, int, int);
 
 }  // namespace rsg

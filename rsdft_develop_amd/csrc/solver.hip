@@ -649,6 +649,35 @@ __global__ void k_resid32(const double *__restrict__ hx, const double *__restric
 }
 }  // namespace
 
+// Uniform [0, 1) start bands, one counter-based value per (band, point):
+// u(j, b) = splitmix64(seed + (b N + j + 1) * golden) / 2^64 with the 53 top
+// bits kept, so the values do not depend on the launch shape.  The same
+// formula runs on the host (systems.counter_bands).
+__global__ __launch_bounds__(256) void k_random_bands(double *__restrict__ u, int64_t N, int n0, uint64_t seed) {
+  const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (j >= N) return;
+  const int64_t b = int64_t(n0) + blockIdx.y;
+  uint64_t z = seed + uint64_t(b * N + j + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  u[b * N + j] = double(z >> 11) * 0x1.0p-53;
+}
+
+// Fills columns n0 .. n0+nb-1 of the band store (random_initial_wf,
+// wf_module.f90:166-191: uniform random numbers per point and band).
+int random_bands(rsg_ctx *ctx, int n0, int nb, uint64_t seed) {
+  if (ctx->mb == 0) return ctx->fail(RSG_ERR_STATE, "random_bands: no band store");
+  if (n0 < 0 || nb < 0 || n0 + nb > ctx->mb) return ctx->fail(RSG_ERR_ARG, "random_bands: band range out of bounds");
+  if (nb == 0) return RSG_OK;
+  if (nb > 65535) return ctx->fail(RSG_ERR_ARG, "random_bands: nb above 65535");
+  hipLaunchKernelGGL(k_random_bands, dim3(unsigned((ctx->N + 255) / 256), unsigned(nb)), dim3(256), 0, ctx->stream,
+                     ctx->unk.as<double>(), ctx->N, n0, seed);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ctx->fail(RSG_ERR_HIP, std::string("random_bands: ") + hipGetErrorString(e));
+  return RSG_OK;
+}
+
 // CG gradients g_n = -2 (H u_n - e_n u_n), e_n = dV <u_n|H u_n>, of the
 // whole store in FP32 (cg_module.f90:207, the first gradient of a CG call):
 // the residuals the learned preconditioner is trained on.

@@ -10,6 +10,7 @@ int conjugate_gradient_f32(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double
 int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double shift, double vweight);
 int band_residuals_f32(rsg_ctx *ctx, float *g);
 int esp_calc(rsg_ctx *ctx, double *esp);
+int random_bands(rsg_ctx *ctx, int n0, int nb, uint64_t seed);
 void solver_release(rsg_ctx *ctx);
 
 // Per-band dot products alpha * sum_i x_q(i,b) y_q(i,b) for up to 6 pairs

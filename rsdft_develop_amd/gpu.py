@@ -71,6 +71,9 @@ def load_library():
         "rsg_get_projectors": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
         "rsg_hpsi_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
         "rsg_hpsi_f32": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
+        "rsg_random_bands": (_c_int, [_vp, _c_int, _c_int, ctypes.c_uint64]),
+        "rsg_op_kinetic_f64": (_c_int, [_vp, _vp, _vp, ctypes.c_int64, _c_int, _c_int]),
+        "rsg_op_kinetic_f32": (_c_int, [_vp, _vp, _vp, ctypes.c_int64, _c_int, _c_int]),
         "rsg_hpsi_f64_host": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
         "rsg_set_hpsi_shape": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
         "rsg_get_hpsi_shape": (_c_int, [_vp, _c_int, _c_int, _P(_c_int), _P(_c_int), _P(_c_int), _P(_c_int), _P(_c_int)]),
@@ -256,6 +259,11 @@ class Context:
         fn = self.L.rsg_hpsi_f64 if np.dtype(dtype) == np.float64 else self.L.rsg_hpsi_f32
         self._check(fn(self.h, _vp(d_psi), _vp(d_hpsi), int(ld), int(nb)))
 
+    def op_kinetic_device(self, d_psi: int, d_hpsi: int, ld: int, nb: int, dtype=np.float64, accumulate=False):
+        """hpsi (+)= (T + Vloc) psi on device arrays (the local part of hpsi)."""
+        fn = self.L.rsg_op_kinetic_f64 if dtype == np.float64 else self.L.rsg_op_kinetic_f32
+        self._check(fn(self.h, d_psi, d_hpsi, int(ld), int(nb), int(bool(accumulate))))
+
     def set_hpsi_shape(self, precision=64, ry=0, ty=0, zlen=0, chunk=0, bt=0):
         self._check(self.L.rsg_set_hpsi_shape(self.h, int(precision), int(ry), int(ty), int(zlen), int(chunk), int(bt)))
 
@@ -294,6 +302,12 @@ class Context:
     def upload_bands(self, u, n0: int = 0):
         u = np.asfortranarray(u, dtype=np.float64)
         self._check(self.L.rsg_upload_bands(self.h, _ptr(u), u.shape[0], int(n0), u.shape[1]))
+
+    def random_bands(self, seed: int, n0: int = 0, nb: int | None = None):
+        """Uniform [0, 1) start bands generated on the device (values equal to
+        systems.counter_bands(N, nb, seed) columns n0 ..)."""
+        nb = self.mb - n0 if nb is None else nb
+        self._check(self.L.rsg_random_bands(self.h, int(n0), int(nb), int(seed) & 0xFFFFFFFFFFFFFFFF))
 
     def download_bands(self, n0: int = 0, nb: int | None = None):
         nb = self.mb - n0 if nb is None else nb

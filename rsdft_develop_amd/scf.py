@@ -30,7 +30,9 @@ class ScfParams:
     etol_per_atom: float = 1e-8   # stop when |Etot_k - Etot_(k-1)| / natom < etol
     max_iter: int = 200     # DITER
     init_width: float = 1.0  # width (bohr) of the initial atomic Gaussians
-    seed: int = 0           # N(0,1) initial bands
+    seed: int = 0           # seed of the initial bands
+    start: str = "normal"   # initial bands: "normal" N(0,1) from the host, "uniform" [0, 1) drawn on the device
+                            # (random_initial_wf, wf_module.f90:166-191)
     mixed: bool = False     # FP32 CG/hpsi while dE/atom >= fp64_below, FP64 after
     fp64_below: float = 1e-4
 
@@ -58,6 +60,7 @@ class ScfResult:
     occ: np.ndarray
     eewald: float
     wall: float
+    setup: float = 0.0      # set-up seconds before the first iteration (tables, density, start bands, GS)
 
     @property
     def niter(self) -> int:
@@ -106,6 +109,7 @@ class SCF:
 
     def __init__(self, cell: systems.Cell, params: ScfParams | None = None, ctx: gpu.Context | None = None,
                  bands: np.ndarray | None = None):
+        t_setup = time.perf_counter()
         self.cell = cell
         self.p = params or ScfParams()
         self.ctx = ctx or gpu.Context(0)
@@ -121,10 +125,17 @@ class SCF:
         if self.p.pc_kind == 5:
             c.lpc_set_potential()  # the network's potential input follows Vloc
         c.alloc_bands(cell.nband)
-        u0 = bands if bands is not None else systems.random_bands(cell.npoints, cell.nband, seed=self.p.seed)
-        c.upload_bands(u0)
+        if bands is None and self.p.start == "uniform":
+            c.random_bands(self.p.seed)
+        else:
+            if self.p.start not in ("normal", "uniform"):
+                raise ValueError(f"start must be 'normal' or 'uniform', got {self.p.start!r}")
+            u0 = bands if bands is not None else systems.random_bands(cell.npoints, cell.nband, seed=self.p.seed)
+            c.upload_bands(u0)
         c.gram_schmidt(1, cell.nband)   # main.f90:392
         c.mix_init(self.p.mmix, self.p.beta, self.p.imix)
+        c.synchronize()
+        self.setup_s = time.perf_counter() - t_setup
 
     def step(self, it: int, e_prev: float | None, precision: int = 64) -> ScfIteration:
         c, p = self.ctx, self.p
@@ -189,4 +200,4 @@ class SCF:
             if conv:
                 break
         wall = time.perf_counter() - w0
-        return ScfResult(conv, its, its[-1].etot, its[-1].fene, its[-1].esp, self.occ, self.eewald, wall)
+        return ScfResult(conv, its, its[-1].etot, its[-1].fene, its[-1].esp, self.occ, self.eewald, wall, self.setup_s)

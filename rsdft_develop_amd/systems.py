@@ -153,6 +153,24 @@ def silicon_supercell(
     )
 
 
+def counter_bands(npoints: int, nband: int, seed: int, n0: int = 0) -> np.ndarray:
+    """Uniform [0, 1) band block, column-major (npoints, nband), band index
+    n0 + b: the host twin of the library's device generator
+    (rsg_random_bands), SplitMix64 of seed + (band * npoints + j + 1) *
+    0x9E3779B97F4A7C15 with the top 53 bits kept."""
+    out = np.empty((npoints, nband), dtype=np.float64, order="F")
+    j = np.arange(npoints, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        for b in range(nband):
+            z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + (np.uint64(n0 + b) * np.uint64(npoints) + j + np.uint64(1)) \
+                * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z ^= z >> np.uint64(31)
+            out[:, b] = (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return out
+
+
 def random_bands(npoints: int, nband: int, seed: int, dtype=np.float64) -> np.ndarray:
     """Seeded i.i.d. N(0,1) band block, column-major ``(npoints, nband)``."""
     rng = np.random.default_rng(seed)

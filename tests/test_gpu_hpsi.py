@@ -113,6 +113,32 @@ def test_hpsi_device_f64_and_f32_with_padded_ld(orc, gpu_mod):
         assert np.all(np.isnan(got[N:]))  # padding rows untouched
 
 
+def test_op_kinetic_is_the_local_part(orc, gpu_mod):
+    """op_kinetic alone gives (T + Vloc) psi; accumulated onto V_NL psi it
+    gives the whole H psi."""
+    from rsdft_develop_amd import hip
+
+    cell = systems.silicon_supercell(1, displacement=0.05, seed=3)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    nb, N = 7, cell.npoints
+    psi = systems.random_bands(N, nb, seed=5)
+    loc = ham.kinetic_local(psi)
+    nl = ham.nonlocal_(psi)
+    for dt, rel in ((np.float64, 1e-12), (np.float32, 2e-5)):
+        d_in = hip.DeviceArray.from_numpy(psi, dtype=dt)
+        d_out = hip.DeviceArray((N, nb), dtype=dt)
+        d_out.fill_bytes(0xFF)
+        ctx.op_kinetic_device(d_in.ptr, d_out.ptr, N, nb, dtype=dt)
+        ctx.synchronize()
+        _close(d_out.to_numpy().astype(np.float64), loc, rel)
+        d_acc = hip.DeviceArray.from_numpy(nl, dtype=dt)
+        ctx.op_kinetic_device(d_in.ptr, d_acc.ptr, N, nb, dtype=dt, accumulate=True)
+        ctx.synchronize()
+        _close(d_acc.to_numpy().astype(np.float64), loc + nl, rel)
+        for a in (d_in, d_out, d_acc):
+            a.free()
+
+
 def test_hpsi_errors(gpu_mod):
     ctx = gpu_mod.Context()
     a = np.zeros((8, 1), order="F")

@@ -179,3 +179,37 @@ def test_si8_mixed_precision_scf_reaches_fp64_result(si8_ref):
     assert precs[0] == 32 and precs[-1] == 64 and precs == sorted(precs)
     assert abs(res.etot - ref["etot"]) / cell.natom < 1e-7
     assert_levels_match(res.esp, ref["esp"], shell_tol=1e-7, level_tol=5e-6)
+
+
+def test_device_start_bands_equal_host_twin():
+    """rsg_random_bands and systems.counter_bands give identical bits."""
+    from rsdft_develop_amd import gpu
+
+    cell = systems.silicon_supercell(1, ngrid_per_cell=24, nband=6)
+    ctx = gpu.Context()
+    ctx.set_grid(cell.ngrid, cell.aa, cell.md)
+    ctx.alloc_bands(6)
+    ctx.random_bands(12345)
+    ctx.synchronize()
+    got = ctx.download_bands()
+    np.testing.assert_array_equal(got, systems.counter_bands(cell.npoints, 6, 12345))
+    ctx.random_bands(7, n0=2, nb=3)  # columns 2..4 only
+    ctx.synchronize()
+    got2 = ctx.download_bands()
+    np.testing.assert_array_equal(got2[:, 2:5], systems.counter_bands(cell.npoints, 3, 7, n0=2))
+    np.testing.assert_array_equal(got2[:, [0, 1, 5]], got[:, [0, 1, 5]])
+    assert got.min() >= 0.0 and got.max() < 1.0
+
+
+def test_si8_scf_from_device_start_reaches_the_same_state(si8_ref):
+    """The uniform start drawn on the device (random_initial_wf) converges
+    to the oracle's ground state from its N(0,1) start."""
+    from rsdft_develop_amd import scf
+
+    cell, ref = si8_ref
+    print("case: Si8 36^3 SCF from device uniform start", flush=True)
+    res = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=1e-10, max_iter=80, start="uniform")).run()
+    assert res.converged
+    assert abs(res.etot - ref["etot"]) / cell.natom < 1e-7
+    assert_levels_match(res.esp, ref["esp"], shell_tol=3e-7, level_tol=5e-6)
+
