@@ -176,6 +176,48 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     return out
 
 
+def step_leg(nrep: int, steps: int):
+    """Time per SCF step of Si(8 nrep^3) in mixed precision and in FP64 for
+    the same number of steps (IMIX 30, EKBT 0.01 Ha, device start bands), the
+    energy difference between the two after each step, and the FP64 GEMM
+    rates of Gram-Schmidt (S = dV U^T U: 2 MB^2 N FLOP) and subspace_diag
+    (U^T HU and the rotation: 4 MB^2 N FLOP) as lower bounds: whole phase
+    time, including its hpsi, Cholesky and eigensolve."""
+    from rsdft_develop_amd import gpu, scf, systems
+
+    cell = systems.silicon_supercell(nrep, nband=NBANDS[nrep], displacement=0.05, seed=0)
+    N, mb = cell.npoints, cell.nband
+    out = {"workload": f"Si{cell.natom} {cell.ngrid[0]}^3 grid, {mb} bands, {steps} SCF steps"}
+    etots = {}
+    for mixed in (True, False):
+        ctx = gpu.Context(0)
+        ctx.set_grid(cell.ngrid, cell.aa, cell.md)
+        ctx.build_gth_projectors(cell)
+        ctx.autotune_hpsi(64, mb, reps=1)
+        ctx.autotune_hpsi(32, mb, reps=1)
+        tag = f"step Si{cell.natom} {'mixed' if mixed else 'fp64'}"
+        progress(f"{tag}: start")
+        run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=0.0, max_iter=steps, mixed=mixed, imix=30,
+                                          start="uniform"), ctx=ctx)
+        res = run.run(callback=lambda r: progress(f"{tag}: iter {r.iter} prec {r.precision} "
+                                                  + " ".join(f"{k} {v:.2f}" for k, v in r.times.items())))
+        its = res.iterations
+        per = {k: round(sum(r.times.get(k, 0.0) for r in its) / len(its), 3)
+               for k in ("sd", "cg", "gs", "esp", "eto", "pot")}
+        key = "mixed" if mixed else "fp64"
+        out[key] = dict(seconds_per_step=round(sum(sum(r.times.values()) for r in its) / len(its), 3),
+                        setup_s=round(res.setup, 3), per_phase_s=per,
+                        precisions=[r.precision for r in its])
+        etots[key] = [r.etot for r in its]
+        if not mixed:
+            out["fp64"]["gs_gemm_tflops_lower_bound"] = round(2.0 * mb * mb * N / per["gs"] / 1e12, 2)
+            out["fp64"]["sd_gemm_tflops_lower_bound"] = round(4.0 * mb * mb * N / per["sd"] / 1e12, 2)
+        ctx.close()
+    out["mixed_vs_fp64_etot_diff_Ha_per_atom"] = [abs(a - b) / cell.natom for a, b in zip(etots["mixed"], etots["fp64"])]
+    out["speedup_mixed_vs_fp64_per_step"] = round(out["fp64"]["seconds_per_step"] / out["mixed"]["seconds_per_step"], 3)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +225,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--scf", type=int, default=3, help="supercell repeat for the SCF-time leg (3 = Si216); 0 skips")
+    ap.add_argument("--si512", type=int, default=2, help="SCF steps of the Si512 per-step leg (0 skips)")
     ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order")
     ap.add_argument("--shape32", default=None, help="pin the FP32 hpsi launch shape: ry,ty,zlen,chunk,bt,order")
     args = ap.parse_args()
@@ -213,6 +256,7 @@ def main() -> int:
                    "speedup_mixed_vs_fp64": round(plain["seconds"] / mixed["seconds"], 3),
                    "mixed_max_eig_diff_vs_fp64_Ha": de,
                    "mixed_etot_diff_vs_fp64_Ha_per_atom": abs(mixed["etot"] - plain["etot"]) / natom}
+    step_out = step_leg(4, args.si512) if args.si512 > 0 else None
     out = {
         "metric": METRIC,
         "value": round(r64["gbs"], 1),
@@ -241,6 +285,7 @@ def main() -> int:
                 "kernel_traffic": kernel_traffic("f32")},
         "cpu_baseline": cpu,
         "scf": scf_out,
+        "si512_step": step_out,
     }
     print(json.dumps(out))
     return 0

@@ -264,6 +264,144 @@ __global__ void k_pc_dir(const W *__restrict__ r, W *__restrict__ p, const doubl
   p[k] = r[k] + W(rr1[b] / rr0[b]) * p[k];
 }
 
+// ---- hprsdft preconditioner, fused passes ----
+// The same mloop-step CG on the 7-point operator as precond_hprsdft, with
+// the stencil, the vector updates and the per-band dot products of a step
+// in one pass each (partial sums per (band, segment), reduced by
+// k_reduce_partials).  A (r + beta p) is formed as A r + beta (A p) so every
+// pass writes only its own points.
+
+template <typename W>
+__device__ inline W pc_lap(const W *__restrict__ w, int64_t j, int n1, int n2, int n3, int64_t n12, W c0, W c1,
+                           W c2, W c3) {
+  const int i1 = int(j % n1), i2 = int((j / n1) % n2), i3 = int(j / n12);
+  const int xp = i1 + 1 == n1 ? 0 : i1 + 1, xm = i1 == 0 ? n1 - 1 : i1 - 1;
+  const int yp = i2 + 1 == n2 ? 0 : i2 + 1, ym = i2 == 0 ? n2 - 1 : i2 - 1;
+  const int zp = i3 + 1 == n3 ? 0 : i3 + 1, zm = i3 == 0 ? n3 - 1 : i3 - 1;
+  const int64_t row = int64_t(i2) * n1 + i3 * n12;
+  return c0 * w[j] + c1 * (w[xp + row] + w[xm + row]) +
+         c2 * (w[i1 + int64_t(yp) * n1 + i3 * n12] + w[i1 + int64_t(ym) * n1 + i3 * n12]) +
+         c3 * (w[i1 + int64_t(i2) * n1 + zp * n12] + w[i1 + int64_t(i2) * n1 + zm * n12]);
+}
+
+struct PcGeom {
+  int n1, n2, n3;
+  int64_t N, n12, seglen;
+  int nseg, nb;
+};
+
+// Blocks of a launch are dealt to the 8 XCDs in turn; give each XCD a
+// contiguous range of logical blocks (segment fastest, then band) so that
+// neighbouring segments, which share stencil planes, meet in one L2.
+__device__ inline void pc_block(const PcGeom &q, int &s, int &b) {
+  const unsigned nblk = gridDim.x, x = blockIdx.x % 8;
+  const unsigned l = x * (nblk / 8) + min(x, nblk % 8) + blockIdx.x / 8;
+  s = int(l % unsigned(q.nseg));
+  b = int(l / unsigned(q.nseg));
+}
+
+// block sum of v into partial[b * nseg + s]
+__device__ inline void block_partial(double v, double *__restrict__ partial, int b, int s, int nseg) {
+  __shared__ double red[kThreads / 64];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < kThreads / 64; ++k) t += red[k];
+    partial[int64_t(b) * nseg + s] = t;
+  }
+}
+
+// r = g - A g ; p = r ; partial r.r
+template <typename W>
+__global__ __launch_bounds__(kThreads) void k_pcf_first(const W *__restrict__ g, W *__restrict__ r,
+                                                        W *__restrict__ p, PcGeom q, W c0, W c1, W c2, W c3,
+                                                        double *__restrict__ partial) {
+  int s, b;
+  pc_block(q, s, b);
+  const int64_t off = int64_t(b) * q.N, i0 = int64_t(s) * q.seglen, i1 = min(q.N, i0 + q.seglen);
+  const W *gb = g + off;
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const W v = gb[i] - pc_lap(gb, i, q.n1, q.n2, q.n3, q.n12, c0, c1, c2, c3);
+    r[off + i] = v;
+    p[off + i] = v;
+    acc += double(v) * double(v);
+  }
+  block_partial(acc, partial, b, s, q.nseg);
+}
+
+// f = A p ; partial p.f
+template <typename W>
+__global__ __launch_bounds__(kThreads) void k_pcf_mat(const W *__restrict__ p, W *__restrict__ f, PcGeom q, W c0,
+                                                      W c1, W c2, W c3, double *__restrict__ partial) {
+  int s, b;
+  pc_block(q, s, b);
+  const int64_t off = int64_t(b) * q.N, i0 = int64_t(s) * q.seglen, i1 = min(q.N, i0 + q.seglen);
+  const W *pb = p + off;
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const W v = pc_lap(pb, i, q.n1, q.n2, q.n3, q.n12, c0, c1, c2, c3);
+    f[off + i] = v;
+    acc += double(pb[i]) * double(v);
+  }
+  block_partial(acc, partial, b, s, q.nseg);
+}
+
+// beta = rr1/rr0 ; p = r + beta p ; f = A r + beta f ; partial p.f
+template <typename W>
+__global__ __launch_bounds__(kThreads) void k_pcf_dirmat(const W *__restrict__ r, W *__restrict__ p,
+                                                         W *__restrict__ f, const double *__restrict__ rr1,
+                                                         const double *__restrict__ rr0, PcGeom q, W c0, W c1,
+                                                         W c2, W c3, double *__restrict__ partial) {
+  int s, b;
+  pc_block(q, s, b);
+  const int64_t off = int64_t(b) * q.N, i0 = int64_t(s) * q.seglen, i1 = min(q.N, i0 + q.seglen);
+  const W beta = W(rr1[b] / rr0[b]);
+  const W *rb = r + off;
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const W pn = rb[i] + beta * p[off + i];
+    const W fn = pc_lap(rb, i, q.n1, q.n2, q.n3, q.n12, c0, c1, c2, c3) + beta * f[off + i];
+    p[off + i] = pn;
+    f[off + i] = fn;
+    acc += double(pn) * double(fn);
+  }
+  block_partial(acc, partial, b, s, q.nseg);
+}
+
+// x = (FIRST ? g : x) + a p ; then r -= a f with partial r.r, or, on the
+// last step, partial x.g (the <Pg, g> of the CG step)
+template <typename W, bool FIRST, bool LAST>
+__global__ __launch_bounds__(kThreads) void k_pcf_step(W *__restrict__ x, const W *__restrict__ g,
+                                                       const W *__restrict__ p, W *__restrict__ r,
+                                                       const W *__restrict__ f, const double *__restrict__ a,
+                                                       PcGeom q, double *__restrict__ partial) {
+  int s, b;
+  pc_block(q, s, b);
+  const int64_t off = int64_t(b) * q.N, i0 = int64_t(s) * q.seglen, i1 = min(q.N, i0 + q.seglen);
+  const W al = W(a[b]);
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const int64_t k = off + i;
+    const W gv = (FIRST || LAST) ? g[k] : W(0);
+    const W xv = (FIRST ? gv : x[k]) + al * p[k];
+    x[k] = xv;
+    if (LAST) {
+      acc += double(xv) * double(gv);
+    } else {
+      const W rv = r[k] - al * f[k];
+      r[k] = rv;
+      acc += double(rv) * double(rv);
+    }
+  }
+  block_partial(acc, partial, b, s, q.nseg);
+}
+
 // A <- s (A + A^T) in place (both triangles), for an n x n matrix
 __global__ void k_symmetrize(double *__restrict__ A, int n, double s) {
   const int i = blockIdx.x * 16 + threadIdx.x, j = blockIdx.y * 16 + threadIdx.y;
@@ -428,6 +566,77 @@ int precond_hprsdft(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W *r, W 
   return RSG_OK;
 }
 
+
+// precond_hprsdft in fused passes for the CG step (no diagonal shift):
+// pg = P g and rb = dv <pg, g>.  The same iteration as precond_hprsdft.
+template <typename W>
+int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W *r, W *p, W *f, Scratch &s,
+                          double dv, double *rb) {
+  const int64_t N = ctx->N;
+  auto plain = [&]() -> int {  // pg = g, rb = dv <g, g>
+    RSG_HIP(hipMemcpyAsync(pg, g, size_t(N) * nn * sizeof(W), hipMemcpyDeviceToDevice, ctx->stream));
+    const W *xs[1] = {pg}, *ys[1] = {g};
+    return multidot(ctx, 1, xs, ys, nn, dv, rb);
+  };
+  if (mloop <= 0) return plain();
+  const double c0 = ctx->ggg[0] / (ctx->hgrid[0] * ctx->hgrid[0]) +
+                    ctx->ggg[1] / (ctx->hgrid[1] * ctx->hgrid[1]) +
+                    ctx->ggg[2] / (ctx->hgrid[2] * ctx->hgrid[2]);
+  const W C0 = W(c0), C1 = W(-0.5 / (ctx->hgrid[0] * ctx->hgrid[0]) * ctx->ggg[0]);
+  const W C2 = W(-0.5 / (ctx->hgrid[1] * ctx->hgrid[1]) * ctx->ggg[1]);
+  const W C3 = W(-0.5 / (ctx->hgrid[2] * ctx->hgrid[2]) * ctx->ggg[2]);
+  PcGeom q;
+  q.n1 = ctx->n[0];
+  q.n2 = ctx->n[1];
+  q.n3 = ctx->n[2];
+  q.N = N;
+  q.n12 = int64_t(q.n1) * q.n2;
+  constexpr int64_t kSeg = 16384;  // points per block
+  q.nseg = int((N + kSeg - 1) / kSeg);
+  q.seglen = (N + q.nseg - 1) / q.nseg;
+  q.nb = nn;
+  if (int64_t(q.nseg) * nn > 0x7fffffff) return ctx->fail(RSG_ERR_UNSUPPORTED, "preconditioner: too many blocks");
+  if (ctx->dotpart.ensure(size_t(nn) * q.nseg * sizeof(double)) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "preconditioner: scratch allocation failed");
+  double *part = ctx->dotpart.as<double>();
+  const dim3 grid(unsigned(int64_t(q.nseg) * nn)), blk(kThreads), gb((nn + 63) / 64), b64(64);
+  const dim3 gr((nn + 255) / 256);
+  auto reduce = [&](double alpha, double *out) {
+    hipLaunchKernelGGL(k_reduce_partials, gr, dim3(256), 0, ctx->stream, part, q.nseg, nn, alpha, out);
+  };
+  hipLaunchKernelGGL((k_pcf_first<W>), grid, blk, 0, ctx->stream, g, r, p, q, C0, C1, C2, C3, part);
+  reduce(1.0, s.rr1);
+  {
+    std::vector<double> h(static_cast<size_t>(nn));
+    RSG_HIP(hipMemcpyAsync(h.data(), s.rr1, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RSG_HIP(hipStreamSynchronize(ctx->stream));
+    bool small = true;
+    for (double v : h) small = small && (v < 1e-24);
+    if (small) return plain();
+  }
+  for (int it = 1; it <= mloop; ++it) {
+    if (it == 1)
+      hipLaunchKernelGGL((k_pcf_mat<W>), grid, blk, 0, ctx->stream, p, f, q, C0, C1, C2, C3, part);
+    else
+      hipLaunchKernelGGL((k_pcf_dirmat<W>), grid, blk, 0, ctx->stream, r, p, f, s.rr1, s.rr0, q, C0, C1, C2, C3,
+                         part);
+    reduce(1.0, s.pap);
+    hipLaunchKernelGGL(k_pc_alpha, gb, b64, 0, ctx->stream, s.rr1, s.pap, s.rr0, s.a, nn);
+    const bool first = it == 1, last = it == mloop;
+    if (first && last)
+      hipLaunchKernelGGL((k_pcf_step<W, true, true>), grid, blk, 0, ctx->stream, pg, g, p, r, f, s.a, q, part);
+    else if (first)
+      hipLaunchKernelGGL((k_pcf_step<W, true, false>), grid, blk, 0, ctx->stream, pg, g, p, r, f, s.a, q, part);
+    else if (last)
+      hipLaunchKernelGGL((k_pcf_step<W, false, true>), grid, blk, 0, ctx->stream, pg, g, p, r, f, s.a, q, part);
+    else
+      hipLaunchKernelGGL((k_pcf_step<W, false, false>), grid, blk, 0, ctx->stream, pg, g, p, r, f, s.a, q, part);
+    reduce(last ? dv : 1.0, last ? rb : s.rr1);
+  }
+  RSG_HIP(hipGetLastError());
+  return RSG_OK;
+}
+
 }  // namespace
 
 int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
@@ -574,7 +783,6 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
       if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
     }
     for (int icg = 1; icg <= ncg + 1; ++icg) {
-      RSG_HIP(hipMemcpyAsync(pgk, gk, size_t(N) * nn * sizeof(W), hipMemcpyDeviceToDevice, ctx->stream));
       hipLaunchKernelGGL(k_scale_copy, gb, b64, 0, ctx->stream, s.rb, 0.25, s.res, nn);
       RSG_HIP(hipMemcpyAsync(hE.data(), s.E, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
       RSG_HIP(hipMemcpyAsync(hE1.data(), s.E1, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -582,9 +790,15 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
       bool conv = true;
       for (int b = 0; b < nn; ++b) conv = conv && (std::fabs(hE[b] - hE1[b]) < 1e-15);
       if (conv || icg == ncg + 1) break;
-      if (pc_kind == 4 && (rc = precond_hprsdft<W>(ctx, mloop, gk, pgk, nn, pr, pp, pf, s))) return rc;
-      if (pc_kind == 5 && (rc = lpc_apply(ctx, gk, pgk, N, nn, kMixed ? 32 : 64))) return rc;
-      {
+      // pgk = P gk and rb = dV <pgk, gk>
+      if (pc_kind == 4) {
+        if ((rc = precond_hprsdft_fused<W>(ctx, mloop, gk, pgk, nn, pr, pp, pf, s, dv, s.rb))) return rc;
+      } else {
+        if (pc_kind == 5) {
+          if ((rc = lpc_apply(ctx, gk, pgk, N, nn, kMixed ? 32 : 64))) return rc;
+        } else {
+          RSG_HIP(hipMemcpyAsync(pgk, gk, size_t(N) * nn * sizeof(W), hipMemcpyDeviceToDevice, ctx->stream));
+        }
         const W *xs[1] = {pgk}, *ys[1] = {gk};
         if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
       }
@@ -598,10 +812,7 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
         for (double v : rbh) good = good && std::isfinite(v) && v > 0.0;
         if (!good) {
           ++ctx->lpc.fallbacks;
-          RSG_HIP(hipMemcpyAsync(pgk, gk, size_t(N) * nn * sizeof(W), hipMemcpyDeviceToDevice, ctx->stream));
-          if ((rc = precond_hprsdft<W>(ctx, mloop, gk, pgk, nn, pr, pp, pf, s))) return rc;
-          const W *xs[1] = {pgk}, *ys[1] = {gk};
-          if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
+          if ((rc = precond_hprsdft_fused<W>(ctx, mloop, gk, pgk, nn, pr, pp, pf, s, dv, s.rb))) return rc;
         }
       }
       hipLaunchKernelGGL((k_cg_dir<W>), gN, blkd, 0, ctx->stream, pgk, pk, s.rb, s.gkgk, icg == 1 ? 1 : 0, N);
