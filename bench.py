@@ -18,7 +18,9 @@ library's stream; its algorithmic bytes are 2 s N nb + s N.  `traffic` is
 that kernel's HBM-side bytes per launch from the committed rocprofv3
 FETCH_SIZE / WRITE_SIZE summary (scaled by a device-to-device copy of known
 size), or null when the summary is absent.  The SCF leg times whole SCF runs
-of Si216 in FP64 and in mixed precision.  The CPU baseline is the FP64
+of Si216 in FP64 and in mixed precision; the step leg times two SCF steps of
+Si512 (144^3, 1100 bands) in each precision, with lower bounds on the GEMM
+rates of the Gram-Schmidt and subspace phases.  The CPU baseline is the FP64
 oracle hpsi on the host's cores, on a bounded band sample of the same cell.
 """
 from __future__ import annotations
