@@ -220,6 +220,32 @@ def step_leg(nrep: int, steps: int):
     return out
 
 
+def batch_leg(ncells: int = 64, workers: int = 8, seq_cells: int = 16):
+    """Config-2 throughput: ncells uneven Si8 cells (scale U(0.95, 1.10),
+    displacements N(0, 0.1), grids {24, 32, 40, 48}^3, U{20..40} bands), plain
+    FP64 SCF to dE < 1e-6 Ha/atom, `workers` concurrent contexts, against the
+    same solves one at a time (on the first seq_cells cells).  Launch shapes
+    are tuned per grid before either timing."""
+    from rsdft_develop_amd import batch
+
+    cells = batch.config2_cells(ncells, seed=2024)
+    shapes = batch.ShapeCache(reps=2)
+    shapes.warm(cells)
+    progress(f"batch: {ncells} cells, {workers} workers")
+    par = batch.run_batch(cells, workers=workers, shapes=shapes)
+    progress(f"batch: {par.wall:.2f} s; sequential sample of {seq_cells}")
+    seq = batch.run_batch(cells[:seq_cells], workers=1, shapes=shapes)
+    its = [r.iterations for r in par.results]
+    return dict(workload=f"{ncells} Si8 cells, scale U(0.95,1.10), displacement N(0,0.1), grids {{24,32,40,48}}^3, "
+                         f"U{{20..40}} bands, FP64 SCF to dE < 1e-6 Ha/atom",
+                workers=workers, batched_seconds=round(par.wall, 3), batched_cells_per_s=round(par.cells_per_s, 3),
+                sequential_cells=seq_cells, sequential_seconds=round(seq.wall, 3),
+                sequential_cells_per_s=round(seq.cells_per_s, 3),
+                speedup=round(par.cells_per_s / seq.cells_per_s, 3),
+                all_converged=all(r.converged for r in par.results) and all(r.converged for r in seq.results),
+                iterations_min_max=[min(its), max(its)])
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +254,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--scf", type=int, default=3, help="supercell repeat for the SCF-time leg (3 = Si216); 0 skips")
     ap.add_argument("--si512", type=int, default=2, help="SCF steps of the Si512 per-step leg (0 skips)")
+    ap.add_argument("--batch", type=int, default=64, help="cells of the batched-SCF leg (0 skips)")
+    ap.add_argument("--batch-workers", type=int, default=8, help="concurrent contexts of the batched-SCF leg")
     ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order")
     ap.add_argument("--shape32", default=None, help="pin the FP32 hpsi launch shape: ry,ty,zlen,chunk,bt,order")
     args = ap.parse_args()
@@ -259,6 +287,7 @@ def main() -> int:
                    "mixed_max_eig_diff_vs_fp64_Ha": de,
                    "mixed_etot_diff_vs_fp64_Ha_per_atom": abs(mixed["etot"] - plain["etot"]) / natom}
     step_out = step_leg(4, args.si512) if args.si512 > 0 else None
+    batch_out = batch_leg(args.batch, args.batch_workers) if args.batch > 0 else None
     out = {
         "metric": METRIC,
         "value": round(r64["gbs"], 1),
@@ -288,6 +317,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "scf": scf_out,
         "si512_step": step_out,
+        "batch": batch_out,
     }
     print(json.dumps(out))
     return 0

@@ -91,6 +91,8 @@ int rsg_set_grid(rsg_ctx *ctx, int n1, int n2, int n3, const double *aa, int md)
   const double det = aa[0] * (aa[4] * aa[8] - aa[5] * aa[7]) - aa[1] * (aa[3] * aa[8] - aa[5] * aa[6]) +
                      aa[2] * (aa[3] * aa[7] - aa[4] * aa[6]);
   if (!(std::fabs(det) > 0)) return ctx->fail(RSG_ERR_ARG, "set_grid: singular lattice");
+  // the SCF fields and FFT plan survive a new cell on the same grid dimensions
+  const bool same_dims = ctx->have_grid && ctx->n[0] == n1 && ctx->n[1] == n2 && ctx->n[2] == n3;
   ctx->n[0] = n1;
   ctx->n[1] = n2;
   ctx->n[2] = n3;
@@ -116,7 +118,10 @@ int rsg_set_grid(rsg_ctx *ctx, int n1, int n2, int n3, const double *aa, int md)
   RSG_HIP(hipMemsetAsync(ctx->vloc64.p, 0, size_t(N) * 8, ctx->stream));
   RSG_HIP(hipMemsetAsync(ctx->vloc32.p, 0, size_t(N) * 4, ctx->stream));
   RSG_HIP(hipStreamSynchronize(ctx->stream));
-  rsg::scf_release(ctx);  // fields and FFT plan belong to the old grid
+  if (same_dims)
+    rsg::scf_reset(ctx);
+  else
+    rsg::scf_release(ctx);  // fields and FFT plan belong to the old grid
   ctx->have_grid = true;
   ctx->have_vloc = false;
   ctx->have_nl = false;

@@ -926,6 +926,21 @@ int scf_mix_pulay(rsg_ctx *ctx) {
   return RSG_OK;
 }
 
+// Forget the cell (local potential, constants, mixing history) but keep the
+// fields and the FFT plan, which depend on the grid dimensions only.
+void scf_reset(rsg_ctx *ctx) {
+  ScfState *st = ctx->scf;
+  if (!st) return;
+  st->have_local = false;
+  st->eewald = st->nelectron = 0.0;
+  st->mmix = st->count = 0;
+  st->zps.clear();
+  st->rloc.clear();
+  st->cloc.clear();
+  st->frac.clear();
+  st->kind.clear();
+}
+
 void scf_release(rsg_ctx *ctx) {
   ScfState *st = ctx->scf;
   if (!st) return;

@@ -639,6 +639,51 @@ int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W 
 
 }  // namespace
 
+namespace {
+// out[t] = alpha * sum_c part[c * count + t]
+__global__ void k_sum_batches(const double *__restrict__ part, int nbat, int count, double alpha,
+                              double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  double v = 0.0;
+  for (int c = 0; c < nbat; ++c) v += part[int64_t(c) * count + t];
+  out[t] = alpha * v;
+}
+
+// C (nb x nb) = alpha A^T B with A, B column-major N x nb (leading dim N).
+// rocBLAS runs this K = N >> nb shape on a handful of output tiles (one
+// workgroup walks all of K when nb is small), so for nb <= 192 K is split
+// into chunks: one strided-batched GEMM into per-chunk partial matrices,
+// then a deterministic sum over the chunks.
+int gemm_tn_gram(rsg_ctx *ctx, int nb, int64_t N, double alpha, const double *A, const double *B, double *C) {
+  const double zero = 0.0, one = 1.0;
+  if (nb > 192 || N < 4096) {
+    RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, int(N), &alpha, A,
+                           int(N), B, int(N), &zero, C, nb));
+    return RSG_OK;
+  }
+  const int64_t kc = std::max<int64_t>(1024, (N + 255) / 256);  // at most 256 chunks of >= 1024 points
+  const int nfull = int(N / kc);
+  const int64_t krem = N - int64_t(nfull) * kc;
+  const int nbat = nfull + (krem > 0 ? 1 : 0);
+  const size_t mat = size_t(nb) * nb;
+  if (ctx->grampart.ensure(mat * nbat * sizeof(double)) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "gram product: scratch allocation failed");
+  double *part = ctx->grampart.as<double>();
+  RSG_BLAS(rocblas_dgemm_strided_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb,
+                                         int(kc), &one, A, int(N), kc, B, int(N), kc, &zero, part, nb,
+                                         rocblas_stride(mat), nfull));
+  if (krem > 0)
+    RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, int(krem), &one,
+                           A + int64_t(nfull) * kc, int(N), B + int64_t(nfull) * kc, int(N), &zero,
+                           part + mat * nfull, nb));
+  const int count = int(mat);
+  hipLaunchKernelGGL(k_sum_batches, dim3((count + 255) / 256), dim3(256), 0, ctx->stream, part, nbat, count, alpha, C);
+  RSG_HIP(hipGetLastError());
+  return RSG_OK;
+}
+}  // namespace
+
 int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
   int rc = check_store(ctx, "gram_schmidt");
   if (rc) return rc;
@@ -652,8 +697,8 @@ int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
   const double dv = ctx->dv, zero = 0.0, one = 1.0;
   // S = dV U^T U, gram_schmidt_lusl_module.f90:268 (calc_dsyrk3), as a full
   // GEMM: rocBLAS's SYRK runs this K = N shape far below the MFMA GEMM
-  RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, int(N), &dv, U,
-                         int(N), U, int(N), &zero, S, nb));
+  rc = gemm_tn_gram(ctx, nb, N, dv, U, U, S);
+  if (rc) return rc;
   // S = R^T R (:286, PDPOTRF)
   RSG_BLAS(rocsolver_dpotrf(ctx->blas, rocblas_fill_upper, nb, S, nb, ctx->info.as<rocblas_int>()));
   int info = 0;
@@ -686,8 +731,8 @@ int subspace_diag(rsg_ctx *ctx, double *esp) {
   // one GEMM, then Hsub = dV (A + A^T) / 2 (the SYR2K path of rocBLAS runs
   // this K = N shape far below the MFMA GEMM)
   const double zero = 0.0, one = 1.0;
-  RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, mb, mb, int(N), &one, U,
-                         int(N), HU, int(N), &zero, Hs, mb));
+  rc = gemm_tn_gram(ctx, mb, N, 1.0, U, HU, Hs);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_symmetrize, dim3((mb + 15) / 16, (mb + 15) / 16), dim3(16, 16), 0, ctx->stream, Hs, mb,
                      0.5 * ctx->dv);
   // eigen-decomposition, ascending (:155, DSYEV 'V','U')
@@ -937,6 +982,9 @@ int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double
   return RSG_OK;
 }
 
-void solver_release(rsg_ctx *ctx) { ctx->dotpart.release(); }
+void solver_release(rsg_ctx *ctx) {
+  ctx->dotpart.release();
+  ctx->grampart.release();
+}
 
 }  // namespace rsg

@@ -34,4 +34,5 @@ int scf_density_integrals(rsg_ctx *ctx, int field, double *eloc, double *eion);
 int scf_mix_init(rsg_ctx *ctx, int mmix, double beta, int imix);
 int scf_mix_pulay(rsg_ctx *ctx);
 void scf_release(rsg_ctx *ctx);
+void scf_reset(rsg_ctx *ctx);
 }  // namespace rsg

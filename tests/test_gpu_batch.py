@@ -1,0 +1,59 @@
+"""Batched SCF: concurrent cells give the same results as one at a time,
+and a batch cell follows the oracle's SCF of that cell step for step."""
+import pytest
+
+from oracle import scf_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+
+def test_concurrent_cells_equal_sequential_cells():
+    from rsdft_develop_amd import batch, scf
+
+    cells = batch.config2_cells(4, seed=11)
+    print("case: 4 config-2 cells, 3 workers vs 1", flush=True)
+    params = scf.ScfParams(ekbt=0.01, etol_per_atom=1e-6, max_iter=40, imix=30, start="uniform")
+    shapes = batch.ShapeCache(reps=1)
+    par = batch.run_batch(cells, workers=3, params=params, shapes=shapes)
+    seq = batch.run_batch(cells, workers=1, params=params, shapes=shapes)
+    for a, b, c in zip(par.results, seq.results, cells):
+        assert (a.converged, a.iterations) == (b.converged, b.iterations)
+        assert max(abs(x - y) for x, y in zip(a.history, b.history)) / c.cell.natom < 1e-10
+
+
+def test_batch_cell_follows_oracle():
+    from rsdft_develop_amd import batch, scf
+
+    bc = min(batch.config2_cells(8, seed=5), key=lambda c: c.cell.npoints * c.cell.nband)
+    print(f"case: config-2 cell {bc.cell.ngrid} {bc.cell.nband} bands vs oracle, 8 iterations", flush=True)
+    ref = so.run_scf(bc.cell, etol_per_atom=0.0, ekbt=0.01, max_iter=8, seed=bc.seed, start="uniform", imix=30)
+    params = scf.ScfParams(ekbt=0.01, etol_per_atom=0.0, max_iter=8, imix=30, start="uniform")
+    got = batch.run_batch([bc], workers=1, params=params).results[0]
+    assert len(got.history) == len(ref["history"]) == 8
+    for it, (a, b) in enumerate(zip(got.history, ref["history"])):
+        assert abs(a - b["etot"]) / bc.cell.natom < 1e-8, it
+
+
+def test_reused_context_forgets_the_previous_cell():
+    """A context reused for a new cell on the same grid dimensions (fields
+    and FFT plan kept) gives the same SCF as a fresh context."""
+    from rsdft_develop_amd import gpu, scf, systems
+
+    a = systems.silicon_supercell(1, nband=22, displacement=0.1, seed=1, scale=1.05, ngrid=24)
+    b = systems.silicon_supercell(1, nband=22, displacement=0.1, seed=2, scale=0.97, ngrid=24)
+    p = scf.ScfParams(ekbt=0.01, etol_per_atom=0.0, max_iter=6, imix=30, start="uniform", seed=3)
+    print("case: Si8 24^3 cell b after cell a on one context vs fresh", flush=True)
+
+    def run(cell, ctx):
+        ctx.set_grid(cell.ngrid, cell.aa, cell.md)
+        ctx.build_gth_projectors(cell)
+        return [r.etot for r in scf.SCF(cell, p, ctx=ctx).run().iterations]
+
+    ctx = gpu.Context()
+    run(a, ctx)
+    reused = run(b, ctx)
+    ctx.close()
+    fresh_ctx = gpu.Context()
+    fresh = run(b, fresh_ctx)
+    fresh_ctx.close()
+    assert reused == fresh
