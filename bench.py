@@ -20,7 +20,8 @@ FETCH_SIZE / WRITE_SIZE summary (scaled by a device-to-device copy of known
 size), or null when the summary is absent.  The SCF leg times whole SCF runs
 of Si216 in FP64 and in mixed precision; the step leg times two SCF steps of
 Si512 (144^3, 1100 bands) in each precision, with lower bounds on the GEMM
-rates of the Gram-Schmidt and subspace phases.  The CPU baseline is the FP64
+rates of the Gram-Schmidt and subspace phases; the batch leg runs 64 uneven
+Si8 cells as concurrent SCFs and one at a time.  The CPU baseline is the FP64
 oracle hpsi on the host's cores, on a bounded band sample of the same cell.
 """
 from __future__ import annotations
@@ -220,15 +221,18 @@ def step_leg(nrep: int, steps: int):
     return out
 
 
-def batch_leg(ncells: int = 64, workers: int = 8, seq_cells: int = 16):
+def batch_leg(ncells: int = 64, workers: int = 8, seq_cells: int | None = None):
     """Config-2 throughput: ncells uneven Si8 cells (scale U(0.95, 1.10),
     displacements N(0, 0.1), grids {24, 32, 40, 48}^3, U{20..40} bands), plain
     FP64 SCF to dE < 1e-6 Ha/atom, `workers` concurrent contexts, against the
-    same solves one at a time (on the first seq_cells cells).  Launch shapes
-    are tuned per grid before either timing."""
+    same solves one at a time (all of them unless seq_cells is given).  Launch
+    shapes are tuned per grid before either timing.  The process runs with
+    16 hardware queues (GPU_MAX_HW_QUEUES, set before HIP starts) so that
+    the workers' streams do not share 4."""
     from rsdft_develop_amd import batch
 
     cells = batch.config2_cells(ncells, seed=2024)
+    seq_cells = ncells if seq_cells is None else seq_cells
     shapes = batch.ShapeCache(reps=2)
     shapes.warm(cells)
     progress(f"batch: {ncells} cells, {workers} workers")
@@ -259,6 +263,9 @@ def main() -> int:
     ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order")
     ap.add_argument("--shape32", default=None, help="pin the FP32 hpsi launch shape: ry,ty,zlen,chunk,bt,order")
     args = ap.parse_args()
+    # more hardware queues for the batched leg's concurrent streams; read by
+    # HIP when it starts, which has not happened yet in this process
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path is not sharded in this build); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -272,6 +279,7 @@ def main() -> int:
     r64 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float64, pin(args.shape64))
     progress("hpsi FP32")
     r32 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float32, pin(args.shape32))
+    batch_out = batch_leg(args.batch, args.batch_workers) if args.batch > 0 else None
     progress("CPU baseline")
     cpu = None if args.no_cpu else cpu_leg(cell, vloc)
     scf_out = None
@@ -287,7 +295,6 @@ def main() -> int:
                    "mixed_max_eig_diff_vs_fp64_Ha": de,
                    "mixed_etot_diff_vs_fp64_Ha_per_atom": abs(mixed["etot"] - plain["etot"]) / natom}
     step_out = step_leg(4, args.si512) if args.si512 > 0 else None
-    batch_out = batch_leg(args.batch, args.batch_workers) if args.batch > 0 else None
     out = {
         "metric": METRIC,
         "value": round(r64["gbs"], 1),
