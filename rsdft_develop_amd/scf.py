@@ -108,7 +108,7 @@ class SCF:
     """Plain FP64 SCF of one cell on one GPU."""
 
     def __init__(self, cell: systems.Cell, params: ScfParams | None = None, ctx: gpu.Context | None = None,
-                 bands: np.ndarray | None = None):
+                 bands: np.ndarray | None = None, rho: np.ndarray | None = None):
         t_setup = time.perf_counter()
         self.cell = cell
         self.p = params or ScfParams()
@@ -119,7 +119,10 @@ class SCF:
         c.build_gth_projectors(cell)
         c.set_local_gth(cell)
         self.eewald, self.nelectron = c.scf_constants()
-        c.init_density(self.p.init_width)
+        if rho is None:
+            c.init_density(self.p.init_width)
+        else:  # restart density (IC with vrho.dat1)
+            c.set_field(gpu.FIELD_RHO, rho)
         self.eh_in, self.exc_in = c.hartree_xc(gpu.FIELD_RHO)
         c.update_vloc()
         if self.p.pc_kind == 5:
@@ -153,7 +156,7 @@ class SCF:
         esp = c.esp_calc()
         t["esp"] = time.perf_counter() - t0
         t0 = time.perf_counter()
-        occ, _ = fermi_dirac(esp, self.nelectron, p.ekbt)
+        occ, self.efermi = fermi_dirac(esp, self.nelectron, p.ekbt)
         eeig = float(np.sum(occ * esp))
         eloc_in, eion_in = c.density_integrals(gpu.FIELD_RHO)
         eharris = eeig - eloc_in + self.eh_in + self.exc_in + eion_in + self.eewald
@@ -167,6 +170,44 @@ class SCF:
         fene = etot - entropy_fd(occ, p.ekbt)
         self.occ = occ
         return ScfIteration(it, etot, eharris, fene, de, dh, esp, t, precision)
+
+    @classmethod
+    def from_restart(cls, cell: systems.Cell, params: ScfParams | None = None, ctx: gpu.Context | None = None,
+                     wf: str | None = None, vrho: str | None = None) -> "SCF":
+        """Start from the reference's restart files: bands from wf.dat1 and/or
+        the density from vrho.dat1 (read_data, io_module.f90:267-491)."""
+        from . import restart
+
+        bands = rho = None
+        if wf is not None:
+            w = restart.read_wf(wf)
+            if w.ngrid != tuple(cell.ngrid) or w.band_range != (1, cell.nband) or w.mb != cell.nband:
+                raise ValueError(f"{wf}: grid {w.ngrid} / bands {w.band_range} do not match the cell")
+            bands = w.bands
+        if vrho is not None:
+            v = restart.read_vrho(vrho)
+            if v.ngrid != tuple(cell.ngrid):
+                raise ValueError(f"{vrho}: grid {v.ngrid} does not match the cell")
+            rho = v.rho
+        return cls(cell, params, ctx=ctx, bands=bands, rho=rho)
+
+    def save(self, wf: str | None = None, vrho: str | None = None, oc: int = 3):
+        """Write the current bands (wf.dat1) and the density of those bands
+        with its Hartree, XC and local potentials (vrho.dat1), in the
+        reference's formats (write_data, io_module.f90:59-263)."""
+        from . import restart
+
+        c, cell = self.ctx, self.cell
+        if wf is not None:
+            occ = getattr(self, "occ", np.zeros(cell.nband))
+            restart.write_wf(wf, c.download_bands(), occ, cell.aa, cell.ngrid, oc=oc)
+        if vrho is not None:
+            c.hartree_xc(gpu.FIELD_RHO_OUT)  # VH, VXC of the bands' density
+            vion, vh, vxc = (c.get_field(f) for f in (gpu.FIELD_VION, gpu.FIELD_VH, gpu.FIELD_VXC))
+            mu = getattr(self, "efermi", 0.0)
+            zn = np.array([restart.atomic_number(sp.name) for sp in cell.species])
+            restart.write_vrho(vrho, restart.VrhoData(tuple(cell.ngrid), cell.aa, cell.frac, cell.kind, zn, mu,
+                                                      c.get_field(gpu.FIELD_RHO_OUT), vion + vh + vxc, vh, vxc))
 
     def update_potential(self) -> float:
         t0 = time.perf_counter()

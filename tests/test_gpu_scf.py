@@ -213,3 +213,22 @@ def test_si8_scf_from_device_start_reaches_the_same_state(si8_ref):
     assert abs(res.etot - ref["etot"]) / cell.natom < 1e-7
     assert_levels_match(res.esp, ref["esp"], shell_tol=3e-7, level_tol=5e-6)
 
+
+
+def test_scf_restarts_from_its_own_files(tmp_path):
+    """wf.dat1 + vrho.dat1 written after convergence restart the SCF at the
+    converged state (IC: read_data, io_module.f90:267-491)."""
+    from rsdft_develop_amd import scf
+
+    cell = systems.silicon_supercell(1, ngrid_per_cell=24, nband=20, displacement=0.05, seed=3)
+    p = scf.ScfParams(ekbt=0.01, etol_per_atom=1e-9, max_iter=80, imix=30)
+    print("case: Si8 24^3 SCF, save, restart", flush=True)
+    run = scf.SCF(cell, p)
+    res = run.run()
+    assert res.converged
+    wf, vr = str(tmp_path / "wf.dat1"), str(tmp_path / "vrho.dat1")
+    run.save(wf=wf, vrho=vr)
+    res2 = scf.SCF.from_restart(cell, p, wf=wf, vrho=vr).run()
+    assert res2.converged and res2.niter <= 4, res2.niter
+    assert abs(res2.iterations[0].etot - res.etot) / cell.natom < 1e-6
+    assert abs(res2.etot - res.etot) / cell.natom < 1e-8
