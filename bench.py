@@ -21,7 +21,9 @@ size), or null when the summary is absent.  The SCF leg times whole SCF runs
 of Si216 in FP64 and in mixed precision; the step leg times two SCF steps of
 Si512 (144^3, 1100 bands) in each precision, with lower bounds on the GEMM
 rates of the Gram-Schmidt and subspace phases; the batch leg runs 64 uneven
-Si8 cells as concurrent SCFs and one at a time.  The CPU baseline is the FP64
+Si8 cells as concurrent SCFs and one at a time; the learned-preconditioner
+leg counts CG rounds to hprsdft's residual tolerance with the committed
+weights.  The CPU baseline is the FP64
 oracle hpsi on the host's cores, on a bounded band sample of the same cell.
 """
 from __future__ import annotations
@@ -250,6 +252,26 @@ def batch_leg(ncells: int = 64, workers: int = 8, seq_cells: int | None = None):
                 iterations_min_max=[min(its), max(its)])
 
 
+LPC_WEIGHTS = os.path.join(ROOT, "rsdft_develop_amd", "data", "lpc_si64.npz")
+
+
+def lpc_leg():
+    """The learned CG preconditioner (weights trained by train_lpc on 20480
+    residuals of 32 Si64 cells) judged by CG rounds to the residual
+    tolerance the plain hprsdft method reaches in 8 rounds, on a held-out
+    Si64 cell, against hprsdft and no preconditioner."""
+    if not os.path.exists(LPC_WEIGHTS):
+        return None
+    from rsdft_develop_amd import lpc, systems, train_lpc
+
+    params = lpc.load(LPC_WEIGHTS)
+    cell = systems.silicon_supercell(2, displacement=0.05, seed=999)
+    ev = train_lpc.evaluate(params, cell)
+    keep = ("tolerance", "rounds_hprsdft", "rounds_learned", "rounds_none", "reached_learned", "fallbacks",
+            "seconds_hprsdft", "seconds_learned", "seconds_none")
+    return {"workload": "held-out Si64 cell, FP64 CG(4) rounds at a fixed potential", **{k: ev[k] for k in keep}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,6 +302,8 @@ def main() -> int:
     progress("hpsi FP32")
     r32 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float32, pin(args.shape32))
     batch_out = batch_leg(args.batch, args.batch_workers) if args.batch > 0 else None
+    progress("learned preconditioner")
+    lpc_out = lpc_leg()
     progress("CPU baseline")
     cpu = None if args.no_cpu else cpu_leg(cell, vloc)
     scf_out = None
@@ -325,6 +349,7 @@ def main() -> int:
         "scf": scf_out,
         "si512_step": step_out,
         "batch": batch_out,
+        "learned_preconditioner": lpc_out,
     }
     print(json.dumps(out))
     return 0
