@@ -52,10 +52,9 @@ def gpu_leg(cell, vloc, steps: int, warmup: int, dtype, pin=None):
 
     ctx = gpu.Context(0).setup(cell, vloc)
     prec = 64 if dtype == np.float64 else 32
-    if pin:  # launch shape given (ry, ty, zlen, chunk, bt, order): a counter run repeats a timed run's shape
-        ry, ty, zlen, chunk, bt, order = pin
-        ctx.set_hpsi_shape(prec, ry=ry, ty=ty, zlen=zlen, chunk=chunk, bt=bt)
-        ctx.set_hpsi_order(prec, order)
+    if pin:  # launch shape given (ry, ty, zlen, chunk, bt, order[, rx]): a counter run repeats a timed run's shape
+        keys = ("ry", "ty", "zlen", "chunk", "bt", "order", "rx")
+        ctx.apply_hpsi_shape(prec, dict(zip(keys, pin)))
     else:
         ctx.autotune_hpsi(prec, cell.nband, reps=3)  # launch shape from the library's autotuner
     shape = ctx.hpsi_shape(prec, cell.nband)
@@ -286,8 +285,8 @@ def main() -> int:
     ap.add_argument("--si512", type=int, default=2, help="SCF steps of the Si512 per-step leg (0 skips)")
     ap.add_argument("--batch", type=int, default=64, help="cells of the batched-SCF leg (0 skips)")
     ap.add_argument("--batch-workers", type=int, default=8, help="concurrent contexts of the batched-SCF leg")
-    ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order")
-    ap.add_argument("--shape32", default=None, help="pin the FP32 hpsi launch shape: ry,ty,zlen,chunk,bt,order")
+    ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx]")
+    ap.add_argument("--shape32", default=None, help="pin the FP32 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx]")
     args = ap.parse_args()
     # more hardware queues for the batched leg's concurrent streams; read by
     # HIP when it starts, which has not happened yet in this process

@@ -128,6 +128,12 @@ int rsg_set_hpsi_shape(rsg_ctx *ctx, int precision, int ry, int ty, int zlen,
  * writes and the nonlocal kernel adds.  No reference counterpart. */
 int rsg_set_hpsi_order(rsg_ctx *ctx, int precision, int order);
 int rsg_get_hpsi_order(rsg_ctx *ctx, int precision, int *order);
+/* Grid points along x per stencil thread: 1, or 2 for the pair kernel
+ * (FP32 with md = 6, n1 even, tile rows dividing n2; elsewhere the call
+ * falls back to 1).  The getter reports what an nb-band call would use.
+ * No reference counterpart. */
+int rsg_set_hpsi_xpoints(rsg_ctx *ctx, int precision, int rx);
+int rsg_get_hpsi_xpoints(rsg_ctx *ctx, int precision, int nb, int *rx);
 /* The shape an nb-band call would use. */
 int rsg_get_hpsi_shape(rsg_ctx *ctx, int precision, int nb, int *ry, int *ty,
                        int *zlen, int *chunk, int *bt);

@@ -99,8 +99,7 @@ class ShapeCache:
                 sh = ctx.hpsi_shape(precision, cell.nband)
                 self.shapes[key] = sh
                 return
-        ctx.set_hpsi_shape(precision, ry=sh["ry"], ty=sh["ty"], zlen=sh["zlen"], chunk=0, bt=sh["bt"])
-        ctx.set_hpsi_order(precision, sh["order"])
+        ctx.apply_hpsi_shape(precision, sh, chunk=0)
 
 
 def run_batch(cells: list, workers: int = 8, params: scf.ScfParams | None = None, shapes: ShapeCache | None = None,

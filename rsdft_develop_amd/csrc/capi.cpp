@@ -308,6 +308,23 @@ int rsg_get_hpsi_order(rsg_ctx *ctx, int precision, int *order) {
   return RSG_OK;
 }
 
+int rsg_set_hpsi_xpoints(rsg_ctx *ctx, int precision, int rx) {
+  if (!ctx) return RSG_ERR_ARG;
+  if ((precision != 64 && precision != 32) || (rx != 1 && rx != 2))
+    return ctx->fail(RSG_ERR_ARG, "set_hpsi_xpoints: precision 64/32 and rx 1/2");
+  (precision == 64 ? ctx->shape64 : ctx->shape32).rx = rx;
+  return RSG_OK;
+}
+
+int rsg_get_hpsi_xpoints(rsg_ctx *ctx, int precision, int nb, int *rx) {
+  if (!ctx || !rx) return RSG_ERR_ARG;
+  rsg::HpsiShape sh;
+  int rc = rsg::resolved_shape(ctx, precision, nb, &sh);
+  if (rc) return rc;
+  *rx = sh.rx;
+  return RSG_OK;
+}
+
 int rsg_get_hpsi_shape(rsg_ctx *ctx, int precision, int nb, int *ry, int *ty, int *zlen, int *chunk, int *bt) {
   if (!ctx) return RSG_ERR_ARG;
   rsg::HpsiShape sh;

@@ -108,6 +108,7 @@ struct HpsiShape {
   int chunk = 0;  // bands per chunk (gather -> scatter -> stencil)
   int bt = 8;     // bands per scatter thread
   int order = 0;  // analytic projectors: 0 = nonlocal writes, stencil adds; 1 = stencil writes, nonlocal adds
+  int rx = 1;     // x-points per stencil thread: 1, or 2 (pair kernel: FP32, Md 6, even n1, ty | n2)
 };
 
 // Learned preconditioner state: bf16 weight matrices for the kernels, the

@@ -26,6 +26,8 @@
 //     in a per-thread register queue, and each thread computes ry rows so
 //     neighbouring rows share their y-window reads.
 #include <algorithm>
+#include <type_traits>
+#include <utility>
 
 #include "context.hpp"
 
@@ -201,6 +203,173 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
       }
     }
   }
+}
+
+// ---- pair variant: two adjacent x-points per thread ----
+// Same operator and streaming as k_stencil.  Each LDS row carries MD wrapped
+// copies of the row's ends on both sides (row length n1 + 2 MD), so the
+// x-window of a pair is MD + 1 aligned pair reads and every y-column read
+// serves both points: about 40% fewer LDS bytes per point, at twice the
+// z-ring registers.  Needs n1 even and full y-tiles (ty | n2); the step loop
+// is unrolled by an index-sequence fold so every ring slot is a compile-time
+// register.
+template <typename T> struct PairOf;
+template <> struct PairOf<float> { typedef float2 V; };
+template <> struct PairOf<double> { typedef double2 V; };
+
+template <class F, int... U> __device__ inline void run_steps(F &f, int zb, std::integer_sequence<int, U...>) {
+  (void)(f(zb, std::integral_constant<int, U>{}) && ...);
+}
+
+template <typename T, int MD, int RY, int HM, bool ACC>
+__global__ __launch_bounds__(512) void k_stencil2(const T *psi, T *hpsi, int64_t ld, const T *vloc,
+                                                  StencilCoef<T> cf, int n1, int n2, int n3, int ty, int zlen,
+                                                  T *dead) {
+  typedef typename PairOf<T>::V V;
+  constexpr int PF = ring_pf(MD);
+  constexpr int R = 2 * MD + 1 + PF;
+  static_assert(R % kHS == 0 && R % 2 == 0, "ring length must be a multiple of the rotations");
+  extern __shared__ unsigned char smem_raw[];
+  T *lds = reinterpret_cast<T *>(smem_raw);
+  const int LS = n1 + 2 * MD;
+  const int plane = (ty + 2 * MD) * LS;
+  const int deadslot = 2 * plane;  // one pair past both plane buffers
+  const int hp = n1 / 2;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const bool active = tid < hp * (ty / RY);
+  const int px = active ? tid % hp : 0, g = active ? tid / hp : 0;
+  const int x = 2 * px;
+  const int ntile_y = n2 / ty, nz = (n3 + zlen - 1) / zlen;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int by = int(lb % unsigned(ntile_y)), bz = int((lb / unsigned(ntile_y)) % unsigned(nz));
+  const int band = int(lb / unsigned(ntile_y * nz));
+  const int y0 = by * ty;
+  const int z0 = bz * zlen, z1 = min(z0 + zlen, n3);
+  const size_t n12 = size_t(n1) * n2;
+  const T *w = psi + int64_t(band) * ld;
+  T *o = hpsi + int64_t(band) * ld;
+  unsigned colofs[RY];
+  int lown[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    colofs[r] = unsigned(x + (y0 + g * RY + r) * n1);
+    lown[r] = active ? (MD + g * RY + r) * LS + MD + x : deadslot;
+  }
+  // the pair's wrapped copy: x < MD -> right ghost, x >= n1 - MD -> left ghost
+  const int gofs = !active ? 0 : (x < MD ? n1 : (x >= n1 - MD ? -n1 : 0));
+  const int nhalo = 2 * MD * hp;
+  int hl[HM], hgh[HM];
+  unsigned hg[HM];
+#pragma unroll
+  for (int k = 0; k < HM; ++k) {
+    const int idx = tid + k * nthr;
+    const int i = idx < nhalo ? idx : 0;
+    const int hr = i / hp, hx = 2 * (i - hr * hp);
+    const int lrow = hr < MD ? hr : hr + ty;
+    hl[k] = idx < nhalo ? lrow * LS + MD + hx : deadslot;
+    hgh[k] = (idx < nhalo && (hx < MD || hx >= n1 - MD)) ? hl[k] + (hx < MD ? n1 : -n1) : deadslot;
+    hg[k] = unsigned(hx + wrapi(y0 - MD + lrow, n2) * n1);
+  }
+  V q[RY][R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const T *wp = w + size_t(wrapi(z0 - MD + k, n3)) * n12;
+#pragma unroll
+    for (int r = 0; r < RY; ++r) q[r][k] = *reinterpret_cast<const V *>(wp + colofs[r]);
+  }
+  V h[kHS][HM], v[kHS][RY], a[kHS][RY];
+#pragma unroll
+  for (int sidx = 0; sidx < kHS - 1; ++sidx) {
+    const size_t zo = size_t(wrapi(z0 + sidx, n3)) * n12;
+#pragma unroll
+    for (int k = 0; k < HM; ++k) h[sidx][k] = *reinterpret_cast<const V *>(w + zo + hg[k]);
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      v[sidx][r] = *reinterpret_cast<const V *>(vloc + zo + colofs[r]);
+      if (ACC) a[sidx][r] = *reinterpret_cast<const V *>(o + zo + colofs[r]);
+    }
+  }
+  V *odead = reinterpret_cast<V *>(dead) + (tid & 63);
+  int zh = wrapi(z0 + kHS - 1, n3);  // plane of the halo / Vloc / H psi prefetch
+  int zq = wrapi(z0 + MD + PF, n3);  // plane of the own-column prefetch
+  auto step = [&](int zb, auto uc) -> bool {
+    constexpr int u = decltype(uc)::value;
+    const int z = zb + u;
+    if (z >= z1) return false;
+    T *buf = lds + (u & 1) * plane;
+    constexpr int sc = u % kHS, sn = (u + kHS - 1) % kHS;
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const V val = q[r][(u + MD) % R];
+      *reinterpret_cast<V *>((lown[r] == deadslot ? lds : buf) + lown[r]) = val;
+      *reinterpret_cast<V *>(gofs ? buf + lown[r] + gofs : lds + deadslot) = val;
+    }
+#pragma unroll
+    for (int k = 0; k < HM; ++k) {
+      *reinterpret_cast<V *>((hl[k] == deadslot ? lds : buf) + hl[k]) = h[sc][k];
+      *reinterpret_cast<V *>((hgh[k] == deadslot ? lds : buf) + hgh[k]) = h[sc][k];
+    }
+    {
+      const size_t zo = size_t(zh) * n12;
+#pragma unroll
+      for (int k = 0; k < HM; ++k) h[sn][k] = *reinterpret_cast<const V *>(w + zo + hg[k]);
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        v[sn][r] = *reinterpret_cast<const V *>(vloc + zo + colofs[r]);
+        if (ACC) a[sn][r] = *reinterpret_cast<const V *>(o + zo + colofs[r]);
+      }
+      const T *wq = w + size_t(zq) * n12;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) q[r][(u + R - 1) % R] = *reinterpret_cast<const V *>(wq + colofs[r]);
+      zh = (zh + 1 == n3) ? 0 : zh + 1;
+      zq = (zq + 1 == n3) ? 0 : zq + 1;
+    }
+    lds_barrier();
+    T *oz = o + size_t(z) * n12;
+    V col[RY + 2 * MD];
+#pragma unroll
+    for (int k = 0; k < RY + 2 * MD; ++k) col[k] = *reinterpret_cast<const V *>(buf + (g * RY + k) * LS + MD + x);
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const T *row = buf + (MD + g * RY + r) * LS + x;  // ghost-padded x - MD .. x + MD + 1
+      T xw[2 * MD + 2];
+#pragma unroll
+      for (int k = 0; k < MD + 1; ++k) {
+        const V p2 = *reinterpret_cast<const V *>(row + 2 * k);
+        xw[2 * k] = p2.x;
+        xw[2 * k + 1] = p2.y;
+      }
+      const V c = q[r][(u + MD) % R];
+      T a0 = (cf.c0 + v[sc][r].x) * c.x;
+      T a1 = (cf.c0 + v[sc][r].y) * c.y;
+#pragma unroll
+      for (int m = 0; m < MD; ++m) {
+        a0 += cf.c[0][m] * (xw[MD - 1 - m] + xw[MD + 1 + m]);
+        a1 += cf.c[0][m] * (xw[MD - m] + xw[MD + 2 + m]);
+      }
+#pragma unroll
+      for (int m = 0; m < MD; ++m) {
+        a0 += cf.c[1][m] * (col[r + MD - 1 - m].x + col[r + MD + 1 + m].x);
+        a1 += cf.c[1][m] * (col[r + MD - 1 - m].y + col[r + MD + 1 + m].y);
+      }
+#pragma unroll
+      for (int m = 0; m < MD; ++m) {
+        const V lo = q[r][(u + MD - 1 - m) % R], hi = q[r][(u + MD + 1 + m) % R];
+        a0 += cf.c[2][m] * (lo.x + hi.x);
+        a1 += cf.c[2][m] * (lo.y + hi.y);
+      }
+      if (ACC) {
+        a0 += a[sc][r].x;
+        a1 += a[sc][r].y;
+      }
+      V res;
+      res.x = a0;
+      res.y = a1;
+      *(active ? reinterpret_cast<V *>(oz + colofs[r]) : odead) = res;
+    }
+    return true;
+  };
+  for (int zb = z0; zb < z1; zb += R) run_steps(step, zb, std::make_integer_sequence<int, R>{});
 }
 
 template <typename T> __device__ inline T wave_sum(T v) {
@@ -544,6 +713,67 @@ int pick_ty(int n1, int n2, int md, int ry, size_t s) {
   return best_ty;
 }
 
+// Pair kernel (two x-points per thread): tile rows for ry rows per
+// thread among the divisors of n2, same cost model as pick_ty.
+constexpr int kPairMd = 6;
+
+inline bool pair_fits(int n1, int n2, int md, int ry, int ty, size_t s) {
+  if (md != kPairMd || n1 % 2 || n1 < 2 * md || ty <= 0 || ty % ry || n2 % ty) return false;
+  const int th = round64((n1 / 2) * (ty / ry));
+  const size_t lds = (2 * size_t(ty + 2 * md) * (n1 + 2 * md) + 2) * s;
+  return th <= 512 && lds <= kMaxDynLds && 2 * md * (n1 / 2) <= 4 * th;
+}
+
+int pick_ty_pair(int n1, int n2, int md, int ry, size_t s) {
+  int best_ty = 0;
+  double best = 1e300;
+  for (int ty = ry; ty <= n2; ty += ry) {
+    if (!pair_fits(n1, n2, md, ry, ty, s)) continue;
+    const int th = (n1 / 2) * (ty / ry);
+    const double cost = double(round64(th)) / th * (1.0 + 0.5 * 2.0 * md / ty) * (th < 128 ? 2.0 : 1.0);
+    if (cost < best) {
+      best = cost;
+      best_ty = ty;
+    }
+  }
+  return best_ty;
+}
+
+template <int RY, int HM>
+int launch_stencil2_t(rsg_ctx *ctx, const float *psi, float *hpsi, int64_t ld, int nb, const HpsiShape &sh,
+                      int accumulate) {
+  constexpr int MD = kPairMd;
+  const int n1 = ctx->n[0], n2 = ctx->n[1], n3 = ctx->n[2];
+  if (!pair_fits(n1, n2, ctx->md, RY, sh.ty, sizeof(float)))
+    return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: pair stencil tile does not fit");
+  const int threads = round64((n1 / 2) * (sh.ty / RY));
+  if (2 * MD * (n1 / 2) > HM * threads) return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: pair stencil halo");
+  const size_t lds = (2 * size_t(sh.ty + 2 * MD) * (n1 + 2 * MD) + 2) * sizeof(float);
+  const int64_t nblk = int64_t(n2 / sh.ty) * ((n3 + sh.zlen - 1) / sh.zlen) * nb;
+  if (nblk > 0x7fffffff) return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: too many stencil workgroups");
+  const dim3 grid{unsigned(nblk), 1u, 1u};
+  if (accumulate)
+    hipLaunchKernelGGL((k_stencil2<float, MD, RY, HM, true>), grid, dim3(threads), lds, ctx->stream, psi, hpsi, ld,
+                       vloc_of<float>(ctx), make_coef<float>(ctx), n1, n2, n3, sh.ty, sh.zlen,
+                       ctx->dead.template as<float>());
+  else
+    hipLaunchKernelGGL((k_stencil2<float, MD, RY, HM, false>), grid, dim3(threads), lds, ctx->stream, psi, hpsi,
+                       ld, vloc_of<float>(ctx), make_coef<float>(ctx), n1, n2, n3, sh.ty, sh.zlen,
+                       ctx->dead.template as<float>());
+  return RSG_OK;
+}
+
+template <int RY>
+int launch_stencil2_hm(rsg_ctx *ctx, const float *psi, float *hpsi, int64_t ld, int nb, const HpsiShape &sh,
+                       int accumulate) {
+  const int hp = ctx->n[0] / 2;
+  const int threads = round64(hp * (sh.ty / RY));
+  const int need = (2 * kPairMd * hp + threads - 1) / threads;
+  if (need <= 1) return launch_stencil2_t<RY, 1>(ctx, psi, hpsi, ld, nb, sh, accumulate);
+  if (need <= 2) return launch_stencil2_t<RY, 2>(ctx, psi, hpsi, ld, nb, sh, accumulate);
+  return launch_stencil2_t<RY, 4>(ctx, psi, hpsi, ld, nb, sh, accumulate);
+}
+
 template <typename T, int MD, int RY, int HM>
 int launch_stencil_t(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const HpsiShape &sh,
                      int accumulate) {
@@ -588,6 +818,27 @@ int launch_stencil_md(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, c
 template <typename T>
 int launch_stencil(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const HpsiShape &sh,
                    int accumulate) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (sh.rx == 2) {
+      // pair loads and stores are 8-byte vectors: every band column must
+      // start 8-byte aligned, else the one-point kernel takes the call
+      const bool aligned = ld % 2 == 0 && reinterpret_cast<uintptr_t>(psi) % 8 == 0 &&
+                           reinterpret_cast<uintptr_t>(hpsi) % 8 == 0;
+      if (aligned) {
+        if (sh.ry == 1) return launch_stencil2_hm<1>(ctx, psi, hpsi, ld, nb, sh, accumulate);
+        if (sh.ry == 2) return launch_stencil2_hm<2>(ctx, psi, hpsi, ld, nb, sh, accumulate);
+        return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: ry must be 1 or 2");
+      }
+      HpsiShape one = sh;
+      one.rx = 1;
+      one.ty = pick_ty(ctx->n[0], ctx->n[1], ctx->md, one.ry, sizeof(T));
+      if (one.ty == 0) {
+        one.ry = 1;
+        one.ty = pick_ty(ctx->n[0], ctx->n[1], ctx->md, 1, sizeof(T));
+      }
+      return launch_stencil<T>(ctx, psi, hpsi, ld, nb, one, accumulate);
+    }
+  }
   switch (ctx->md) {
 #define RSG_MD_CASE(M) \
   case M: return launch_stencil_md<T, M>(ctx, psi, hpsi, ld, nb, sh, accumulate);
@@ -610,9 +861,18 @@ HpsiShape resolve_shape(const rsg_ctx *ctx, const HpsiShape &in, size_t s, int n
   HpsiShape sh = in;
   const int n1 = ctx->n[0], n2 = ctx->n[1], n3 = ctx->n[2];
   if (sh.ry < 1 || sh.ry > 2) sh.ry = 2;
-  if (sh.ty <= 0 || sh.ty % sh.ry || round64(n1 * (sh.ty / sh.ry)) > 512 ||
+  if (sh.rx == 2) {  // pair kernel: FP32 only; else the one-point kernel
+    if (s != sizeof(float)) sh.rx = 1;
+    else if (!pair_fits(n1, n2, ctx->md, sh.ry, sh.ty, s)) {
+      sh.ty = pick_ty_pair(n1, n2, ctx->md, sh.ry, s);
+      if (sh.ty == 0) sh.rx = 1;
+    }
+  } else {
+    sh.rx = 1;
+  }
+  if (sh.rx == 1 && (sh.ty <= 0 || sh.ty % sh.ry || round64(n1 * (sh.ty / sh.ry)) > 512 ||
       2 * ctx->md * n1 > 4 * round64(n1 * (sh.ty / sh.ry)) ||
-      (2 * size_t(sh.ty + 2 * ctx->md) * n1 + 1) * s > kMaxDynLds) {
+      (2 * size_t(sh.ty + 2 * ctx->md) * n1 + 1) * s > kMaxDynLds)) {
     sh.ty = pick_ty(n1, n2, ctx->md, sh.ry, s);
     if (sh.ty == 0) {
       sh.ry = 1;
@@ -829,11 +1089,36 @@ template <typename T> int autotune_t(rsg_ctx *ctx, int nb, int reps, float *best
     c.zlen = 0;
     consider(c);
   }
-  for (int f : {-2, -1, 1, 2}) {
-    HpsiShape c = best;
-    c.ty = best.ty + f * best.ry;
-    c.zlen = 0;
-    if (c.ty > 0) consider(c);
+  if (std::is_same<T, float>::value) {
+    for (int ry = 1; ry <= 2; ++ry) {
+      HpsiShape c = best;
+      c.rx = 2;
+      c.ry = ry;
+      c.ty = 0;
+      c.zlen = 0;
+      consider(c);
+    }
+  }
+  if (best.rx == 2) {  // neighbouring divisors of n2
+    const int n2 = ctx->n[1];
+    int lo = 0, hi = 0;
+    for (int t = best.ty - best.ry; t > 0 && !lo; t -= best.ry)
+      if (pair_fits(ctx->n[0], n2, ctx->md, best.ry, t, s)) lo = t;
+    for (int t = best.ty + best.ry; t <= n2 && !hi; t += best.ry)
+      if (pair_fits(ctx->n[0], n2, ctx->md, best.ry, t, s)) hi = t;
+    for (int t : {lo, hi}) {
+      HpsiShape c = best;
+      c.ty = t;
+      c.zlen = 0;
+      if (t > 0) consider(c);
+    }
+  } else {
+    for (int f : {-2, -1, 1, 2}) {
+      HpsiShape c = best;
+      c.ty = best.ty + f * best.ry;
+      c.zlen = 0;
+      if (c.ty > 0) consider(c);
+    }
   }
   if (ctx->have_nl) {
     for (int f : {2, 4}) {

@@ -80,6 +80,8 @@ def load_library():
         "rsg_autotune_hpsi": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(ctypes.c_float)]),
         "rsg_set_hpsi_order": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_get_hpsi_order": (_c_int, [_vp, _c_int, _P(_c_int)]),
+        "rsg_set_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int]),
+        "rsg_get_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int, _P(_c_int)]),
         "rsg_alloc_bands": (_c_int, [_vp, _c_int]),
         "rsg_upload_bands": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int]),
         "rsg_download_bands": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int]),
@@ -275,12 +277,26 @@ class Context:
         self._check(self.L.rsg_get_hpsi_order(self.h, int(precision), ctypes.byref(o)))
         return o.value
 
+    def set_hpsi_xpoints(self, precision=64, rx=1):
+        """x-points per stencil thread: 2 selects the FP32 pair kernel where it fits."""
+        self._check(self.L.rsg_set_hpsi_xpoints(self.h, int(precision), int(rx)))
+
     def hpsi_shape(self, precision=64, nb=1) -> dict:
         v = [_c_int() for _ in range(5)]
         self._check(self.L.rsg_get_hpsi_shape(self.h, int(precision), int(nb), *[ctypes.byref(x) for x in v]))
         d = dict(zip(("ry", "ty", "zlen", "chunk", "bt"), (x.value for x in v)))
         d["order"] = self.hpsi_order(precision)
+        rx = _c_int()
+        self._check(self.L.rsg_get_hpsi_xpoints(self.h, int(precision), int(nb), ctypes.byref(rx)))
+        d["rx"] = rx.value
         return d
+
+    def apply_hpsi_shape(self, precision, sh: dict, chunk=None):
+        """Set a shape dict as returned by hpsi_shape (chunk None: keep sh's)."""
+        self.set_hpsi_shape(precision, ry=sh["ry"], ty=sh["ty"], zlen=sh["zlen"],
+                            chunk=sh["chunk"] if chunk is None else chunk, bt=sh["bt"])
+        self.set_hpsi_order(precision, sh.get("order", 0))
+        self.set_hpsi_xpoints(precision, sh.get("rx", 1))
 
     def autotune_hpsi(self, precision=64, nb=1, reps=5) -> float:
         ms = ctypes.c_float()

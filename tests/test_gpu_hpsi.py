@@ -139,6 +139,71 @@ def test_op_kinetic_is_the_local_part(orc, gpu_mod):
             a.free()
 
 
+def _f32_local_and_full(ham, ctx, psi, ld, rel=2e-5):
+    """op_kinetic (write and accumulate) and hpsi in FP32 against the oracle."""
+    from rsdft_develop_amd import hip
+
+    N, nb = psi.shape
+    loc, full = ham.kinetic_local(psi), ham.hpsi(psi)
+    host = np.zeros((ld, nb), order="F")
+    host[:N] = psi
+    d_in = hip.DeviceArray.from_numpy(host, dtype=np.float32)
+    d_out = hip.DeviceArray((ld, nb), dtype=np.float32)
+    d_out.fill_bytes(0xFF)
+    ctx.op_kinetic_device(d_in.ptr, d_out.ptr, ld, nb, dtype=np.float32)
+    ctx.synchronize()
+    got = d_out.to_numpy().astype(np.float64)
+    _close(got[:N], loc, rel)
+    assert np.all(np.isnan(got[N:]))
+    ctx.op_kinetic_device(d_in.ptr, d_out.ptr, ld, nb, dtype=np.float32, accumulate=True)
+    ctx.synchronize()
+    _close(d_out.to_numpy().astype(np.float64)[:N], 2 * loc, rel)
+    ctx.hpsi_device(d_in.ptr, d_out.ptr, ld, nb, dtype=np.float32)
+    ctx.synchronize()
+    _close(d_out.to_numpy().astype(np.float64)[:N], full, rel)
+    d_in.free()
+    d_out.free()
+
+
+@pytest.mark.parametrize("ng,ry,ty,zlen", [(36, 2, 0, 0), (36, 1, 0, 0), (36, 2, 12, 7), (30, 2, 0, 0),
+                                           (24, 1, 8, 5), (24, 2, 24, 24)])
+def test_pair_stencil_f32(orc, gpu_mod, ng, ry, ty, zlen):
+    """The FP32 kernel with two x-points per thread, over tile shapes,
+    z-runs that do not divide n3, and both passes of hpsi."""
+    cell = systems.silicon_supercell(1, ngrid_per_cell=ng, displacement=0.05, seed=ng)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    ctx.set_hpsi_shape(32, ry=ry, ty=ty, zlen=zlen)
+    ctx.set_hpsi_xpoints(32, 2)
+    nb = 5
+    sh = ctx.hpsi_shape(32, nb)
+    print(f"case: pair stencil {ng}^3, shape {sh}", flush=True)
+    assert sh["rx"] == 2 and sh["ry"] == ry and ng % sh["ty"] == 0
+    if ty:
+        assert sh["ty"] == ty
+    _f32_local_and_full(ham, ctx, systems.random_bands(cell.npoints, nb, seed=ng + ry), cell.npoints)
+
+
+def test_pair_stencil_falls_back_where_it_does_not_apply(orc, gpu_mod):
+    """Odd n1, md other than 6, FP64, and an odd leading dimension all run
+    the one-point kernel and stay exact."""
+    base = systems.silicon_supercell(1, displacement=0.1, seed=3)
+    for ngrid, md in (((21, 24, 22), 6), ((24, 24, 20), 4)):
+        cell = systems.Cell(aa=np.diag([9.7, 11.3, 10.1]), frac=base.frac, kind=base.kind, species=base.species,
+                            ngrid=ngrid, nband=4, md=md)
+        ham, ctx = _pair(orc, gpu_mod, cell)
+        ctx.set_hpsi_xpoints(32, 2)
+        ctx.set_hpsi_xpoints(64, 2)
+        assert ctx.hpsi_shape(32, 4)["rx"] == 1 and ctx.hpsi_shape(64, 4)["rx"] == 1
+        psi = systems.random_bands(cell.npoints, 4, seed=md)
+        _f32_local_and_full(ham, ctx, psi, cell.npoints)
+        _close(ctx.hamiltonian(psi), ham.hpsi(psi), 1e-12)
+    cell = systems.silicon_supercell(1, ngrid_per_cell=24, displacement=0.05, seed=8)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    ctx.set_hpsi_xpoints(32, 2)
+    assert ctx.hpsi_shape(32, 3)["rx"] == 2
+    _f32_local_and_full(ham, ctx, systems.random_bands(cell.npoints, 3, seed=4), cell.npoints + 37)
+
+
 def test_hpsi_errors(gpu_mod):
     ctx = gpu_mod.Context()
     a = np.zeros((8, 1), order="F")
