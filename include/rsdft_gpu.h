@@ -72,7 +72,10 @@ int rsg_set_projectors(rsg_ctx *ctx, int nzlma, int maxmjj, const int *mjj,
  * ippform 4 (ps_nloc2_module.f90:42-1167 with prep_ps_nloc_gth,
  * ps_nloc_gth_module.f90:19-180, and Ylm, ylm_module.f90:11-73).
  * frac[3*a+i] fractional positions, kind[a] species; per-species arrays are
- * [kind*4 + iorb] (norb[kind] <= 4), h < 0 flips iuV. */
+ * [kind*4 + iorb] (norb[kind] <= 4), h < 0 flips iuV.  Where the analytic
+ * form (below) applies, the reference-layout tables are built on first
+ * use: by rsg_get_projector_sizes / rsg_get_projectors (host tables) or
+ * rsg_set_nonlocal_mode(ctx, 1) (tables and their device layouts). */
 int rsg_build_gth_projectors(rsg_ctx *ctx, int natom, const double *frac,
                              const int *kind, int nkind, const int *norb,
                              const int *lo, const int *no, const double *rps,

@@ -127,6 +127,7 @@ int rsg_set_grid(rsg_ctx *ctx, int n1, int n2, int n3, const double *aa, int md)
   ctx->have_nl = false;
   ctx->have_ana = false;
   ctx->tables = rsg::HostTables();
+  ctx->tables_state = 0;
   ctx->mb = 0;
   return RSG_OK;
 }
@@ -179,10 +180,38 @@ int rsg_set_projectors(rsg_ctx *ctx, int nzlma, int maxmjj, const int *mjj, cons
   }
   RSG_HIP(hipSetDevice(ctx->device));
   ctx->tables = std::move(t);
+  ctx->tables_state = 0;
   ctx->have_nl = false;
   ctx->have_ana = false;  // caller tables carry no analytic form
   return rsg::upload_nonlocal(ctx);
 }
+
+namespace {
+
+// Reference-layout tables of the deferred GTH input on the host.
+int ensure_host_tables(rsg_ctx *ctx) {
+  if (ctx->tables_state != 1) return RSG_OK;
+  const rsg_ctx::GthInput &g = ctx->gth;
+  rsg::HostTables t;
+  int rc = rsg::build_gth_tables(ctx, g.natom, g.frac.data(), g.kind.data(), g.nkind, g.norb.data(), g.lo.data(),
+                                 g.no.data(), g.rps.data(), g.hnl.data(), t);
+  if (rc) return ctx->fail(rc, "build_gth: invalid species parameters");
+  ctx->tables = std::move(t);
+  ctx->tables_state = 2;
+  return RSG_OK;
+}
+
+// ... and their device layouts (the table form of the nonlocal operator).
+int ensure_device_tables(rsg_ctx *ctx) {
+  if (ctx->tables_state == 0) return RSG_OK;
+  int rc = ensure_host_tables(ctx);
+  if (rc) return rc;
+  if ((rc = rsg::upload_nonlocal(ctx))) return rc;
+  ctx->tables_state = 0;
+  return RSG_OK;
+}
+
+}  // namespace
 
 int rsg_build_gth_projectors(rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind,
                              const int *norb, const int *lo, const int *no, const double *rps,
@@ -191,24 +220,51 @@ int rsg_build_gth_projectors(rsg_ctx *ctx, int natom, const double *frac, const 
   if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "build_gth: grid not set");
   if (natom < 0 || nkind < 1 || (natom > 0 && (!frac || !kind)) || !norb || !lo || !no || !rps || !hnl)
     return ctx->fail(RSG_ERR_ARG, "build_gth: bad arguments");
+  // the checks of build_gth_tables, made before either form is built
+  for (int ik = 0; ik < nkind; ++ik) {
+    if (norb[ik] < 0 || norb[ik] > 4) return ctx->fail(RSG_ERR_ARG, "build_gth: invalid species parameters");
+    for (int io = 0; io < norb[ik]; ++io) {
+      const int q = ik * 4 + io;
+      if (lo[q] < 0 || lo[q] > 3 || no[q] < 1 || !(rps[q] > 0))
+        return ctx->fail(RSG_ERR_ARG, "build_gth: invalid species parameters");
+    }
+  }
+  for (int a = 0; a < natom; ++a)
+    if (kind[a] < 0 || kind[a] >= nkind) return ctx->fail(RSG_ERR_ARG, "build_gth: invalid species parameters");
   RSG_HIP(hipSetDevice(ctx->device));
-  rsg::HostTables t;
-  int rc = rsg::build_gth_tables(ctx, natom, frac, kind, nkind, norb, lo, no, rps, hnl, t);
-  if (rc) return ctx->fail(rc, "build_gth: invalid species parameters");
-  ctx->tables = std::move(t);
+  rsg_ctx::GthInput &g = ctx->gth;
+  g.natom = natom;
+  g.nkind = nkind;
+  g.frac.assign(frac, frac + 3 * size_t(natom));
+  g.kind.assign(kind, kind + natom);
+  g.norb.assign(norb, norb + nkind);
+  g.lo.assign(lo, lo + 4 * size_t(nkind));
+  g.no.assign(no, no + 4 * size_t(nkind));
+  g.rps.assign(rps, rps + 4 * size_t(nkind));
+  g.hnl.assign(hnl, hnl + 4 * size_t(nkind));
+  ctx->tables = rsg::HostTables();
   ctx->have_nl = false;
   ctx->have_ana = false;
-  if ((rc = rsg::upload_nonlocal(ctx))) return rc;
+  ctx->tables_state = 1;
   rsg::HostAnalytic h;
-  rc = rsg::build_gth_analytic(ctx, natom, frac, kind, nkind, norb, lo, no, rps, hnl, h);
-  if (rc == RSG_ERR_UNSUPPORTED) return RSG_OK;  // the table path covers this set
+  int rc = rsg::build_gth_analytic(ctx, natom, frac, kind, nkind, norb, lo, no, rps, hnl, h);
+  if (rc == RSG_ERR_UNSUPPORTED) return ensure_device_tables(ctx);  // the table form covers this set
   if (rc) return ctx->fail(rc, "build_gth: invalid atom kinds");
-  return rsg::upload_analytic(ctx, h);
+  if ((rc = rsg::upload_analytic(ctx, h))) return rc;
+  ctx->have_nl = h.ngroup > 0;
+  // the table form is built when a caller asks for it (tables, mode 1)
+  if (ctx->nl_mode == 1) return ensure_device_tables(ctx);
+  return RSG_OK;
 }
 
 int rsg_set_nonlocal_mode(rsg_ctx *ctx, int mode) {
   if (!ctx) return RSG_ERR_ARG;
   if (mode < 0 || mode > 2) return ctx->fail(RSG_ERR_ARG, "set_nonlocal_mode: mode must be 0, 1 or 2");
+  if (mode == 1) {
+    RSG_HIP(hipSetDevice(ctx->device));
+    int rc = ensure_device_tables(ctx);
+    if (rc) return rc;
+  }
   ctx->nl_mode = mode;
   return RSG_OK;
 }
@@ -222,6 +278,8 @@ int rsg_get_nonlocal_mode(rsg_ctx *ctx, int *mode, int *analytic_available) {
 
 int rsg_get_projector_sizes(rsg_ctx *ctx, int *nzlma, int *maxmjj, int64_t *nent) {
   if (!ctx) return RSG_ERR_ARG;
+  int rc = ensure_host_tables(ctx);
+  if (rc) return rc;
   if (nzlma) *nzlma = ctx->tables.nzlma;
   if (maxmjj) *maxmjj = ctx->tables.maxmjj;
   if (nent) *nent = ctx->tables.nentries();
@@ -230,6 +288,8 @@ int rsg_get_projector_sizes(rsg_ctx *ctx, int *nzlma, int *maxmjj, int64_t *nent
 
 int rsg_get_projectors(rsg_ctx *ctx, int *mjj, int *jjp, double *uvk, int *iuv) {
   if (!ctx) return RSG_ERR_ARG;
+  int rc = ensure_host_tables(ctx);
+  if (rc) return rc;
   const rsg::HostTables &t = ctx->tables;
   if (mjj) std::memcpy(mjj, t.mjj.data(), t.mjj.size() * sizeof(int));
   if (jjp) std::memcpy(jjp, t.jjp.data(), t.jjp.size() * sizeof(int));

@@ -161,6 +161,16 @@ struct rsg_ctx {
   bool have_vloc = false;
 
   rsg::HostTables tables;
+  // GTH input of the last build_gth_projectors call.  With the analytic
+  // form in use the reference-layout tables and their device layouts are
+  // built on first use only: tables_state 1 = neither built, 2 = host
+  // tables built, 0 = nothing deferred.
+  struct GthInput {
+    int natom = 0, nkind = 0;
+    std::vector<double> frac, rps, hnl;
+    std::vector<int> kind, norb, lo, no;
+  } gth;
+  int tables_state = 0;
   rsg::NonlocalDev<double> nl64;
   rsg::NonlocalDev<float> nl32;
   bool have_nl = false;

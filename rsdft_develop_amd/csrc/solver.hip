@@ -264,41 +264,28 @@ __global__ void k_pc_dir(const W *__restrict__ r, W *__restrict__ p, const doubl
   p[k] = r[k] + W(rr1[b] / rr0[b]) * p[k];
 }
 
-// ---- hprsdft preconditioner, fused passes ----
-// The same mloop-step CG on the 7-point operator as precond_hprsdft, with
-// the stencil, the vector updates and the per-band dot products of a step
-// in one pass each (partial sums per (band, segment), reduced by
-// k_reduce_partials).  A (r + beta p) is formed as A r + beta (A p) so every
-// pass writes only its own points.
+// ---- hprsdft preconditioner, z-streaming passes ----
+// The same mloop-step CG on the 7-point operator A = -1/2 lap_7 as
+// precond_hprsdft, one pass per step.  A block owns kThreads consecutive
+// points of an xy-plane and a run of z-planes of one band; each thread
+// walks its z-column holding the column's values at z-1, z and z+1 in
+// registers, so every stencil input is read from HBM once, and takes the
+// four in-plane neighbours from its neighbours' loads of the same plane
+// (L1/L2).  A p is never stored: the step pass recomputes it from p, and
+// the direction pass evaluates A p' for p' = r + beta p from r and p
+// (cgpc_hprsdft_module.f90:180-186 applies precond_cg_mat to the new
+// direction the same way).  The new direction is written to the second
+// buffer, since other blocks still read the old one as neighbours; the
+// host swaps the two.  Per-band dot products: one partial per block,
+// summed by k_reduce_partials.
 
-template <typename W>
-__device__ inline W pc_lap(const W *__restrict__ w, int64_t j, int n1, int n2, int n3, int64_t n12, W c0, W c1,
-                           W c2, W c3) {
-  const int i1 = int(j % n1), i2 = int((j / n1) % n2), i3 = int(j / n12);
-  const int xp = i1 + 1 == n1 ? 0 : i1 + 1, xm = i1 == 0 ? n1 - 1 : i1 - 1;
-  const int yp = i2 + 1 == n2 ? 0 : i2 + 1, ym = i2 == 0 ? n2 - 1 : i2 - 1;
-  const int zp = i3 + 1 == n3 ? 0 : i3 + 1, zm = i3 == 0 ? n3 - 1 : i3 - 1;
-  const int64_t row = int64_t(i2) * n1 + i3 * n12;
-  return c0 * w[j] + c1 * (w[xp + row] + w[xm + row]) +
-         c2 * (w[i1 + int64_t(yp) * n1 + i3 * n12] + w[i1 + int64_t(ym) * n1 + i3 * n12]) +
-         c3 * (w[i1 + int64_t(i2) * n1 + zp * n12] + w[i1 + int64_t(i2) * n1 + zm * n12]);
-}
+enum PzOp { kPzFirst, kPzPap, kPzDir, kPzStepF, kPzStepM, kPzStepL, kPzStepFL };
 
-struct PcGeom {
+struct PzGeom {
   int n1, n2, n3;
-  int64_t N, n12, seglen;
-  int nseg, nb;
+  int64_t N, n12;
+  int nq, nzr, zlen, nseg;
 };
-
-// Blocks of a launch are dealt to the 8 XCDs in turn; give each XCD a
-// contiguous range of logical blocks (segment fastest, then band) so that
-// neighbouring segments, which share stencil planes, meet in one L2.
-__device__ inline void pc_block(const PcGeom &q, int &s, int &b) {
-  const unsigned nblk = gridDim.x, x = blockIdx.x % 8;
-  const unsigned l = x * (nblk / 8) + min(x, nblk % 8) + blockIdx.x / 8;
-  s = int(l % unsigned(q.nseg));
-  b = int(l / unsigned(q.nseg));
-}
 
 // block sum of v into partial[b * nseg + s]
 __device__ inline void block_partial(double v, double *__restrict__ partial, int b, int s, int nseg) {
@@ -316,90 +303,133 @@ __device__ inline void block_partial(double v, double *__restrict__ partial, int
   }
 }
 
-// r = g - A g ; p = r ; partial r.r
-template <typename W>
-__global__ __launch_bounds__(kThreads) void k_pcf_first(const W *__restrict__ g, W *__restrict__ r,
-                                                        W *__restrict__ p, PcGeom q, W c0, W c1, W c2, W c3,
-                                                        double *__restrict__ partial) {
-  int s, b;
-  pc_block(q, s, b);
-  const int64_t off = int64_t(b) * q.N, i0 = int64_t(s) * q.seglen, i1 = min(q.N, i0 + q.seglen);
-  const W *gb = g + off;
-  double acc = 0.0;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
-    const W v = gb[i] - pc_lap(gb, i, q.n1, q.n2, q.n3, q.n12, c0, c1, c2, c3);
-    r[off + i] = v;
-    p[off + i] = v;
-    acc += double(v) * double(v);
-  }
-  block_partial(acc, partial, b, s, q.nseg);
-}
+// Pointers of one pass: the stencil input is g (first), p (pap, steps) or
+// r + beta p (dir); x accumulates P g (it is g before the first step).
+template <typename W> struct PzArgs {
+  const W *g;
+  W *x, *r;
+  const W *p;
+  W *pw;  // first: p = r; dir: the new direction
+  const double *a, *rr1, *rr0;
+  double *partial;
+};
 
-// f = A p ; partial p.f
-template <typename W>
-__global__ __launch_bounds__(kThreads) void k_pcf_mat(const W *__restrict__ p, W *__restrict__ f, PcGeom q, W c0,
-                                                      W c1, W c2, W c3, double *__restrict__ partial) {
-  int s, b;
-  pc_block(q, s, b);
-  const int64_t off = int64_t(b) * q.N, i0 = int64_t(s) * q.seglen, i1 = min(q.N, i0 + q.seglen);
-  const W *pb = p + off;
-  double acc = 0.0;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
-    const W v = pc_lap(pb, i, q.n1, q.n2, q.n3, q.n12, c0, c1, c2, c3);
-    f[off + i] = v;
-    acc += double(pb[i]) * double(v);
-  }
-  block_partial(acc, partial, b, s, q.nseg);
-}
+// Raw values of one plane the thread needs: the four in-plane neighbours
+// of the stencil input (r and p separately for the direction pass, which
+// combines them where they are used) and the pointwise streams (g or x,
+// and r).
+template <typename W> struct PzPlane {
+  W n[4], m[4];  // neighbours x+, x-, y+, y- of the input (m: p for the direction pass)
+  W u, v;        // pointwise: first stream (g or x), r
+};
 
-// beta = rr1/rr0 ; p = r + beta p ; f = A r + beta f ; partial p.f
-template <typename W>
-__global__ __launch_bounds__(kThreads) void k_pcf_dirmat(const W *__restrict__ r, W *__restrict__ p,
-                                                         W *__restrict__ f, const double *__restrict__ rr1,
-                                                         const double *__restrict__ rr0, PcGeom q, W c0, W c1,
-                                                         W c2, W c3, double *__restrict__ partial) {
-  int s, b;
-  pc_block(q, s, b);
-  const int64_t off = int64_t(b) * q.N, i0 = int64_t(s) * q.seglen, i1 = min(q.N, i0 + q.seglen);
-  const W beta = W(rr1[b] / rr0[b]);
-  const W *rb = r + off;
+template <typename W, int OP>
+__global__ __launch_bounds__(kThreads) void k_pcz(PzArgs<W> A, PzGeom q, W c0, W c1, W c2, W c3) {
+  constexpr bool kStencil = OP != kPzStepL && OP != kPzStepFL;
+  constexpr bool kStep = OP == kPzStepF || OP == kPzStepM;
+  // logical block: plane chunk fastest, then z-run, then band; the 8 XCDs
+  // each take one contiguous range, so neighbouring chunks share an L2
+  const unsigned nblk = gridDim.x, xc = blockIdx.x % 8;
+  const unsigned lb = xc * (nblk / 8) + min(xc, nblk % 8) + blockIdx.x / 8;
+  const int qb = int(lb % unsigned(q.nq));
+  const int zr = int((lb / unsigned(q.nq)) % unsigned(q.nzr));
+  const int b = int(lb / unsigned(q.nq * q.nzr));
+  const int64_t qi = int64_t(qb) * kThreads + threadIdx.x;
+  const bool act = qi < q.n12;
+  const int64_t qc = act ? qi : 0;
+  const int i1 = int(qc % q.n1), i2 = int(qc / q.n1);
+  const int64_t nb4[4] = {i1 == q.n1 - 1 ? qc - (q.n1 - 1) : qc + 1, i1 == 0 ? qc + q.n1 - 1 : qc - 1,
+                          i2 == q.n2 - 1 ? qc - int64_t(q.n2 - 1) * q.n1 : qc + q.n1,
+                          i2 == 0 ? qc + int64_t(q.n2 - 1) * q.n1 : qc - q.n1};
+  const int z0 = zr * q.zlen, z1 = min(z0 + q.zlen, q.n3);
+  const int64_t off = int64_t(b) * q.N;
+  const W *gb = A.g + off, *pb = A.p + off;
+  W *xb = A.x + off, *rb = A.r + off;
+  const W al = (kStep || OP == kPzStepL || OP == kPzStepFL) ? W(A.a[b]) : W(0);
+  const W beta = OP == kPzDir ? W(A.rr1[b] / A.rr0[b]) : W(0);
+  const W *src = OP == kPzFirst ? gb : (OP == kPzDir ? rb : pb);  // stencil input (r for the direction pass)
+  auto wrapz = [&](int z) { return ((z % q.n3) + q.n3) % q.n3; };  // any n3 >= 1
   double acc = 0.0;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
-    const W pn = rb[i] + beta * p[off + i];
-    const W fn = pc_lap(rb, i, q.n1, q.n2, q.n3, q.n12, c0, c1, c2, c3) + beta * f[off + i];
-    p[off + i] = pn;
-    f[off + i] = fn;
-    acc += double(pn) * double(fn);
-  }
-  block_partial(acc, partial, b, s, q.nseg);
-}
-
-// x = (FIRST ? g : x) + a p ; then r -= a f with partial r.r, or, on the
-// last step, partial x.g (the <Pg, g> of the CG step)
-template <typename W, bool FIRST, bool LAST>
-__global__ __launch_bounds__(kThreads) void k_pcf_step(W *__restrict__ x, const W *__restrict__ g,
-                                                       const W *__restrict__ p, W *__restrict__ r,
-                                                       const W *__restrict__ f, const double *__restrict__ a,
-                                                       PcGeom q, double *__restrict__ partial) {
-  int s, b;
-  pc_block(q, s, b);
-  const int64_t off = int64_t(b) * q.N, i0 = int64_t(s) * q.seglen, i1 = min(q.N, i0 + q.seglen);
-  const W al = W(a[b]);
-  double acc = 0.0;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
-    const int64_t k = off + i;
-    const W gv = (FIRST || LAST) ? g[k] : W(0);
-    const W xv = (FIRST ? gv : x[k]) + al * p[k];
-    x[k] = xv;
-    if (LAST) {
-      acc += double(xv) * double(gv);
-    } else {
-      const W rv = r[k] - al * f[k];
-      r[k] = rv;
-      acc += double(rv) * double(rv);
+  if constexpr (kStencil) {
+    // loads of a plane are issued one iteration before its use (the centre
+    // column two), ahead of the stores of the current plane
+    auto load = [&](int z, PzPlane<W> &P) {
+      const int64_t o = int64_t(z) * q.n12;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        P.n[t] = src[o + nb4[t]];
+        if (OP == kPzDir) P.m[t] = pb[o + nb4[t]];
+      }
+      if (kStep) {
+        P.u = OP == kPzStepF ? gb[o + qc] : xb[o + qc];
+        P.v = rb[o + qc];
+      }
+    };
+    auto val = [&](W a, W m) -> W { return OP == kPzDir ? std::fma(beta, m, a) : a; };
+    W cm = src[int64_t(wrapz(z0 - 1)) * q.n12 + qc], c0v = src[int64_t(z0) * q.n12 + qc];
+    W cp = src[int64_t(wrapz(z0 + 1)) * q.n12 + qc];
+    W dm = W(0), d0 = W(0), dp = W(0);  // p of the centre column (direction pass)
+    if (OP == kPzDir) {
+      dm = pb[int64_t(wrapz(z0 - 1)) * q.n12 + qc];
+      d0 = pb[int64_t(z0) * q.n12 + qc];
+      dp = pb[int64_t(wrapz(z0 + 1)) * q.n12 + qc];
+    }
+    PzPlane<W> cur, nxt;
+    load(z0, nxt);
+    for (int z = z0; z < z1; ++z) {
+      cur = nxt;
+      load(wrapz(z + 1), nxt);
+      const int64_t o2 = int64_t(wrapz(z + 2)) * q.n12 + qc;
+      const W cn = src[o2];
+      const W dn = OP == kPzDir ? pb[o2] : W(0);
+      const W v0 = val(c0v, d0);
+      const W av = c0 * v0 + c1 * (val(cur.n[0], cur.m[0]) + val(cur.n[1], cur.m[1])) +
+                   c2 * (val(cur.n[2], cur.m[2]) + val(cur.n[3], cur.m[3])) + c3 * (val(cp, dp) + val(cm, dm));
+      const int64_t k = int64_t(z) * q.n12 + qc;
+      if (act) {
+        if (OP == kPzFirst) {  // r = g - A g ; p = r
+          const W rv = v0 - av;
+          rb[k] = rv;
+          A.pw[off + k] = rv;
+          acc += double(rv) * double(rv);
+        } else if (OP == kPzPap) {
+          acc += double(v0) * double(av);
+        } else if (OP == kPzDir) {  // p' = r + beta p (written apart)
+          A.pw[off + k] = v0;
+          acc += double(v0) * double(av);
+        } else {  // steps before the last: x += a p ; r -= a A p
+          const W xv = cur.u + al * v0;
+          xb[k] = xv;
+          const W rv = cur.v - al * av;
+          rb[k] = rv;
+          acc += double(rv) * double(rv);
+        }
+      }
+      cm = c0v;
+      c0v = cp;
+      cp = cn;
+      dm = d0;
+      d0 = dp;
+      dp = dn;
+    }
+  } else {  // last step: x += a p ; partial <x, g>
+    W gn = gb[int64_t(z0) * q.n12 + qc], pn = pb[int64_t(z0) * q.n12 + qc];
+    W xn = OP == kPzStepFL ? W(0) : xb[int64_t(z0) * q.n12 + qc];
+    for (int z = z0; z < z1; ++z) {
+      const W gv = gn, pv = pn, xo = xn;
+      const int64_t o1 = int64_t(wrapz(z + 1)) * q.n12 + qc;
+      gn = gb[o1];
+      pn = pb[o1];
+      if (OP != kPzStepFL) xn = xb[o1];
+      const int64_t k = int64_t(z) * q.n12 + qc;
+      if (act) {
+        const W xv = (OP == kPzStepFL ? gv : xo) + al * pv;
+        xb[k] = xv;
+        acc += double(xv) * double(gv);
+      }
     }
   }
-  block_partial(acc, partial, b, s, q.nseg);
+  block_partial(acc, A.partial, b, zr * q.nq + qb, q.nseg);
 }
 
 // A <- s (A + A^T) in place (both triangles), for an n x n matrix
@@ -567,8 +597,9 @@ int precond_hprsdft(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W *r, W 
 }
 
 
-// precond_hprsdft in fused passes for the CG step (no diagonal shift):
-// pg = P g and rb = dv <pg, g>.  The same iteration as precond_hprsdft.
+// precond_hprsdft in z-streaming passes for the CG step (no diagonal
+// shift): pg = P g and rb = dv <pg, g>.  The same iteration as
+// precond_hprsdft; f is the second direction buffer.
 template <typename W>
 int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W *r, W *p, W *f, Scratch &s,
                           double dv, double *rb) {
@@ -585,26 +616,38 @@ int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W 
   const W C0 = W(c0), C1 = W(-0.5 / (ctx->hgrid[0] * ctx->hgrid[0]) * ctx->ggg[0]);
   const W C2 = W(-0.5 / (ctx->hgrid[1] * ctx->hgrid[1]) * ctx->ggg[1]);
   const W C3 = W(-0.5 / (ctx->hgrid[2] * ctx->hgrid[2]) * ctx->ggg[2]);
-  PcGeom q;
+  PzGeom q;
   q.n1 = ctx->n[0];
   q.n2 = ctx->n[1];
   q.n3 = ctx->n[2];
   q.N = N;
   q.n12 = int64_t(q.n1) * q.n2;
-  constexpr int64_t kSeg = 16384;  // points per block
-  q.nseg = int((N + kSeg - 1) / kSeg);
-  q.seglen = (N + q.nseg - 1) / q.nseg;
-  q.nb = nn;
+  q.nq = int((q.n12 + kThreads - 1) / kThreads);
+  // enough blocks to fill the chip; z-runs of at least 4 planes
+  const int64_t per_run = int64_t(q.nq) * nn;
+  q.nzr = int(std::min<int64_t>(std::max(1, q.n3 / 4), std::max<int64_t>(1, (4096 + per_run - 1) / per_run)));
+  q.zlen = (q.n3 + q.nzr - 1) / q.nzr;
+  q.nzr = (q.n3 + q.zlen - 1) / q.zlen;
+  q.nseg = q.nq * q.nzr;
   if (int64_t(q.nseg) * nn > 0x7fffffff) return ctx->fail(RSG_ERR_UNSUPPORTED, "preconditioner: too many blocks");
   if (ctx->dotpart.ensure(size_t(nn) * q.nseg * sizeof(double)) != hipSuccess)
     return ctx->fail(RSG_ERR_HIP, "preconditioner: scratch allocation failed");
-  double *part = ctx->dotpart.as<double>();
+  PzArgs<W> A;
+  A.g = g;
+  A.x = pg;
+  A.r = r;
+  A.p = p;
+  A.pw = p;
+  A.a = s.a;
+  A.rr1 = s.rr1;
+  A.rr0 = s.rr0;
+  A.partial = ctx->dotpart.as<double>();
   const dim3 grid(unsigned(int64_t(q.nseg) * nn)), blk(kThreads), gb((nn + 63) / 64), b64(64);
   const dim3 gr((nn + 255) / 256);
   auto reduce = [&](double alpha, double *out) {
-    hipLaunchKernelGGL(k_reduce_partials, gr, dim3(256), 0, ctx->stream, part, q.nseg, nn, alpha, out);
+    hipLaunchKernelGGL(k_reduce_partials, gr, dim3(256), 0, ctx->stream, A.partial, q.nseg, nn, alpha, out);
   };
-  hipLaunchKernelGGL((k_pcf_first<W>), grid, blk, 0, ctx->stream, g, r, p, q, C0, C1, C2, C3, part);
+  hipLaunchKernelGGL((k_pcz<W, kPzFirst>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
   reduce(1.0, s.rr1);
   {
     std::vector<double> h(static_cast<size_t>(nn));
@@ -614,23 +657,28 @@ int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W 
     for (double v : h) small = small && (v < 1e-24);
     if (small) return plain();
   }
+  W *pcur = p, *pnext = f;
   for (int it = 1; it <= mloop; ++it) {
-    if (it == 1)
-      hipLaunchKernelGGL((k_pcf_mat<W>), grid, blk, 0, ctx->stream, p, f, q, C0, C1, C2, C3, part);
-    else
-      hipLaunchKernelGGL((k_pcf_dirmat<W>), grid, blk, 0, ctx->stream, r, p, f, s.rr1, s.rr0, q, C0, C1, C2, C3,
-                         part);
+    A.p = pcur;
+    if (it == 1) {
+      hipLaunchKernelGGL((k_pcz<W, kPzPap>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
+    } else {
+      A.pw = pnext;
+      hipLaunchKernelGGL((k_pcz<W, kPzDir>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
+      std::swap(pcur, pnext);
+      A.p = pcur;
+    }
     reduce(1.0, s.pap);
     hipLaunchKernelGGL(k_pc_alpha, gb, b64, 0, ctx->stream, s.rr1, s.pap, s.rr0, s.a, nn);
     const bool first = it == 1, last = it == mloop;
     if (first && last)
-      hipLaunchKernelGGL((k_pcf_step<W, true, true>), grid, blk, 0, ctx->stream, pg, g, p, r, f, s.a, q, part);
+      hipLaunchKernelGGL((k_pcz<W, kPzStepFL>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
     else if (first)
-      hipLaunchKernelGGL((k_pcf_step<W, true, false>), grid, blk, 0, ctx->stream, pg, g, p, r, f, s.a, q, part);
+      hipLaunchKernelGGL((k_pcz<W, kPzStepF>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
     else if (last)
-      hipLaunchKernelGGL((k_pcf_step<W, false, true>), grid, blk, 0, ctx->stream, pg, g, p, r, f, s.a, q, part);
+      hipLaunchKernelGGL((k_pcz<W, kPzStepL>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
     else
-      hipLaunchKernelGGL((k_pcf_step<W, false, false>), grid, blk, 0, ctx->stream, pg, g, p, r, f, s.a, q, part);
+      hipLaunchKernelGGL((k_pcz<W, kPzStepM>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
     reduce(last ? dv : 1.0, last ? rb : s.rr1);
   }
   RSG_HIP(hipGetLastError());

@@ -225,11 +225,12 @@ def test_nonlocal_modes_agree_with_oracle(orc, gpu_mod, nrep, ng, disp, nb):
     assert ctx.nonlocal_mode() == (0, True)
     psi = systems.random_bands(cell.npoints, nb, seed=300 + nb)
     want = ham.hpsi(psi)
-    for mode, order in ((1, 0), (2, 0), (2, 1)):
+    for mode, order, chunk in ((1, 0, 0), (2, 0, 0), (2, 1, 0), (2, 0, 4), (2, 1, 4)):
         ctx.set_nonlocal_mode(mode)
-        ctx.set_hpsi_order(64, order)
-        ctx.set_hpsi_order(32, order)
-        print(f"case: mode {mode} order {order}, Si{8 * nrep ** 3}, {nb} bands", flush=True)
+        for prec in (64, 32):
+            ctx.set_hpsi_shape(prec, chunk=chunk)
+            ctx.set_hpsi_order(prec, order)
+        print(f"case: mode {mode} order {order} chunk {chunk}, Si{8 * nrep ** 3}, {nb} bands", flush=True)
         _close(ctx.hamiltonian(psi), want, 1e-12)
         d_in = hip.DeviceArray.from_numpy(psi, dtype=np.float32)
         d_out = hip.DeviceArray(psi.shape, dtype=np.float32)
