@@ -46,6 +46,12 @@ __host__ __device__ inline int wrapi(int i, int n) {
   return i < 0 ? i + n : i;
 }
 
+// (b + u) mod n for b in [0, n) and u >= 0: selects for the usual u < 2n
+__host__ __device__ inline int add_wrap(int b, int u, int n) {
+  const int t = b + u;
+  return t < n ? t : (t < 2 * n ? t - n : t % n);
+}
+
 // Software pipeline of the z-stream.  Plane zz of the thread's own columns
 // lives in register slot (zz - z0 + MD) % R of a ring of R = 2*MD + 1 + PF
 // planes (PF planes fetched ahead, no register moves).  The halo rows, Vloc
@@ -85,9 +91,12 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
   extern __shared__ unsigned char smem_raw[];
   T *lds = reinterpret_cast<T *>(smem_raw);
   const int rows = ty + 2 * MD;
-  const int plane = rows * n1;  // one LDS buffer; slot [2*plane] is the dead slot
+  const int plane = rows * n1;  // one LDS buffer
   const int G = ty / RY;
   const int tid = threadIdx.x, nthr = blockDim.x;
+  // idle lanes write a dead slot of their own past both buffers: one slot
+  // shared by a whole wave would serialise its 64 writes
+  const int dslot = 2 * plane + (tid & 63);
   const bool active = tid < n1 * G;
   const int x = active ? tid % n1 : 0;
   const int g = active ? tid / n1 : 0;
@@ -98,7 +107,8 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
   const int band = int(lb / unsigned(ntile_y * nz));
   const int y0 = by * ty;
   const int z0 = bz * zlen, z1 = min(z0 + zlen, n3);
-  const int64_t n12 = int64_t(n1) * n2;
+  // offsets inside one band stay below N < 2^31: 32-bit arithmetic
+  const unsigned n12 = unsigned(n1) * unsigned(n2);
   // No __restrict__ on the streams: with it the loads count as invariant
   // and the register allocator re-issues them next to their uses instead of
   // keeping the prefetched values, which turns every plane into a full
@@ -106,15 +116,15 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
   const T *w = psi + int64_t(band) * ld;
   T *o = hpsi + int64_t(band) * ld;
 
-  int64_t colofs[RY];
+  unsigned colofs[RY];
   bool own[RY];
   int lown[RY];
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
     const int y = y0 + g * RY + r;
     own[r] = active && y < n2;
-    colofs[r] = x + int64_t(y % n2) * n1;  // rows past n2 in the last tile hold wrapped data
-    lown[r] = active ? (MD + g * RY + r) * n1 + x : 2 * plane;
+    colofs[r] = unsigned(x + (y % n2) * n1);  // rows past n2 in the last tile hold wrapped data
+    lown[r] = active ? (MD + g * RY + r) * n1 + x : dslot;
   }
   int xm[MD], xp[MD];
 #pragma unroll
@@ -125,26 +135,26 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
   // halo elements this thread stages: LDS offset and in-plane offset
   const int nhalo = 2 * MD * n1;
   int hl[HM];
-  int64_t hg[HM];
+  unsigned hg[HM];
 #pragma unroll
   for (int k = 0; k < HM; ++k) {
     const int idx = tid + k * nthr;
     const int i = idx < nhalo ? idx : 0;
     const int hr = i / n1, hx = i - hr * n1;
     const int lrow = hr < MD ? hr : hr + ty;
-    hl[k] = idx < nhalo ? lrow * n1 + hx : 2 * plane;
-    hg[k] = hx + int64_t(wrapi(y0 - MD + lrow, n2)) * n1;
+    hl[k] = idx < nhalo ? lrow * n1 + hx : dslot;
+    hg[k] = unsigned(hx + wrapi(y0 - MD + lrow, n2) * n1);
   }
   T q[RY][R];
 #pragma unroll
   for (int r = 0; r < RY; ++r)
 #pragma unroll
-    for (int k = 0; k < R; ++k) q[r][k] = w[colofs[r] + int64_t(wrapi(z0 - MD + k, n3)) * n12];
+    for (int k = 0; k < R; ++k) q[r][k] = w[colofs[r] + unsigned(wrapi(z0 - MD + k, n3)) * n12];
   // rotating sets: halo h[s][k], Vloc v[s][r], previous H psi a[s][r]
   T h[kHS][HM], v[kHS][RY], a[kHS][RY];
 #pragma unroll
   for (int sidx = 0; sidx < kHS - 1; ++sidx) {
-    const int64_t zo = int64_t(wrapi(z0 + sidx, n3)) * n12;
+    const unsigned zo = unsigned(wrapi(z0 + sidx, n3)) * n12;
 #pragma unroll
     for (int k = 0; k < HM; ++k) h[sidx][k] = w[hg[k] + zo];
 #pragma unroll
@@ -156,6 +166,9 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
   T *dst_dead = dead + (tid & 63);
 
   for (int zb = z0; zb < z1; zb += R) {
+    // planes of the halo / Vloc / H psi prefetch and of the own-column
+    // prefetch: one modulo per ring turn, a compare-select per step
+    const int zh0 = wrapi(zb + kHS - 1, n3), zq0 = wrapi(zb + MD + PF, n3);
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const int z = zb + u;
@@ -165,11 +178,12 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
       const int sc = u % kHS;              // set holding plane z
       const int sn = (u + kHS - 1) % kHS;  // set to fill with plane z+kHS-1
 #pragma unroll
-      for (int r = 0; r < RY; ++r) (lown[r] == 2 * plane ? lds : bufw)[lown[r]] = q[r][(u + MD) % R];
+      for (int r = 0; r < RY; ++r) (lown[r] == dslot ? lds : bufw)[lown[r]] = q[r][(u + MD) % R];
 #pragma unroll
-      for (int k = 0; k < HM; ++k) (hl[k] == 2 * plane ? lds : bufw)[hl[k]] = h[sc][k];
+      for (int k = 0; k < HM; ++k) (hl[k] == dslot ? lds : bufw)[hl[k]] = h[sc][k];
       {
-        const int64_t zo = int64_t(wrapi(z + kHS - 1, n3)) * n12;
+        const int zh = add_wrap(zh0, u, n3), zq = add_wrap(zq0, u, n3);
+        const unsigned zo = unsigned(zh) * n12;
 #pragma unroll
         for (int k = 0; k < HM; ++k) h[sn][k] = w[hg[k] + zo];
 #pragma unroll
@@ -177,12 +191,12 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
           v[sn][r] = vloc[colofs[r] + zo];
           if (ACC) a[sn][r] = o[colofs[r] + zo];
         }
-        const int64_t zf = int64_t(wrapi(z + MD + PF, n3)) * n12;
+        const unsigned zf = unsigned(zq) * n12;
 #pragma unroll
         for (int r = 0; r < RY; ++r) q[r][(u + R - 1) % R] = w[colofs[r] + zf];
       }
       lds_barrier();
-      const int64_t zb2 = int64_t(z) * n12;
+      const unsigned zb2 = unsigned(z) * n12;
       T col[RY + 2 * MD];
 #pragma unroll
       for (int k = 0; k < RY + 2 * MD; ++k) col[k] = buf[(g * RY + k) * n1 + x];
@@ -233,9 +247,9 @@ __global__ __launch_bounds__(512) void k_stencil2(const T *psi, T *hpsi, int64_t
   T *lds = reinterpret_cast<T *>(smem_raw);
   const int LS = n1 + 2 * MD;
   const int plane = (ty + 2 * MD) * LS;
-  const int deadslot = 2 * plane;  // one pair past both plane buffers
   const int hp = n1 / 2;
   const int tid = threadIdx.x, nthr = blockDim.x;
+  const int deadslot = 2 * plane + 2 * (tid & 63);  // a pair of this lane's own past both buffers
   const bool active = tid < hp * (ty / RY);
   const int px = active ? tid % hp : 0, g = active ? tid / hp : 0;
   const int x = 2 * px;
@@ -245,7 +259,7 @@ __global__ __launch_bounds__(512) void k_stencil2(const T *psi, T *hpsi, int64_t
   const int band = int(lb / unsigned(ntile_y * nz));
   const int y0 = by * ty;
   const int z0 = bz * zlen, z1 = min(z0 + zlen, n3);
-  const size_t n12 = size_t(n1) * n2;
+  const unsigned n12 = unsigned(n1) * unsigned(n2);  // in-band offsets < N < 2^31
   const T *w = psi + int64_t(band) * ld;
   T *o = hpsi + int64_t(band) * ld;
   unsigned colofs[RY];
@@ -273,14 +287,14 @@ __global__ __launch_bounds__(512) void k_stencil2(const T *psi, T *hpsi, int64_t
   V q[RY][R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    const T *wp = w + size_t(wrapi(z0 - MD + k, n3)) * n12;
+    const T *wp = w + unsigned(wrapi(z0 - MD + k, n3)) * n12;
 #pragma unroll
     for (int r = 0; r < RY; ++r) q[r][k] = *reinterpret_cast<const V *>(wp + colofs[r]);
   }
   V h[kHS][HM], v[kHS][RY], a[kHS][RY];
 #pragma unroll
   for (int sidx = 0; sidx < kHS - 1; ++sidx) {
-    const size_t zo = size_t(wrapi(z0 + sidx, n3)) * n12;
+    const unsigned zo = unsigned(wrapi(z0 + sidx, n3)) * n12;
 #pragma unroll
     for (int k = 0; k < HM; ++k) h[sidx][k] = *reinterpret_cast<const V *>(w + zo + hg[k]);
 #pragma unroll
@@ -310,7 +324,7 @@ __global__ __launch_bounds__(512) void k_stencil2(const T *psi, T *hpsi, int64_t
       *reinterpret_cast<V *>((hgh[k] == deadslot ? lds : buf) + hgh[k]) = h[sc][k];
     }
     {
-      const size_t zo = size_t(zh) * n12;
+      const unsigned zo = unsigned(zh) * n12;
 #pragma unroll
       for (int k = 0; k < HM; ++k) h[sn][k] = *reinterpret_cast<const V *>(w + zo + hg[k]);
 #pragma unroll
@@ -318,14 +332,14 @@ __global__ __launch_bounds__(512) void k_stencil2(const T *psi, T *hpsi, int64_t
         v[sn][r] = *reinterpret_cast<const V *>(vloc + zo + colofs[r]);
         if (ACC) a[sn][r] = *reinterpret_cast<const V *>(o + zo + colofs[r]);
       }
-      const T *wq = w + size_t(zq) * n12;
+      const T *wq = w + unsigned(zq) * n12;
 #pragma unroll
       for (int r = 0; r < RY; ++r) q[r][(u + R - 1) % R] = *reinterpret_cast<const V *>(wq + colofs[r]);
       zh = (zh + 1 == n3) ? 0 : zh + 1;
       zq = (zq + 1 == n3) ? 0 : zq + 1;
     }
     lds_barrier();
-    T *oz = o + size_t(z) * n12;
+    T *oz = o + unsigned(z) * n12;
     V col[RY + 2 * MD];
 #pragma unroll
     for (int k = 0; k < RY + 2 * MD; ++k) col[k] = *reinterpret_cast<const V *>(buf + (g * RY + k) * LS + MD + x);
@@ -699,7 +713,7 @@ int pick_ty(int n1, int n2, int md, int ry, size_t s) {
   for (int G = 1; n1 * G <= 512; ++G) {
     const int ty = G * ry;
     if (ty > ((n2 + ry - 1) / ry) * ry) break;
-    if ((2 * size_t(ty + 2 * md) * n1 + 1) * s > kMaxDynLds) break;
+    if ((2 * size_t(ty + 2 * md) * n1 + 64) * s > kMaxDynLds) break;
     const int th = n1 * G;
     if (2 * md * n1 > 4 * round64(th)) continue;
     const int tiles = (n2 + ty - 1) / ty;
@@ -720,7 +734,7 @@ constexpr int kPairMd = 6;
 inline bool pair_fits(int n1, int n2, int md, int ry, int ty, size_t s) {
   if (md != kPairMd || n1 % 2 || n1 < 2 * md || ty <= 0 || ty % ry || n2 % ty) return false;
   const int th = round64((n1 / 2) * (ty / ry));
-  const size_t lds = (2 * size_t(ty + 2 * md) * (n1 + 2 * md) + 2) * s;
+  const size_t lds = (2 * size_t(ty + 2 * md) * (n1 + 2 * md) + 128) * s;
   return th <= 512 && lds <= kMaxDynLds && 2 * md * (n1 / 2) <= 4 * th;
 }
 
@@ -748,7 +762,7 @@ int launch_stencil2_t(rsg_ctx *ctx, const float *psi, float *hpsi, int64_t ld, i
     return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: pair stencil tile does not fit");
   const int threads = round64((n1 / 2) * (sh.ty / RY));
   if (2 * MD * (n1 / 2) > HM * threads) return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: pair stencil halo");
-  const size_t lds = (2 * size_t(sh.ty + 2 * MD) * (n1 + 2 * MD) + 2) * sizeof(float);
+  const size_t lds = (2 * size_t(sh.ty + 2 * MD) * (n1 + 2 * MD) + 128) * sizeof(float);
   const int64_t nblk = int64_t(n2 / sh.ty) * ((n3 + sh.zlen - 1) / sh.zlen) * nb;
   if (nblk > 0x7fffffff) return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: too many stencil workgroups");
   const dim3 grid{unsigned(nblk), 1u, 1u};
@@ -780,7 +794,7 @@ int launch_stencil_t(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, co
   const int n1 = ctx->n[0], n2 = ctx->n[1], n3 = ctx->n[2];
   const int G = sh.ty / RY;
   const int threads = round64(n1 * G);
-  const size_t lds = (2 * size_t(sh.ty + 2 * MD) * n1 + 1) * sizeof(T);
+  const size_t lds = (2 * size_t(sh.ty + 2 * MD) * n1 + 64) * sizeof(T);
   if (threads > 512 || lds > kMaxDynLds || 2 * MD * n1 > HM * threads)
     return ctx->fail(RSG_ERR_UNSUPPORTED, "hpsi: stencil tile does not fit");
   const int64_t nblk = int64_t((n2 + sh.ty - 1) / sh.ty) * ((n3 + sh.zlen - 1) / sh.zlen) * nb;
@@ -872,7 +886,7 @@ HpsiShape resolve_shape(const rsg_ctx *ctx, const HpsiShape &in, size_t s, int n
   }
   if (sh.rx == 1 && (sh.ty <= 0 || sh.ty % sh.ry || round64(n1 * (sh.ty / sh.ry)) > 512 ||
       2 * ctx->md * n1 > 4 * round64(n1 * (sh.ty / sh.ry)) ||
-      (2 * size_t(sh.ty + 2 * ctx->md) * n1 + 1) * s > kMaxDynLds)) {
+      (2 * size_t(sh.ty + 2 * ctx->md) * n1 + 64) * s > kMaxDynLds)) {
     sh.ty = pick_ty(n1, n2, ctx->md, sh.ry, s);
     if (sh.ty == 0) {
       sh.ry = 1;
