@@ -34,7 +34,8 @@ class ScfParams:
     start: str = "normal"   # initial bands: "normal" N(0,1) from the host, "uniform" [0, 1) drawn on the device
                             # (random_initial_wf, wf_module.f90:166-191)
     mixed: bool = False     # FP32 CG/hpsi while dE/atom >= fp64_below, FP64 after
-    fp64_below: float = 1e-4
+    fp64_below: float = 0.0  # 0: twice etol_per_atom (FP32 energies follow FP64 to well below 1e-6 Ha/atom)
+    fp32_stall: int = 3      # ... or after this many FP32 iterations without a new lowest dE/atom
 
 
 @dataclasses.dataclass
@@ -226,11 +227,19 @@ class SCF:
         conv = False
         w0 = time.perf_counter()
         prec = 32 if self.p.mixed else 64
+        switch_below = self.p.fp64_below if self.p.fp64_below > 0 else 2.0 * self.p.etol_per_atom
+        best, stall = np.inf, 0
         for it in range(1, self.p.max_iter + 1):
             rec = self.step(it, e_prev, prec)
-            # FP32 is never trusted for the stop test: switch first
-            if prec == 32 and rec.de_per_atom < self.p.fp64_below:
-                prec = 64
+            # FP32 is never trusted for the stop test: switch first, also
+            # when FP32 stops making progress (its rounding floor)
+            if prec == 32:
+                if rec.de_per_atom < best:
+                    best, stall = rec.de_per_atom, 0
+                else:
+                    stall += 1
+                if rec.de_per_atom < switch_below or stall >= self.p.fp32_stall:
+                    prec = 64
             conv = rec.precision == 64 and rec.de_per_atom < self.p.etol_per_atom
             if not conv and it < self.p.max_iter:
                 rec.times["pot"] = self.update_potential()
