@@ -182,6 +182,18 @@ int rsg_subspace_diag(rsg_ctx *ctx, double *esp);
 int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop,
                            double *esp, double *res);
 /* pc_kind 5 = the learned preconditioner declared below. */
+
+/* The hprsdft preconditioner on its own: pg = P g for nb device bands of
+ * leading dimension ld (g, pg), mloop CG steps on -1/2 lap_7 x = g from
+ * x0 = g.  Replaces precond_hprsdft (cgpc_hprsdft_module.f90:133-270) with
+ * Pgk entering as gk (its caller, cg_module.f90, sets Pgk = gk first); E
+ * is not an argument, since the reference's E0 is zero.  Asynchronous on
+ * rsg_stream(ctx) apart from one read-back of the first residual norms (the
+ * reference's all(rr1 < 1e-24) early exit). */
+int rsg_precond_hprsdft_f64(rsg_ctx *ctx, int mloop, const double *g, double *pg,
+                            int64_t ld, int nb);
+int rsg_precond_hprsdft_f32(rsg_ctx *ctx, int mloop, const float *g, float *pg,
+                            int64_t ld, int nb);
 /* The same band update with every CG and preconditioner vector, hpsi
  * included, in FP32 storage with FP32 arithmetic; dot products accumulate
  * in FP64, the 2x2 Ritz step is FP64 and the accepted update lands in the

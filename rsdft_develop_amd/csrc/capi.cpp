@@ -474,6 +474,18 @@ int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double
   return rsg::conjugate_gradient(ctx, ncg, pc_kind, mloop, esp, res);
 }
 
+int rsg_precond_hprsdft_f64(rsg_ctx *ctx, int mloop, const double *g, double *pg, int64_t ld, int nb) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::precond_hprsdft_dev(ctx, mloop, g, pg, ld, nb);
+}
+
+int rsg_precond_hprsdft_f32(rsg_ctx *ctx, int mloop, const float *g, float *pg, int64_t ld, int nb) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::precond_hprsdft_dev(ctx, mloop, g, pg, ld, nb);
+}
+
 int rsg_conjugate_gradient_f32(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res) {
   if (!ctx || !esp || !res) return RSG_ERR_ARG;
   RSG_HIP(hipSetDevice(ctx->device));

@@ -192,6 +192,7 @@ struct rsg_ctx {
   rsg::DevBuf hostio;              // staging for the host hpsi entry
   rsg::DevBuf dotpart;             // multidot partial sums
   rsg::DevBuf grampart;            // per-chunk Gram matrices of the K-split U^T X products
+  rsg::DevBuf pcbuf;               // work vectors of the stand-alone preconditioner call
   rsg::DevBuf dead;                // store target of padding lanes (64 doubles)
   rsg::ScfState *scf = nullptr;    // density/potential state of the SCF step
   rsg::LpcState lpc;               // learned CG preconditioner

@@ -284,6 +284,7 @@ enum PzOp { kPzFirst, kPzPap, kPzDir, kPzStepF, kPzStepM, kPzStepL, kPzStepFL };
 struct PzGeom {
   int n1, n2, n3;
   int64_t N, n12;
+  int64_t ld;  // leading dimension of g and x (the work vectors r, p use N)
   int nq, nzr, zlen, nseg;
 };
 
@@ -343,8 +344,8 @@ __global__ __launch_bounds__(kThreads) void k_pcz(PzArgs<W> A, PzGeom q, W c0, W
                           i2 == 0 ? qc + int64_t(q.n2 - 1) * q.n1 : qc - q.n1};
   const int z0 = zr * q.zlen, z1 = min(z0 + q.zlen, q.n3);
   const int64_t off = int64_t(b) * q.N;
-  const W *gb = A.g + off, *pb = A.p + off;
-  W *xb = A.x + off, *rb = A.r + off;
+  const W *gb = A.g + int64_t(b) * q.ld, *pb = A.p + off;
+  W *xb = A.x + int64_t(b) * q.ld, *rb = A.r + off;
   const W al = (kStep || OP == kPzStepL || OP == kPzStepFL) ? W(A.a[b]) : W(0);
   const W beta = OP == kPzDir ? W(A.rr1[b] / A.rr0[b]) : W(0);
   const W *src = OP == kPzFirst ? gb : (OP == kPzDir ? rb : pb);  // stencil input (r for the direction pass)
@@ -602,13 +603,17 @@ int precond_hprsdft(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W *r, W 
 // precond_hprsdft; f is the second direction buffer.
 template <typename W>
 int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W *r, W *p, W *f, Scratch &s,
-                          double dv, double *rb) {
+                          double dv, double *rb, int64_t ld = 0) {
   const int64_t N = ctx->N;
-  auto plain = [&]() -> int {  // pg = g, rb = dv <g, g>
-    RSG_HIP(hipMemcpyAsync(pg, g, size_t(N) * nn * sizeof(W), hipMemcpyDeviceToDevice, ctx->stream));
+  if (ld <= 0) ld = N;
+  auto plain = [&]() -> int {  // pg = g, rb = dv <g, g> (rb only for ld = N)
+    RSG_HIP(hipMemcpy2DAsync(pg, size_t(ld) * sizeof(W), g, size_t(ld) * sizeof(W), size_t(N) * sizeof(W), nn,
+                             hipMemcpyDeviceToDevice, ctx->stream));
+    if (!rb || ld != N) return RSG_OK;
     const W *xs[1] = {pg}, *ys[1] = {g};
     return multidot(ctx, 1, xs, ys, nn, dv, rb);
   };
+  if (!rb) rb = s.rb;
   if (mloop <= 0) return plain();
   const double c0 = ctx->ggg[0] / (ctx->hgrid[0] * ctx->hgrid[0]) +
                     ctx->ggg[1] / (ctx->hgrid[1] * ctx->hgrid[1]) +
@@ -622,6 +627,7 @@ int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W 
   q.n3 = ctx->n[2];
   q.N = N;
   q.n12 = int64_t(q.n1) * q.n2;
+  q.ld = ld;
   q.nq = int((q.n12 + kThreads - 1) / kThreads);
   // enough blocks to fill the chip; z-runs of at least 4 planes
   const int64_t per_run = int64_t(q.nq) * nn;
@@ -943,6 +949,35 @@ int conjugate_gradient_f32(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double
 }
 
 namespace {
+
+// pg = P g for nb device bands of leading dimension ld: the hprsdft
+// preconditioner on its own, with its own work vectors.
+template <typename W> int precond_entry(rsg_ctx *ctx, int mloop, const W *g, W *pg, int64_t ld, int nb) {
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "precond_hprsdft: grid not set");
+  if (nb < 0 || ld < ctx->N || mloop < 0) return ctx->fail(RSG_ERR_ARG, "precond_hprsdft: bad nb, ld or mloop");
+  if (nb == 0) return RSG_OK;
+  if (!g || !pg) return ctx->fail(RSG_ERR_ARG, "precond_hprsdft: null buffer");
+  const size_t blk = size_t(ctx->N) * nb;
+  if (ctx->pcbuf.ensure(3 * blk * sizeof(W)) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "precond_hprsdft: work allocation failed");
+  Scratch s;
+  int rc = ensure_scalars(ctx, nb, 1, s);
+  if (rc) return rc;
+  W *r = ctx->pcbuf.as<W>(), *p = r + blk, *f = p + blk;
+  return precond_hprsdft_fused<W>(ctx, mloop, g, pg, nb, r, p, f, s, ctx->dv, nullptr, ld);
+}
+
+}  // namespace
+
+int precond_hprsdft_dev(rsg_ctx *ctx, int mloop, const double *g, double *pg, int64_t ld, int nb) {
+  return precond_entry<double>(ctx, mloop, g, pg, ld, nb);
+}
+
+int precond_hprsdft_dev(rsg_ctx *ctx, int mloop, const float *g, float *pg, int64_t ld, int nb) {
+  return precond_entry<float>(ctx, mloop, g, pg, ld, nb);
+}
+
+namespace {
 __global__ void k_resid32(const double *__restrict__ hx, const double *__restrict__ x, const double *__restrict__ E,
                           float *__restrict__ g, int64_t N) {
   const int b = blockIdx.y;
@@ -1033,6 +1068,7 @@ int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double
 void solver_release(rsg_ctx *ctx) {
   ctx->dotpart.release();
   ctx->grampart.release();
+  ctx->pcbuf.release();
 }
 
 }  // namespace rsg

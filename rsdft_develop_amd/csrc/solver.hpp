@@ -7,6 +7,8 @@ int gram_schmidt(rsg_ctx *ctx, int n0, int n1);
 int subspace_diag(rsg_ctx *ctx, double *esp);
 int conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res);
 int conjugate_gradient_f32(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res);
+int precond_hprsdft_dev(rsg_ctx *ctx, int mloop, const double *g, double *pg, int64_t ld, int nb);
+int precond_hprsdft_dev(rsg_ctx *ctx, int mloop, const float *g, float *pg, int64_t ld, int nb);
 int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double shift, double vweight);
 int band_residuals_f32(rsg_ctx *ctx, float *g);
 int esp_calc(rsg_ctx *ctx, double *esp);

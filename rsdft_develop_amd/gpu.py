@@ -81,6 +81,8 @@ def load_library():
         "rsg_set_hpsi_order": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_get_hpsi_order": (_c_int, [_vp, _c_int, _P(_c_int)]),
         "rsg_set_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int]),
+        "rsg_precond_hprsdft_f64": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
+        "rsg_precond_hprsdft_f32": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
         "rsg_get_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int, _P(_c_int)]),
         "rsg_alloc_bands": (_c_int, [_vp, _c_int]),
         "rsg_upload_bands": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int]),
@@ -265,6 +267,11 @@ class Context:
         """hpsi (+)= (T + Vloc) psi on device arrays (the local part of hpsi)."""
         fn = self.L.rsg_op_kinetic_f64 if dtype == np.float64 else self.L.rsg_op_kinetic_f32
         self._check(fn(self.h, d_psi, d_hpsi, int(ld), int(nb), int(bool(accumulate))))
+
+    def precond_device(self, d_g: int, d_pg: int, ld: int, nb: int, mloop: int = 3, dtype=np.float64):
+        """pg = P g, the hprsdft preconditioner on device arrays (precond_hprsdft)."""
+        fn = self.L.rsg_precond_hprsdft_f64 if np.dtype(dtype) == np.float64 else self.L.rsg_precond_hprsdft_f32
+        self._check(fn(self.h, int(mloop), _vp(d_g), _vp(d_pg), int(ld), int(nb)))
 
     def set_hpsi_shape(self, precision=64, ry=0, ty=0, zlen=0, chunk=0, bt=0):
         self._check(self.L.rsg_set_hpsi_shape(self.h, int(precision), int(ry), int(ty), int(zlen), int(chunk), int(bt)))

@@ -149,3 +149,53 @@ def test_fp32_conjugate_gradient_tracks_fp64(setup):
     e_ref = np.sort(ham.esp_calc(u))
     # lowest 12 states: converged in both runs
     np.testing.assert_allclose(e_mixed[:12], e_ref[:12], atol=1e-8)
+
+
+def _precond_gpu(ctx, g, ld, mloop, dtype):
+    from rsdft_develop_amd import hip
+
+    N, nb = g.shape
+    host = np.zeros((ld, nb), order="F")
+    host[:N] = g
+    d_g = hip.DeviceArray.from_numpy(host, dtype=dtype)
+    d_p = hip.DeviceArray((ld, nb), dtype=dtype)
+    d_p.fill_bytes(0xFF)  # NaN pattern: padding rows must stay untouched
+    ctx.precond_device(d_g.ptr, d_p.ptr, ld, nb, mloop=mloop, dtype=dtype)
+    ctx.synchronize()
+    got = d_p.to_numpy().astype(np.float64)
+    d_g.free()
+    d_p.free()
+    assert np.all(np.isnan(got[N:]))
+    return got[:N]
+
+
+@pytest.mark.parametrize("mloop", [1, 2, 3])
+def test_precond_hprsdft_matches_oracle(setup, mloop):
+    """rsg_precond_hprsdft against the oracle's precond_hprsdft.
+
+    Tolerances, relative to each band's largest |P g|: FP64 1e-12 (summation
+    order only); FP32 5e-5 (FP32 vectors and stencil, FP64 dot products)."""
+    cell, ham, ctx, _ = setup
+    N = cell.npoints
+    rng = np.random.default_rng(40 + mloop)
+    g = rng.standard_normal((N, 5)) * np.array([1.0, 1e-3, 30.0, 1.0, 2.0])
+    g[:, 3] = np.sin(np.arange(N) * 0.002)  # smooth band: long-wavelength content
+    want = ham.precond(g, mloop=mloop)
+    for dt, rel in ((np.float64, 1e-12), (np.float32, 5e-5)):
+        print(f"case: precond mloop {mloop} {np.dtype(dt).name}", flush=True)
+        got = _precond_gpu(ctx, g, N + 3, mloop, dt)
+        for b in range(g.shape[1]):
+            err = np.abs(got[:, b] - want[:, b]).max() / np.abs(want[:, b]).max()
+            assert err <= rel, (b, err)
+
+
+def test_precond_hprsdft_early_exits(setup):
+    """mloop 0 and residual norms all below 1e-24 both return P g = g."""
+    cell, ham, ctx, _ = setup
+    N = cell.npoints
+    rng = np.random.default_rng(7)
+    g = rng.standard_normal((N, 3))
+    np.testing.assert_array_equal(_precond_gpu(ctx, g, N, 0, np.float64), g)
+    tiny = 1e-17 * g  # |r|^2 ~ 1e-26 per band
+    np.testing.assert_array_equal(ham.precond(tiny, mloop=3), tiny)
+    np.testing.assert_array_equal(_precond_gpu(ctx, tiny, N, 3, np.float64), tiny)
