@@ -256,6 +256,7 @@ def batch_leg(ncells: int = 64, workers: int = 8, seq_cells: int | None = None):
 
 
 LPC_WEIGHTS = os.path.join(ROOT, "rsdft_develop_amd", "data", "lpc_si64.npz")
+LPC_HYBRID = os.path.join(ROOT, "rsdft_develop_amd", "data", "lpc_si64_hybrid.npz")
 
 
 def lpc_leg():
@@ -272,7 +273,12 @@ def lpc_leg():
     ev = train_lpc.evaluate(params, cell)
     keep = ("tolerance", "rounds_hprsdft", "rounds_learned", "rounds_none", "reached_learned", "fallbacks",
             "seconds_hprsdft", "seconds_learned", "seconds_none")
-    return {"workload": "held-out Si64 cell, FP64 CG(4) rounds at a fixed potential", **{k: ev[k] for k in keep}}
+    out = {"workload": "held-out Si64 cell, FP64 CG(4) rounds at a fixed potential", **{k: ev[k] for k in keep}}
+    if os.path.exists(LPC_HYBRID):  # the network as a correction of hprsdft (CG pc_kind 6)
+        hy = train_lpc.evaluate(lpc.load(LPC_HYBRID), cell, hybrid_scale=lpc.load_hybrid_scale(LPC_HYBRID))
+        out.update(rounds_hybrid=hy["rounds_learned"], reached_hybrid=hy["reached_learned"],
+                   fallbacks_hybrid=hy["fallbacks"], seconds_hybrid=hy["seconds_learned"])
+    return out
 
 
 def main() -> int:

@@ -181,7 +181,9 @@ int rsg_subspace_diag(rsg_ctx *ctx, double *esp);
  * mb). */
 int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop,
                            double *esp, double *res);
-/* pc_kind 5 = the learned preconditioner declared below. */
+/* pc_kind 5 = the learned preconditioner declared below; pc_kind 6 = the
+ * hybrid of both, P g = scale * P_hprsdft g + net(g)
+ * (rsg_lpc_set_hybrid_scale), with the same descent check. */
 
 /* The hprsdft preconditioner on its own: pg = P g for nb device bands of
  * leading dimension ld (g, pg), mloop CG steps on -1/2 lap_7 x = g from
@@ -297,6 +299,14 @@ int rsg_lpc_reset_optimizer(rsg_ctx *ctx);
 int rsg_band_residuals_f32(rsg_ctx *ctx, float *d_g);
 /* CG band blocks that fell back to precond_hprsdft since creation */
 int rsg_lpc_fallbacks(rsg_ctx *ctx, int64_t *count);
+/* pc_kind 6: the factor on the hprsdft output the network corrects
+ * (default 1). */
+int rsg_lpc_set_hybrid_scale(rsg_ctx *ctx, double scale);
+/* y = a y + b x on n device floats, on rsg_stream(ctx): training-target
+ * plumbing for the hybrid (t - mu P_hprsdft g).  No reference
+ * counterpart. */
+int rsg_axpby_f32(rsg_ctx *ctx, float *d_y, const float *d_x, float a,
+                  float b, int64_t n);
 
 #ifdef __cplusplus
 }

@@ -648,4 +648,17 @@ int rsg_lpc_fallbacks(rsg_ctx *ctx, int64_t *count) {
   return RSG_OK;
 }
 
+int rsg_lpc_set_hybrid_scale(rsg_ctx *ctx, double scale) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (!std::isfinite(scale)) return ctx->fail(RSG_ERR_ARG, "lpc_set_hybrid_scale: scale not finite");
+  ctx->lpc.hybrid_scale = scale;
+  return RSG_OK;
+}
+
+int rsg_axpby_f32(rsg_ctx *ctx, float *d_y, const float *d_x, float a, float b, int64_t n) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::axpby_f32(ctx, d_y, d_x, a, b, n);
+}
+
 }  // extern "C"

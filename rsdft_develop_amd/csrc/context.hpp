@@ -121,6 +121,7 @@ struct LpcState {
   int64_t vN = 0;
   int adam_t = 0;
   int64_t fallbacks = 0;  // CG blocks that fell back to hprsdft
+  double hybrid_scale = 1.0;  // pc_kind 6: P g = hybrid_scale * P_hprsdft g + network(g)
   LpcState() = default;
   LpcState(const LpcState &) = delete;
   LpcState &operator=(const LpcState &) = delete;

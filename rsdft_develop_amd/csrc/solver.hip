@@ -180,6 +180,13 @@ __global__ void k_cg_accept(double *__restrict__ x, W *__restrict__ x32, const W
   if (x32) x32[k] = W(v);
 }
 
+// y = a y + b x (n values)
+template <typename W>
+__global__ void k_axpby(W *__restrict__ y, const W *__restrict__ x, W a, W b, int64_t n) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < n) y[k] = a * y[k] + b * x[k];
+}
+
 // x32 = (W) x for an nn-band block
 template <typename W>
 __global__ void k_to_work(const double *__restrict__ x, W *__restrict__ x32, int64_t total) {
@@ -835,8 +842,8 @@ template <typename W>
 int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res) {
   int rc = check_store(ctx, "conjugate_gradient");
   if (rc) return rc;
-  if (ncg < 0 || (pc_kind != 0 && pc_kind != 4 && pc_kind != 5))
-    return ctx->fail(RSG_ERR_ARG, "conjugate_gradient: ncg < 0 or pc_kind not 0/4/5");
+  if (ncg < 0 || (pc_kind != 0 && pc_kind != 4 && pc_kind != 5 && pc_kind != 6))
+    return ctx->fail(RSG_ERR_ARG, "conjugate_gradient: ncg < 0 or pc_kind not 0/4/5/6");
   constexpr bool kMixed = sizeof(W) == 4;
   const int mb = ctx->mb;
   const int64_t N = ctx->N;
@@ -890,8 +897,18 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
       for (int b = 0; b < nn; ++b) conv = conv && (std::fabs(hE[b] - hE1[b]) < 1e-15);
       if (conv || icg == ncg + 1) break;
       // pgk = P gk and rb = dV <pgk, gk>
-      if (pc_kind == 4) {
+      if (pc_kind == 4 || pc_kind == 6) {
         if ((rc = precond_hprsdft_fused<W>(ctx, mloop, gk, pgk, nn, pr, pp, pf, s, dv, s.rb))) return rc;
+        if (pc_kind == 6) {
+          // hybrid: pgk = scale * P_hprsdft g + network(g); the
+          // preconditioner's work vectors are free again, pf takes the net
+          if ((rc = lpc_apply(ctx, gk, pf, N, nn, kMixed ? 32 : 64))) return rc;
+          const int64_t tot = N * nn;
+          hipLaunchKernelGGL((k_axpby<W>), dim3(unsigned((tot + kThreads - 1) / kThreads)), blkd, 0, ctx->stream,
+                             pgk, pf, W(ctx->lpc.hybrid_scale), W(1), tot);
+          const W *xs[1] = {pgk}, *ys[1] = {gk};
+          if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
+        }
       } else {
         if (pc_kind == 5) {
           if ((rc = lpc_apply(ctx, gk, pgk, N, nn, kMixed ? 32 : 64))) return rc;
@@ -901,7 +918,7 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
         const W *xs[1] = {pgk}, *ys[1] = {gk};
         if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
       }
-      if (pc_kind == 5) {
+      if (pc_kind == 5 || pc_kind == 6) {
         // descent check: <Pg, g> must be positive and finite for every band,
         // else the block falls back to the hprsdft smoother
         std::vector<double> rbh(static_cast<size_t>(nn));
@@ -946,6 +963,15 @@ int conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *es
 
 int conjugate_gradient_f32(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res) {
   return cg_impl<float>(ctx, ncg, pc_kind, mloop, esp, res);
+}
+
+int axpby_f32(rsg_ctx *ctx, float *y, const float *x, float a, float b, int64_t n) {
+  if (n < 0 || (n > 0 && (!x || !y))) return ctx->fail(RSG_ERR_ARG, "axpby: bad arguments");
+  if (n == 0) return RSG_OK;
+  hipLaunchKernelGGL((k_axpby<float>), dim3(unsigned((n + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     ctx->stream, y, x, a, b, n);
+  RSG_HIP(hipGetLastError());
+  return RSG_OK;
 }
 
 namespace {

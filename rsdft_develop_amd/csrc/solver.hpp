@@ -9,6 +9,7 @@ int conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *es
 int conjugate_gradient_f32(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res);
 int precond_hprsdft_dev(rsg_ctx *ctx, int mloop, const double *g, double *pg, int64_t ld, int nb);
 int precond_hprsdft_dev(rsg_ctx *ctx, int mloop, const float *g, float *pg, int64_t ld, int nb);
+int axpby_f32(rsg_ctx *ctx, float *y, const float *x, float a, float b, int64_t n);
 int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double shift, double vweight);
 int band_residuals_f32(rsg_ctx *ctx, float *g);
 int esp_calc(rsg_ctx *ctx, double *esp);

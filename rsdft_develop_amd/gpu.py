@@ -82,6 +82,8 @@ def load_library():
         "rsg_get_hpsi_order": (_c_int, [_vp, _c_int, _P(_c_int)]),
         "rsg_set_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_precond_hprsdft_f64": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
+        "rsg_lpc_set_hybrid_scale": (_c_int, [_vp, _c_dbl]),
+        "rsg_axpby_f32": (_c_int, [_vp, _vp, _vp, ctypes.c_float, ctypes.c_float, _c_i64]),
         "rsg_precond_hprsdft_f32": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
         "rsg_get_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int, _P(_c_int)]),
         "rsg_alloc_bands": (_c_int, [_vp, _c_int]),
@@ -272,6 +274,14 @@ class Context:
         """pg = P g, the hprsdft preconditioner on device arrays (precond_hprsdft)."""
         fn = self.L.rsg_precond_hprsdft_f64 if np.dtype(dtype) == np.float64 else self.L.rsg_precond_hprsdft_f32
         self._check(fn(self.h, int(mloop), _vp(d_g), _vp(d_pg), int(ld), int(nb)))
+
+    def lpc_set_hybrid_scale(self, scale: float):
+        """pc_kind 6: P g = scale * P_hprsdft g + network(g)."""
+        self._check(self.L.rsg_lpc_set_hybrid_scale(self.h, float(scale)))
+
+    def axpby_f32(self, d_y: int, d_x: int, a: float, b: float, n: int):
+        """y = a y + b x on n device floats (context stream)."""
+        self._check(self.L.rsg_axpby_f32(self.h, _vp(d_y), _vp(d_x), float(a), float(b), int(n)))
 
     def set_hpsi_shape(self, precision=64, ry=0, ty=0, zlen=0, chunk=0, bt=0):
         self._check(self.L.rsg_set_hpsi_shape(self.h, int(precision), int(ry), int(ty), int(zlen), int(chunk), int(bt)))

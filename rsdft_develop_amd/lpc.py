@@ -69,13 +69,15 @@ class Dataset:
     cells: int
     seconds: float
     blocks: list    # (first sample, count, Vloc of that cell and iteration)
+    h: object = None  # hip.DeviceArray (N, nsamp) FP32 hprsdft outputs P_h g (hybrid training), or None
 
 
 def make_dataset(ctx_factory, cells, scf_iters: int = 2, target_steps: int = 40, shift: float = 0.5,
-                 vweight: float = 1.0, max_samples: int | None = None, log=None) -> Dataset:
+                 vweight: float = 1.0, max_samples: int | None = None, log=None, hybrid_mloop: int = 0) -> Dataset:
     """Residual/target pairs from plain SCF runs on `cells` (all on one grid
     shape).  ctx_factory() returns a fresh gpu.Context.  Each cell gives
-    nband samples per SCF iteration."""
+    nband samples per SCF iteration.  hybrid_mloop > 0 also keeps the
+    hprsdft output of every residual (mloop steps), for hybrid training."""
     from . import gpu, hip, scf
 
     N = cells[0].npoints
@@ -85,6 +87,7 @@ def make_dataset(ctx_factory, cells, scf_iters: int = 2, target_steps: int = 40,
         total = min(total, max_samples)
     g_all = hip.DeviceArray((N, total), dtype=np.float32)
     t_all = hip.DeviceArray((N, total), dtype=np.float32)
+    h_all = hip.DeviceArray((N, total), dtype=np.float32) if hybrid_mloop > 0 else None
     t0 = time.perf_counter()
     filled = 0
     blocks = []
@@ -110,17 +113,60 @@ def make_dataset(ctx_factory, cells, scf_iters: int = 2, target_steps: int = 40,
             t_ptr = t_all.ptr + filled * N * 4
             hip.copy_d2d(g_ptr, stage.ptr, nb * N * 4)
             ctx.lpc_target_device(g_ptr, t_ptr, nb, steps=target_steps, shift=shift, vweight=vweight)
+            if h_all is not None:
+                ctx.precond_device(g_ptr, h_all.ptr + filled * N * 4, N, nb, mloop=hybrid_mloop, dtype=np.float32)
             stage.free()
             blocks.append((filled, nb, ctx.get_field(gpu.FIELD_VLOC)))
             filled += nb
         ctx.close()
         if log:
             log(f"dataset: cell {ic + 1}/{len(cells)}, {filled} samples, {time.perf_counter() - t0:.1f} s")
-    return Dataset(g_all, t_all, filled, N, len(cells), time.perf_counter() - t0, blocks)
+    return Dataset(g_all, t_all, filled, N, len(cells), time.perf_counter() - t0, blocks, h_all)
+
+
+def _probe_columns(data: Dataset, nprobe: int):
+    """Host copies (FP64) of g, t and h (if kept) at nprobe evenly spaced samples."""
+    import ctypes
+
+    from . import hip
+
+    N = data.npoints
+    idx = np.linspace(0, data.nsamp - 1, min(nprobe, data.nsamp)).astype(int)
+    out = []
+    for i in idx:
+        cols = []
+        for arr in (data.g, data.t, data.h):
+            if arr is None:
+                cols.append(None)
+                continue
+            v = np.empty(N, dtype=np.float32)
+            hip.runtime().hipMemcpy(v.ctypes.data, ctypes.c_void_p(arr.ptr + int(i) * N * 4), N * 4, 2)
+            cols.append(v.astype(np.float64))
+        out.append(cols)
+    return out
+
+
+def hybrid_targets(ctx, data: Dataset, nprobe: int = 16) -> tuple:
+    """Turn the targets into corrections of the hprsdft output for hybrid
+    training: kappa from the original targets, mu = <t, h> / <h, h> over
+    probe samples (the least-squares factor on h = P_h g), then t <- t - mu h
+    on the device.  The network then learns kappa t - kappa mu h, and CG's
+    pc_kind 6 applies P g = (kappa mu) P_h g + net(g).  Returns (kappa, mu)."""
+    if data.h is None:
+        raise ValueError("dataset was made without hprsdft outputs (hybrid_mloop = 0)")
+    kappa = target_scale(data)
+    th = hh = 0.0
+    for g, t, h in _probe_columns(data, nprobe):
+        th += float(np.dot(t, h))
+        hh += float(np.dot(h, h))
+    mu = th / hh
+    ctx.axpby_f32(data.t.ptr, data.h.ptr, 1.0, -mu, data.npoints * data.nsamp)
+    ctx.synchronize()
+    return kappa, mu
 
 
 def train(ctx, data: Dataset, params: np.ndarray, epochs: int = 4, batch: int = 16, lr: float = 2e-3,
-          seed: int = 0, log=None) -> tuple:
+          seed: int = 0, log=None, kappa: float | None = None) -> tuple:
     """Adam over shuffled mini-batches; a batch never mixes cells, so the
     network's potential input is that cell's Vloc.  Returns the trained
     parameters and the per-epoch mean loss."""
@@ -128,7 +174,8 @@ def train(ctx, data: Dataset, params: np.ndarray, epochs: int = 4, batch: int = 
     ctx.lpc_reset_optimizer()
     rng = np.random.default_rng(seed)
     N = data.npoints
-    kappa = target_scale(data)
+    if kappa is None:
+        kappa = target_scale(data)
     batches = [(ib, s) for ib, (first, count, _) in enumerate(data.blocks)
                for s in range(first, first + count - batch + 1, batch)]
     hist = []
@@ -173,6 +220,12 @@ def target_scale(data: Dataset, nprobe: int = 8) -> float:
 
 def save(path: str, params: np.ndarray, meta: dict | None = None):
     np.savez(path, params=np.asarray(params, dtype=np.float32), **{k: np.asarray(v) for k, v in (meta or {}).items()})
+
+
+def load_hybrid_scale(path: str):
+    """The pc_kind 6 scale saved with hybrid-trained weights, else None."""
+    with np.load(path, allow_pickle=False) as z:
+        return float(z["hybrid_scale"]) if "hybrid_scale" in z.files else None
 
 
 def load(path: str) -> np.ndarray:

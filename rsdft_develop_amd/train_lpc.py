@@ -2,6 +2,12 @@
 
     python -m rsdft_develop_amd.train_lpc --cells 32 --scf-iters 4 --epochs 4 \
         --out rsdft_develop_amd/data/lpc_si64.npz
+    python -m rsdft_develop_amd.train_lpc ... --hybrid 3 \
+        --out rsdft_develop_amd/data/lpc_si64_hybrid.npz
+
+With --hybrid the network is trained as a correction of the hprsdft output
+(CG pc_kind 6, P g = scale P_hprsdft g + net(g)); the scale is saved with
+the weights.
 
 Data: residuals of the plain method's own bands from `scf-iters` plain SCF
 iterations on `cells` perturbed Si64 cells (2x2x2, 72^3, 160 bands, atoms
@@ -31,10 +37,12 @@ def si64_cells(n, seed0=100):
     return [systems.silicon_supercell(2, displacement=0.05, seed=seed0 + i) for i in range(n)]
 
 
-def evaluate(params, cell, ref_rounds=8, max_rounds=40, scf_warm=3, precision=64, log=None):
+def evaluate(params, cell, ref_rounds=8, max_rounds=40, scf_warm=3, precision=64, log=None, hybrid_scale=None):
     """Rounds (subspace_diag -> CG(4) -> GS) to reach the plain method's
     residual tolerance with pc_kind 4 (hprsdft) and 5 (learned), at the
-    potential of `scf_warm` plain SCF iterations; same start bands."""
+    potential of `scf_warm` plain SCF iterations; same start bands.  With
+    hybrid_scale (weights trained as a correction of hprsdft) the learned
+    runs use pc_kind 6, P g = hybrid_scale P_hprsdft g + net(g)."""
     from . import gpu, scf, systems
 
     ctx = gpu.Context(0)
@@ -47,6 +55,10 @@ def evaluate(params, cell, ref_rounds=8, max_rounds=40, scf_warm=3, precision=64
     u0 = ctx.download_bands()
     ctx.lpc_set_params(params)
     ctx.lpc_set_potential()
+    pc_learned = 5
+    if hybrid_scale is not None:
+        ctx.lpc_set_hybrid_scale(hybrid_scale)
+        pc_learned = 6
 
     def rounds(pc, tol=None, nmax=max_rounds):
         ctx.upload_bands(u0)
@@ -65,9 +77,10 @@ def evaluate(params, cell, ref_rounds=8, max_rounds=40, scf_warm=3, precision=64
     ref_hist, _ = rounds(4, nmax=ref_rounds)
     tol = ref_hist[-1]
     plain, t_plain = rounds(4, tol)
-    learned, t_learned = rounds(5, tol)
+    learned, t_learned = rounds(pc_learned, tol)
     none, t_none = rounds(0, tol)
-    out = dict(tolerance=tol, rounds_hprsdft=len(plain), rounds_learned=len(learned), rounds_none=len(none),
+    out = dict(pc_kind_learned=pc_learned, tolerance=tol, rounds_hprsdft=len(plain), rounds_learned=len(learned),
+               rounds_none=len(none),
                reached_learned=learned[-1] <= tol, reached_none=none[-1] <= tol,
                seconds_hprsdft=round(t_plain, 3), seconds_learned=round(t_learned, 3), seconds_none=round(t_none, 3),
                fallbacks=ctx.lpc_fallbacks(), residuals_hprsdft=plain, residuals_learned=learned)
@@ -89,27 +102,40 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default="rsdft_develop_amd/data/lpc_si64.npz")
     ap.add_argument("--eval", action="store_true", help="evaluate on a held-out Si64 cell afterwards")
+    ap.add_argument("--hybrid", type=int, default=0,
+                    help="mloop > 0: train the network as a correction of hprsdft(mloop) (pc_kind 6)")
     args = ap.parse_args(argv)
     from . import gpu, lpc
 
     cells = si64_cells(args.cells)
     t0 = time.perf_counter()
     data = lpc.make_dataset(lambda: gpu.Context(0), cells, scf_iters=args.scf_iters, target_steps=args.target_steps,
-                            shift=args.shift, vweight=args.vweight, max_samples=args.max_samples, log=log)
+                            shift=args.shift, vweight=args.vweight, max_samples=args.max_samples, log=log,
+                            hybrid_mloop=args.hybrid)
     log(f"dataset: {data.nsamp} samples of {data.npoints} points in {data.seconds:.1f} s")
     ctx = gpu.Context(0)
     c0 = cells[0]
     ctx.set_grid(c0.ngrid, c0.aa, c0.md)
     log(f"training: {len(data.blocks)} blocks, batch {args.batch}, {args.epochs} epochs")
+    kappa, hybrid_scale = None, None
+    if args.hybrid > 0:
+        kappa, mu = lpc.hybrid_targets(ctx, data)
+        hybrid_scale = kappa * mu
+        log(f"hybrid: kappa {kappa:.4e}, mu {mu:.4e}, scale {hybrid_scale:.4e}")
     p0 = lpc.init_params(args.seed)
     t1 = time.perf_counter()
-    params, hist = lpc.train(ctx, data, p0, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed, log=log)
+    params, hist = lpc.train(ctx, data, p0, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed, log=log,
+                             kappa=kappa)
     train_s = time.perf_counter() - t1
     data.g.free()
     data.t.free()
+    if data.h is not None:
+        data.h.free()
     ctx.close()
     meta = dict(loss_history=np.array(hist), samples=data.nsamp, cells=args.cells, epochs=args.epochs,
                 shift=args.shift, vweight=args.vweight, target_steps=args.target_steps)
+    if hybrid_scale is not None:
+        meta.update(hybrid_scale=hybrid_scale, hybrid_mloop=args.hybrid)
     lpc.save(args.out, params, meta)
     summary = dict(samples=data.nsamp, dataset_s=round(data.seconds, 1), train_s=round(train_s, 1),
                    loss_history=hist, out=args.out, shift=args.shift, vweight=args.vweight,
@@ -118,7 +144,7 @@ def main(argv=None) -> int:
         from . import systems
 
         held_out = systems.silicon_supercell(2, displacement=0.05, seed=999)
-        summary["evaluation"] = evaluate(params, held_out)
+        summary["evaluation"] = evaluate(params, held_out, hybrid_scale=hybrid_scale)
     summary["total_s"] = round(time.perf_counter() - t0, 1)
     print(json.dumps(summary))
     return 0

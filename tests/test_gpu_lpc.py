@@ -188,3 +188,34 @@ def test_cg_with_learned_preconditioner_lowers_energies():
         assert np.all(e1 <= e0 + (1e-10 if prec == 64 else 1e-5))
         e0 = e1
     assert ctx.lpc_fallbacks() >= 0
+
+
+def test_hybrid_with_a_silent_network_is_hprsdft():
+    """pc_kind 6 with a network whose output is zero is hprsdft scaled by
+    the hybrid factor; CG is invariant to a positive scale of its
+    preconditioner, so both factors give the pc_kind 4 result (to the
+    rounding of the recomputed <Pg, g>)."""
+    from rsdft_develop_amd import gpu
+
+    cell = systems.silicon_supercell(1, ngrid_per_cell=24, nband=12, displacement=0.05, seed=4)
+    v = systems.model_vloc(cell)
+    ctx = gpu.Context().setup(cell, v)
+    d = lpc.unpack(lpc.init_params(0))
+    d["W3"] = np.zeros_like(d["W3"])
+    d["b3"] = np.zeros_like(d["b3"])
+    ctx.lpc_set_params(lpc.pack(d))
+    ctx.lpc_set_potential(v)
+    ctx.alloc_bands(cell.nband)
+    u0 = systems.random_bands(cell.npoints, cell.nband, seed=1)
+    out = {}
+    for key, pc, scale in (("hprsdft", 4, None), ("hybrid1", 6, 1.0), ("hybrid2", 6, 2.5)):
+        if scale is not None:
+            ctx.lpc_set_hybrid_scale(scale)
+        ctx.upload_bands(u0)
+        ctx.gram_schmidt()
+        ctx.subspace_diag()
+        out[key] = ctx.conjugate_gradient(ncg=4, pc_kind=pc, mloop=3)[0], ctx.download_bands()
+    for key in ("hybrid1", "hybrid2"):
+        np.testing.assert_allclose(out[key][0], out["hprsdft"][0], rtol=0, atol=1e-10)
+        assert np.abs(out[key][1] - out["hprsdft"][1]).max() < 1e-8 * np.abs(out["hprsdft"][1]).max()
+    assert ctx.lpc_fallbacks() == 0
