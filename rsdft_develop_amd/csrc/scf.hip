@@ -399,7 +399,12 @@ double ewald_energy(const double *aa, const double *bb, double vol, int natom, c
     q += z[a];
     q2 += z[a] * z[a];
   }
-  const double eta = kPi / std::pow(vol, 2.0 / 3.0);  // Gaussian width parameter
+  // Gaussian split parameter.  The energy does not depend on it; eta = pi
+  // natom^(1/3) / vol^(2/3) balances the real-space pair-image sum against
+  // the reciprocal sum (pi / vol^(2/3) alone makes the real-space cutoff grow
+  // with the cell: 13^3 image cells per pair at Si216, and its large
+  // background term cancels to 1e-7 Ha there in double precision).
+  const double eta = kPi * std::cbrt(double(std::max(natom, 1))) / std::pow(vol, 2.0 / 3.0);
   const double se = std::sqrt(eta);
   const double rcut = 8.0 / se, gcut = 16.0 * se;
   auto len = [](const double *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
