@@ -20,6 +20,45 @@ namespace {
 constexpr int kThreads = 256;
 
 // ---------------------------------------------------------------- reductions
+// The six products of the CG Ritz step, (x.x, p.x, p.p, x.Hx, p.Hx, p.Hp),
+// from one load of each of the four vectors per point; partial sums in the
+// k_multidot layout, partial[(q nb + b) nseg + s].
+template <typename W>
+__global__ __launch_bounds__(kThreads) void k_ritz_dots(const W *x, const W *p, const W *hx, const W *hp, int64_t N,
+                                                        int nb, int nseg, int64_t seglen, double *partial) {
+  constexpr int NP = 6;
+  __shared__ double red[kThreads / 64][NP];
+  const int s = blockIdx.x, b = blockIdx.y;
+  const int64_t i0 = int64_t(s) * seglen, i1 = min(N, i0 + seglen);
+  double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int64_t off = int64_t(b) * N;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const double xv = double(x[off + i]), pv = double(p[off + i]);
+    const double hxv = double(hx[off + i]), hpv = double(hp[off + i]);
+    acc[0] += xv * xv;
+    acc[1] += pv * xv;
+    acc[2] += pv * pv;
+    acc[3] += xv * hxv;
+    acc[4] += pv * hxv;
+    acc[5] += pv * hpv;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    double v = acc[q];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wid][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NP) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < kThreads / 64; ++k) v += red[k][threadIdx.x];
+    partial[(int64_t(threadIdx.x) * nb + b) * nseg + s] = v;
+  }
+}
+
 template <typename W> struct DotArgs {
   const W *x[6];
   const W *y[6];
@@ -481,7 +520,9 @@ int ensure_scalars(rsg_ctx *ctx, int nb, int nseg, Scratch &s) {
 
 int seg_count(int64_t N, int nb) {
   int nseg = 1;
-  while (nseg < 64 && int64_t(nseg) * nb < 1024 && N / (nseg * 2) >= 4096) nseg *= 2;
+  // enough blocks to fill the chip (8 four-wave blocks per CU), each
+  // segment at least 4096 points
+  while (nseg < 64 && int64_t(nseg) * nb < 4096 && N / (nseg * 2) >= 4096) nseg *= 2;
   return nseg;
 }
 
@@ -531,6 +572,28 @@ int multidot(rsg_ctx *ctx, int npair, const double *const *x, const double *cons
              double alpha, double *out) {
   return multidot_t<double>(ctx, npair, x, y, nb, alpha, out);
 }
+
+namespace {
+
+// out[q*nb + b] = alpha * the six Ritz-step products of band b
+template <typename W>
+int ritz_dots(rsg_ctx *ctx, const W *x, const W *p, const W *hx, const W *hp, int nb, double alpha, double *out) {
+  if (nb == 0) return RSG_OK;
+  const int64_t N = ctx->N;
+  const int nseg = seg_count(N, nb);
+  const int64_t seglen = (N + nseg - 1) / nseg;
+  if (ctx->dotpart.ensure(size_t(6) * nb * nseg * sizeof(double)) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "ritz_dots: scratch allocation failed");
+  double *partial = ctx->dotpart.as<double>();
+  hipLaunchKernelGGL((k_ritz_dots<W>), dim3(nseg, nb), dim3(kThreads), 0, ctx->stream, x, p, hx, hp, N, nb, nseg,
+                     seglen, partial);
+  const int count = 6 * nb;
+  hipLaunchKernelGGL(k_reduce_partials, dim3((count + 255) / 256), dim3(256), 0, ctx->stream, partial, nseg, count,
+                     alpha, out);
+  return RSG_OK;
+}
+
+}  // namespace
 
 int multidot(rsg_ctx *ctx, int npair, const float *const *x, const float *const *y, int nb, double alpha,
              double *out) {
@@ -934,10 +997,7 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
       hipLaunchKernelGGL((k_cg_dir<W>), gN, blkd, 0, ctx->stream, pgk, pk, s.rb, s.gkgk, icg == 1 ? 1 : 0, N);
       RSG_HIP(hipMemcpyAsync(s.gkgk, s.rb, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
       if ((rc = launch_hpsi<W>(ctx, pk, hpk, N, nn))) return rc;
-      {
-        const W *xs[6] = {x, pk, pk, x, pk, pk}, *ys[6] = {x, x, pk, hxk, hxk, hpk};
-        if ((rc = multidot(ctx, 6, xs, ys, nn, dv, s.d6))) return rc;
-      }
+      if ((rc = ritz_dots<W>(ctx, x, pk, hxk, hpk, nn, dv, s.d6))) return rc;
       hipLaunchKernelGGL(k_cg_ritz, gb, b64, 0, ctx->stream, s.d6, nn, s.E, s.E1, s.uc);
       hipLaunchKernelGGL((k_cg_update<W>), gN, blkd, 0, ctx->stream, hxk, hpk, x, pk, gk, s.uc, s.E, N);
       {
