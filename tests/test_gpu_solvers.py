@@ -181,12 +181,14 @@ def test_precond_hprsdft_matches_oracle(setup, mloop):
     g = rng.standard_normal((N, 5)) * np.array([1.0, 1e-3, 30.0, 1.0, 2.0])
     g[:, 3] = np.sin(np.arange(N) * 0.002)  # smooth band: long-wavelength content
     want = ham.precond(g, mloop=mloop)
-    for dt, rel in ((np.float64, 1e-12), (np.float32, 5e-5)):
-        print(f"case: precond mloop {mloop} {np.dtype(dt).name}", flush=True)
-        got = _precond_gpu(ctx, g, N + 3, mloop, dt)
-        for b in range(g.shape[1]):
-            err = np.abs(got[:, b] - want[:, b]).max() / np.abs(want[:, b]).max()
-            assert err <= rel, (b, err)
+    # odd leading dimension: one point per thread; even: two (pair loads)
+    for ld in (N + 3, N + 4):
+        for dt, rel in ((np.float64, 1e-12), (np.float32, 5e-5)):
+            print(f"case: precond mloop {mloop} ld N+{ld - N} {np.dtype(dt).name}", flush=True)
+            got = _precond_gpu(ctx, g, ld, mloop, dt)
+            for b in range(g.shape[1]):
+                err = np.abs(got[:, b] - want[:, b]).max() / np.abs(want[:, b]).max()
+                assert err <= rel, (ld - N, b, err)
 
 
 def test_precond_hprsdft_early_exits(setup):
