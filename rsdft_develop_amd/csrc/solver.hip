@@ -480,18 +480,18 @@ __global__ __launch_bounds__(kThreads) void k_pcz(PzArgs<W> A, PzGeom q, W c0, W
   block_partial(acc, A.partial, b, zr * q.nq + qb, q.nseg);
 }
 
-// Two x-points per thread (n1 and the leading dimension even): the same
-// passes with 2-wide vector loads and stores, half the load instructions
-// per point.  The in-plane neighbours of the pair are the two x-outer
-// points (scalar loads) and the y-rows above and below (pair loads); the
-// stencil sums are formed in the same order as in k_pcz.
-template <typename W> struct PzVec;
-template <> struct PzVec<float> { typedef float2 T; };
-template <> struct PzVec<double> { typedef double2 T; };
+// VP x-points per thread (n1 and the leading dimension multiples of VP):
+// the same passes with VP-wide vector loads and stores, 1/VP of the load
+// instructions per point.  The in-plane neighbours of the group are its two
+// x-outer points (scalar loads) and the y-rows above and below (vector
+// loads); each point's stencil sum is formed in the same order as in k_pcz.
+template <typename W, int VP> struct alignas(VP * sizeof(W)) PzVec {
+  W v[VP];
+};
 
-template <typename W, int OP>
-__global__ __launch_bounds__(kThreads) void k_pcz2(PzArgs<W> A, PzGeom q, W c0, W c1, W c2, W c3) {
-  typedef typename PzVec<W>::T V;
+template <typename W, int OP, int VP>
+__global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W> A, PzGeom q, W c0, W c1, W c2, W c3) {
+  typedef PzVec<W, VP> V;
   constexpr bool kStencil = OP != kPzStepL && OP != kPzStepFL;
   constexpr bool kStep = OP == kPzStepF || OP == kPzStepM;
   const unsigned nblk = gridDim.x, xc = blockIdx.x % 8;
@@ -499,14 +499,14 @@ __global__ __launch_bounds__(kThreads) void k_pcz2(PzArgs<W> A, PzGeom q, W c0, 
   const int qb = int(lb % unsigned(q.nq));
   const int zr = int((lb / unsigned(q.nq)) % unsigned(q.nzr));
   const int b = int(lb / unsigned(q.nq * q.nzr));
-  const int hp = q.n1 / 2;
+  const int hv = q.n1 / VP;  // groups per row
   const int64_t qi = int64_t(qb) * kThreads + threadIdx.x;
-  const bool act = qi < q.n12 / 2;
+  const bool act = qi < q.n12 / VP;
   const int64_t qc = act ? qi : 0;
-  const int x = 2 * int(qc % hp), y = int(qc / hp);
-  const int64_t e0 = x + int64_t(y) * q.n1;  // first point of the pair in its plane
+  const int x = VP * int(qc % hv), y = int(qc / hv);
+  const int64_t e0 = x + int64_t(y) * q.n1;  // first point of the group in its plane
   const int64_t exl = (x == 0 ? q.n1 - 1 : x - 1) + int64_t(y) * q.n1;
-  const int64_t exr = (x + 2 == q.n1 ? 0 : x + 2) + int64_t(y) * q.n1;
+  const int64_t exr = (x + VP == q.n1 ? 0 : x + VP) + int64_t(y) * q.n1;
   const int64_t eyu = x + int64_t(y == q.n2 - 1 ? 0 : y + 1) * q.n1;
   const int64_t eyd = x + int64_t(y == 0 ? q.n2 - 1 : y - 1) * q.n1;
   const int z0 = zr * q.zlen, z1 = min(z0 + q.zlen, q.n3);
@@ -516,64 +516,80 @@ __global__ __launch_bounds__(kThreads) void k_pcz2(PzArgs<W> A, PzGeom q, W c0, 
   const W al = (kStep || OP == kPzStepL || OP == kPzStepFL) ? W(A.a[b]) : W(0);
   const W beta = OP == kPzDir ? W(A.rr1[b] / A.rr0[b]) : W(0);
   const W *src = OP == kPzFirst ? gb : (OP == kPzDir ? rb : pb);
-  auto ld2 = [](const W *base, int64_t e) { return *reinterpret_cast<const V *>(base + e); };
-  auto st2 = [](W *base, int64_t e, W a, W c) {
+  auto ldv = [](const W *base, int64_t e) { return *reinterpret_cast<const V *>(base + e); };
+  auto stv = [](W *base, int64_t e, const W *w) {
     V v;
-    v.x = a;
-    v.y = c;
+#pragma unroll
+    for (int i = 0; i < VP; ++i) v.v[i] = w[i];
     *reinterpret_cast<V *>(base + e) = v;
   };
   auto comb = [&](W r, W m) -> W { return OP == kPzDir ? std::fma(beta, m, r) : r; };
   auto wrapz = [&](int z) { return ((z % q.n3) + q.n3) % q.n3; };
   double acc = 0.0;
   if constexpr (kStencil) {
-    // ring of the pair at z-1, z, z+1 (raw input; p alongside for the direction pass)
-    V cm = ld2(src, int64_t(wrapz(z0 - 1)) * q.n12 + e0), cz = ld2(src, int64_t(z0) * q.n12 + e0);
-    V cp = ld2(src, int64_t(wrapz(z0 + 1)) * q.n12 + e0);
+    // ring of the group at z-1, z, z+1 (raw input; p alongside for the direction pass)
+    V cm = ldv(src, int64_t(wrapz(z0 - 1)) * q.n12 + e0), cz = ldv(src, int64_t(z0) * q.n12 + e0);
+    V cp = ldv(src, int64_t(wrapz(z0 + 1)) * q.n12 + e0);
     V dm{}, dz{}, dp{};
     if (OP == kPzDir) {
-      dm = ld2(pb, int64_t(wrapz(z0 - 1)) * q.n12 + e0);
-      dz = ld2(pb, int64_t(z0) * q.n12 + e0);
-      dp = ld2(pb, int64_t(wrapz(z0 + 1)) * q.n12 + e0);
+      dm = ldv(pb, int64_t(wrapz(z0 - 1)) * q.n12 + e0);
+      dz = ldv(pb, int64_t(z0) * q.n12 + e0);
+      dp = ldv(pb, int64_t(wrapz(z0 + 1)) * q.n12 + e0);
     }
     int zn = wrapz(z0 + 1);
     for (int z = z0; z < z1; ++z) {
       const int64_t o = int64_t(z) * q.n12;
       zn = zn + 1 == q.n3 ? 0 : zn + 1;  // plane z + 2
-      const V cn = ld2(src, int64_t(zn) * q.n12 + e0);
+      const V cn = ldv(src, int64_t(zn) * q.n12 + e0);
       V dn{};
-      if (OP == kPzDir) dn = ld2(pb, int64_t(zn) * q.n12 + e0);
+      if (OP == kPzDir) dn = ldv(pb, int64_t(zn) * q.n12 + e0);
       const W sl = comb(src[o + exl], OP == kPzDir ? pb[o + exl] : W(0));
       const W sr = comb(src[o + exr], OP == kPzDir ? pb[o + exr] : W(0));
-      const V yu = ld2(src, o + eyu), yd = ld2(src, o + eyd);
+      const V yu = ldv(src, o + eyu), yd = ldv(src, o + eyd);
       V mu{}, md{};
       if (OP == kPzDir) {
-        mu = ld2(pb, o + eyu);
-        md = ld2(pb, o + eyd);
+        mu = ldv(pb, o + eyu);
+        md = ldv(pb, o + eyd);
       }
-      const W v0 = comb(cz.x, dz.x), v1 = comb(cz.y, dz.y);
-      const W a0 = c0 * v0 + c1 * (v1 + sl) + c2 * (comb(yu.x, mu.x) + comb(yd.x, md.x)) +
-                   c3 * (comb(cp.x, dp.x) + comb(cm.x, dm.x));
-      const W a1 = c0 * v1 + c1 * (sr + v0) + c2 * (comb(yu.y, mu.y) + comb(yd.y, md.y)) +
-                   c3 * (comb(cp.y, dp.y) + comb(cm.y, dm.y));
+      W v[VP], av[VP];
+#pragma unroll
+      for (int i = 0; i < VP; ++i) v[i] = comb(cz.v[i], dz.v[i]);
+#pragma unroll
+      for (int i = 0; i < VP; ++i) {
+        const W xp = i + 1 < VP ? v[i + 1 < VP ? i + 1 : i] : sr;
+        const W xm = i > 0 ? v[i > 0 ? i - 1 : i] : sl;
+        av[i] = c0 * v[i] + c1 * (xp + xm) + c2 * (comb(yu.v[i], mu.v[i]) + comb(yd.v[i], md.v[i])) +
+                c3 * (comb(cp.v[i], dp.v[i]) + comb(cm.v[i], dm.v[i]));
+      }
       const int64_t k = o + e0;
       if (act) {
         if (OP == kPzFirst) {  // r = g - A g ; p = r
-          const W r0 = v0 - a0, r1 = v1 - a1;
-          st2(rb, k, r0, r1);
-          st2(A.pw + off, k, r0, r1);
-          acc += double(r0) * double(r0) + double(r1) * double(r1);
+          W rv[VP];
+#pragma unroll
+          for (int i = 0; i < VP; ++i) {
+            rv[i] = v[i] - av[i];
+            acc += double(rv[i]) * double(rv[i]);
+          }
+          stv(rb, k, rv);
+          stv(A.pw + off, k, rv);
         } else if (OP == kPzPap) {
-          acc += double(v0) * double(a0) + double(v1) * double(a1);
+#pragma unroll
+          for (int i = 0; i < VP; ++i) acc += double(v[i]) * double(av[i]);
         } else if (OP == kPzDir) {  // p' = r + beta p (written apart)
-          st2(A.pw + off, k, v0, v1);
-          acc += double(v0) * double(a0) + double(v1) * double(a1);
+          stv(A.pw + off, k, v);
+#pragma unroll
+          for (int i = 0; i < VP; ++i) acc += double(v[i]) * double(av[i]);
         } else {  // steps before the last: x += a p ; r -= a A p
-          const V u = ld2(OP == kPzStepF ? gb : xb, k), rr = ld2(rb, k);
-          st2(xb, k, u.x + al * v0, u.y + al * v1);
-          const W r0 = rr.x - al * a0, r1 = rr.y - al * a1;
-          st2(rb, k, r0, r1);
-          acc += double(r0) * double(r0) + double(r1) * double(r1);
+          const V u = ldv(OP == kPzStepF ? gb : xb, k), rr = ldv(rb, k);
+          W xv[VP], rv[VP];
+#pragma unroll
+          for (int i = 0; i < VP; ++i) {
+            xv[i] = u.v[i] + al * v[i];
+            rv[i] = rr.v[i] - al * av[i];
+            acc += double(rv[i]) * double(rv[i]);
+          }
+          stv(xb, k, xv);
+          stv(rb, k, rv);
         }
       }
       cm = cz;
@@ -587,11 +603,15 @@ __global__ __launch_bounds__(kThreads) void k_pcz2(PzArgs<W> A, PzGeom q, W c0, 
     for (int z = z0; z < z1; ++z) {
       const int64_t k = int64_t(z) * q.n12 + e0;
       if (act) {
-        const V gv = ld2(gb, k), pv = ld2(pb, k);
-        const V xo = OP == kPzStepFL ? gv : ld2(xb, k);
-        const W x0 = xo.x + al * pv.x, x1 = xo.y + al * pv.y;
-        st2(xb, k, x0, x1);
-        acc += double(x0) * double(gv.x) + double(x1) * double(gv.y);
+        const V gv = ldv(gb, k), pv = ldv(pb, k);
+        const V xo = OP == kPzStepFL ? gv : ldv(xb, k);
+        W xv[VP];
+#pragma unroll
+        for (int i = 0; i < VP; ++i) {
+          xv[i] = xo.v[i] + al * pv.v[i];
+          acc += double(xv[i]) * double(gv.v[i]);
+        }
+        stv(xb, k, xv);
       }
     }
   }
@@ -817,12 +837,16 @@ int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W 
   q.N = N;
   q.n12 = int64_t(q.n1) * q.n2;
   q.ld = ld;
-  // two points per thread where every band column starts on a pair (even
-  // n1 and leading dimension, vector-aligned caller buffers)
-  const uintptr_t va = 2 * sizeof(W);
-  const bool pairs = q.n1 % 2 == 0 && ld % 2 == 0 && reinterpret_cast<uintptr_t>(g) % va == 0 &&
-                     reinterpret_cast<uintptr_t>(pg) % va == 0;
-  q.nq = int(((pairs ? q.n12 / 2 : q.n12) + kThreads - 1) / kThreads);
+  // VP points per thread where every band column starts on a group: n1 and
+  // the leading dimension multiples of VP, caller buffers aligned to the
+  // VP-wide vector (4 floats or 2 doubles = 16 bytes, else 2 floats)
+  auto fits = [&](int v) {
+    const uintptr_t va = uintptr_t(v) * sizeof(W);
+    return q.n1 % v == 0 && ld % v == 0 && reinterpret_cast<uintptr_t>(g) % va == 0 &&
+           reinterpret_cast<uintptr_t>(pg) % va == 0;
+  };
+  const int vp = (sizeof(W) == 4 && fits(4)) ? 4 : (fits(2) ? 2 : 1);
+  q.nq = int((q.n12 / vp + kThreads - 1) / kThreads);
   // enough blocks to fill the chip; z-runs of at least 4 planes
   const int64_t per_run = int64_t(q.nq) * nn;
   q.nzr = int(std::min<int64_t>(std::max(1, q.n3 / 4), std::max<int64_t>(1, (4096 + per_run - 1) / per_run)));
@@ -849,8 +873,10 @@ int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W 
   };
   auto pass = [&](auto opc) {
     constexpr int OP = decltype(opc)::value;
-    if (pairs)
-      hipLaunchKernelGGL((k_pcz2<W, OP>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
+    if (vp == 4)
+      hipLaunchKernelGGL((k_pczv<W, OP, 4>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
+    else if (vp == 2)
+      hipLaunchKernelGGL((k_pczv<W, OP, 2>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
     else
       hipLaunchKernelGGL((k_pcz<W, OP>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
   };

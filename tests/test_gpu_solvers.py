@@ -181,8 +181,9 @@ def test_precond_hprsdft_matches_oracle(setup, mloop):
     g = rng.standard_normal((N, 5)) * np.array([1.0, 1e-3, 30.0, 1.0, 2.0])
     g[:, 3] = np.sin(np.arange(N) * 0.002)  # smooth band: long-wavelength content
     want = ham.precond(g, mloop=mloop)
-    # odd leading dimension: one point per thread; even: two (pair loads)
-    for ld in (N + 3, N + 4):
+    # points per thread from the leading dimension (N = 36^3): odd -> 1;
+    # N + 2 -> 2; N + 4 -> 4 in FP32 (16-byte loads), 2 in FP64
+    for ld in (N + 3, N + 2, N + 4):
         for dt, rel in ((np.float64, 1e-12), (np.float32, 5e-5)):
             print(f"case: precond mloop {mloop} ld N+{ld - N} {np.dtype(dt).name}", flush=True)
             got = _precond_gpu(ctx, g, ld, mloop, dt)
