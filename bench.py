@@ -180,6 +180,22 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     return out
 
 
+def scf_warmup(nrep: int):
+    """One FP32 and one FP64 SCF iteration of the same cell, untimed, so that
+    neither timed run pays the process's first use of the kernels and
+    library paths at this size (rocSOLVER's eigensolver for its band count
+    among them)."""
+    from rsdft_develop_amd import gpu, scf, systems
+
+    cell = systems.silicon_supercell(nrep, nband=NBANDS[nrep], displacement=0.05, seed=0)
+    ctx = gpu.Context(0)
+    ctx.set_grid(cell.ngrid, cell.aa, cell.md)
+    ctx.build_gth_projectors(cell)
+    scf.SCF(cell, scf.ScfParams(ekbt=0.01, max_iter=2, mixed=True, fp64_below=1e9, imix=30, start="uniform"),
+            ctx=ctx).run()
+    ctx.close()
+
+
 def step_leg(nrep: int, steps: int):
     """Time per SCF step of Si(8 nrep^3) in mixed precision and in FP64 for
     the same number of steps (IMIX 30, EKBT 0.01 Ha, device start bands), the
@@ -317,6 +333,8 @@ def main() -> int:
     cpu = None if args.no_cpu else cpu_leg(cell, vloc)
     scf_out = None
     if args.scf > 0:
+        progress("scf warm-up")
+        scf_warmup(args.scf)
         plain = scf_leg(args.scf, mixed=False)
         mixed = scf_leg(args.scf, mixed=True)
         natom = 8 * args.scf ** 3
