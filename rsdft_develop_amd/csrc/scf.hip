@@ -31,6 +31,9 @@ struct ScfState {
   int imix = 10;
   std::vector<DevBuf> gxin, gxou;
   DevBuf wq, wq2, gpart;
+  // atom positions, kinds and species parameters of the local potential
+  // (kept: a free would wait for the whole device)
+  DevBuf dfrac, dkind, dsp;
   // GTH local parameters per species
   std::vector<double> zps, rloc, cloc;
   std::vector<double> frac;
@@ -495,7 +498,7 @@ int scf_set_local_gth(rsg_ctx *ctx, int natom, const double *frac, const int *ki
   }
   for (int a = 0; a < natom; ++a)
     if (kind[a] < 0 || kind[a] >= nkind) return ctx->fail(RSG_ERR_ARG, "set_local_gth: kind out of range");
-  DevBuf dfrac, dkind, dsp;
+  DevBuf &dfrac = st->dfrac, &dkind = st->dkind, &dsp = st->dsp;
   if (dfrac.ensure(size_t(std::max(natom, 1)) * 24) || dkind.ensure(size_t(std::max(natom, 1)) * 4) ||
       dsp.ensure(sp.size() * 8))
     return ctx->fail(RSG_ERR_HIP, "set_local_gth: allocation failed");
@@ -512,9 +515,6 @@ int scf_set_local_gth(rsg_ctx *ctx, int natom, const double *frac, const int *ki
   hipLaunchKernelGGL(k_real_part, dim3(nblk(N)), dim3(kT), 0, ctx->stream, st->zbuf.as<double2>(),
                      st->vion.as<double>(), N);
   RSG_HIP(hipStreamSynchronize(ctx->stream));
-  dfrac.release();
-  dkind.release();
-  dsp.release();
   std::vector<double> zat(static_cast<size_t>(natom));
   st->nelectron = 0.0;
   for (int a = 0; a < natom; ++a) {
@@ -732,27 +732,29 @@ int scf_mix_init(rsg_ctx *ctx, int mmix, double beta, int imix) {
   if (imix != 10 && imix != 30) return ctx->fail(RSG_ERR_ARG, "mix_init: imix must be 10 or 30");
   if (imix == 30 && mmix > 4) return ctx->fail(RSG_ERR_ARG, "mix_init: imix 30 takes mmix <= 4");
   ScfState *st = ctx->scf;
-  for (auto &b : st->xin) b.release();
-  for (auto &b : st->xou) b.release();
-  for (auto &b : st->gxin) b.release();
-  for (auto &b : st->gxou) b.release();
-  st->xin.clear();
-  st->xou.clear();
-  st->gxin.clear();
-  st->gxou.clear();
+  // history slots are kept across SCF runs and only grown: a free waits for
+  // the whole device, which stalls every other context's stream
+  auto resize = [](std::vector<DevBuf> &v, size_t n) {
+    for (size_t i = n; i < v.size(); ++i) v[i].release();
+    v.resize(n);
+  };
   st->imix = imix;
   const size_t n8 = size_t(ctx->N) * 8;
   if (imix == 10) {
-    st->xin.assign(size_t(mmix), DevBuf());
-    st->xou.assign(size_t(mmix), DevBuf());
+    resize(st->gxin, 0);
+    resize(st->gxou, 0);
+    resize(st->xin, size_t(mmix));
+    resize(st->xou, size_t(mmix));
     for (int k = 0; k < mmix; ++k)
       if (st->xin[size_t(k)].ensure(n8) || st->xou[size_t(k)].ensure(n8))
         return ctx->fail(RSG_ERR_HIP, "mix_init: history allocation failed");
     if (st->res.ensure(n8 * size_t(mmix)) || st->occ.ensure(size_t(mmix) * mmix * 8 + 64 * 8))
       return ctx->fail(RSG_ERR_HIP, "mix_init: residual allocation failed");
   } else {
-    st->gxin.assign(size_t(mmix), DevBuf());
-    st->gxou.assign(size_t(mmix), DevBuf());
+    resize(st->xin, 0);
+    resize(st->xou, 0);
+    resize(st->gxin, size_t(mmix));
+    resize(st->gxou, size_t(mmix));
     for (int k = 0; k < mmix; ++k)
       if (st->gxin[size_t(k)].ensure(2 * n8) || st->gxou[size_t(k)].ensure(2 * n8))
         return ctx->fail(RSG_ERR_HIP, "mix_init: history allocation failed");
@@ -955,7 +957,7 @@ void scf_release(rsg_ctx *ctx) {
   for (auto &b : st->xou) b.release();
   for (auto &b : st->gxin) b.release();
   for (auto &b : st->gxou) b.release();
-  for (DevBuf *b : {&st->res, &st->wq, &st->wq2, &st->gpart}) b->release();
+  for (DevBuf *b : {&st->res, &st->wq, &st->wq2, &st->gpart, &st->dfrac, &st->dkind, &st->dsp}) b->release();
   delete st;
   ctx->scf = nullptr;
 }

@@ -261,9 +261,12 @@ struct HostLayouts {
 
 template <typename T>
 int upload_layouts(rsg_ctx *ctx, NonlocalDev<T> &d, const HostLayouts &h) {
+  // stream-ordered copies (a plain hipMemcpy waits for every stream of the
+  // device), synchronised before the host vectors go out of scope
   auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
     if (b.ensure(bytes) != hipSuccess) return RSG_ERR_HIP;
-    if (bytes && hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return RSG_ERR_HIP;
+    if (bytes && hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+      return RSG_ERR_HIP;
     return RSG_OK;
   };
   std::vector<T> c(h.grp_coef.begin(), h.grp_coef.end()), s(h.scale.begin(), h.scale.end()),
@@ -285,6 +288,7 @@ int upload_layouts(rsg_ctx *ctx, NonlocalDev<T> &d, const HostLayouts &h) {
   rc |= up(d.sl_len, h.sl_len.data(), h.sl_len.size() * sizeof(int));
   rc |= up(d.sl_lma, h.sl_lma.data(), h.sl_lma.size() * sizeof(int));
   rc |= up(d.sl_uv, u.data(), u.size() * sizeof(T));
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = RSG_ERR_HIP;
   return rc ? ctx->fail(RSG_ERR_HIP, "upload of projector tables failed") : RSG_OK;
 }
 
@@ -547,9 +551,12 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
 namespace {
 
 template <typename T> int upload_analytic_t(rsg_ctx *ctx, AnalyticDev<T> &d, const HostAnalytic &h) {
+  // stream-ordered copies (a plain hipMemcpy waits for every stream of the
+  // device), synchronised before the host vectors go out of scope
   auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
     if (b.ensure(bytes) != hipSuccess) return RSG_ERR_HIP;
-    if (bytes && hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return RSG_ERR_HIP;
+    if (bytes && hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+      return RSG_ERR_HIP;
     return RSG_OK;
   };
   auto conv = [](const std::vector<double> &v) { return std::vector<T>(v.begin(), v.end()); };
@@ -573,6 +580,7 @@ template <typename T> int upload_analytic_t(rsg_ctx *ctx, AnalyticDev<T> &d, con
   rc |= up(d.tile_off, h.tile_off.data(), h.tile_off.size() * sizeof(int));
   rc |= up(d.tile_g, h.tile_g.data(), h.tile_g.size() * sizeof(int));
   d.ntile = int(h.tile_off.size()) - 1;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = RSG_ERR_HIP;
   return rc ? ctx->fail(RSG_ERR_HIP, "upload of analytic projector data failed") : RSG_OK;
 }
 
