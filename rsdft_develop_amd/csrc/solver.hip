@@ -76,7 +76,21 @@ __global__ __launch_bounds__(kThreads) void k_multidot(DotArgs<W> a, int64_t N, 
 #pragma unroll
   for (int q = 0; q < NP; ++q) acc[q] = 0.0;
   const int64_t off = int64_t(b) * N;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+  // four independent points per trip keep more loads in flight
+  int64_t i = i0 + threadIdx.x;
+  for (; i + 3 * kThreads < i1; i += 4 * kThreads) {
+    double part[4][NP];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+        part[u][q] = double(a.x[q][off + i + u * kThreads]) * double(a.y[q][off + i + u * kThreads]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int q = 0; q < NP; ++q) acc[q] += part[u][q];
+  }
+  for (; i < i1; i += kThreads) {
 #pragma unroll
     for (int q = 0; q < NP; ++q) acc[q] += double(a.x[q][off + i]) * double(a.y[q][off + i]);
   }
