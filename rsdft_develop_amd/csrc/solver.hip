@@ -334,7 +334,7 @@ __global__ void k_pc_dir(const W *__restrict__ r, W *__restrict__ p, const doubl
 // four in-plane neighbours from its neighbours' loads of the same plane
 // (L1/L2).  A p is never stored: the step pass recomputes it from p, and
 // the direction pass evaluates A p' for p' = r + beta p from r and p
-// (cgpc_hprsdft_module.f90:180-186 applies precond_cg_mat to the new
+// (cgpc_hprsdft_module.f90:193-197 applies precond_cg_mat to the new
 // direction the same way).  The new direction is written to the second
 // buffer, since other blocks still read the old one as neighbours; the
 // host swaps the two.  Per-band dot products: one partial per block,
