@@ -159,6 +159,7 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     ctx.build_gth_projectors(cell)
     ctx.autotune_hpsi(64, cell.nband, reps=2)
     ctx.autotune_hpsi(32, cell.nband, reps=2)
+    gram = ctx.autotune_gram(cell.nband)
     if params is not None:
         ctx.lpc_set_params(params)
     tag = f"scf Si{cell.natom} {'mixed' if mixed else 'fp64'} pc{pc_kind}"
@@ -171,6 +172,7 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     wall = time.perf_counter() - t0
     its = res.iterations
     out = dict(seconds=round(wall, 3), setup_s=round(res.setup, 3), iterations=len(its), converged=bool(res.converged),
+               gram_chunk=gram[1], gram_ms=round(gram[0], 3),
                etot=res.etot,
                fp32_iterations=sum(1 for r in its if r.precision == 32),
                per_phase_s={k: round(sum(r.times.get(k, 0.0) for r in its), 3)
@@ -215,6 +217,7 @@ def step_leg(nrep: int, steps: int):
         ctx.build_gth_projectors(cell)
         ctx.autotune_hpsi(64, mb, reps=1)
         ctx.autotune_hpsi(32, mb, reps=1)
+        ctx.autotune_gram(mb, reps=1)
         tag = f"step Si{cell.natom} {'mixed' if mixed else 'fp64'}"
         progress(f"{tag}: start")
         run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=0.0, max_iter=steps, mixed=mixed, imix=30,

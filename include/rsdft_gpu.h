@@ -145,6 +145,15 @@ int rsg_get_hpsi_shape(rsg_ctx *ctx, int precision, int nb, int *ry, int *ty,
 int rsg_autotune_hpsi(rsg_ctx *ctx, int precision, int nb, int reps,
                       float *best_ms);
 
+/* The Gram products U^T X (N x nb by N x nb, K = N) of Gram-Schmidt and
+ * subspace_diag: time one rocBLAS GEMM over all of K against K-splits into
+ * 8..512 chunks (one strided-batched GEMM plus a chunk sum), reps calls
+ * each on scratch bands, and keep the fastest for (N, nb) in this context.
+ * chunk receives the points per chunk (0 = one GEMM); best_ms and chunk
+ * may be NULL.  No reference counterpart (launch tuning). */
+int rsg_autotune_gram(rsg_ctx *ctx, int nb, int reps, float *best_ms,
+                      int64_t *chunk);
+
 /* Start bands on the device: columns n0 .. n0+nb-1 (0-based) of the band
  * store get uniform [0, 1) values, one counter-based draw per point and band
  * (SplitMix64 of seed + (b N + j + 1) * 0x9E3779B97F4A7C15, top 53 bits), so

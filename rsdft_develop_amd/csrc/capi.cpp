@@ -474,6 +474,12 @@ int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double
   return rsg::conjugate_gradient(ctx, ncg, pc_kind, mloop, esp, res);
 }
 
+int rsg_autotune_gram(rsg_ctx *ctx, int nb, int reps, float *best_ms, int64_t *chunk) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::autotune_gram(ctx, nb, reps, best_ms, chunk);
+}
+
 int rsg_precond_hprsdft_f64(rsg_ctx *ctx, int mloop, const double *g, double *pg, int64_t ld, int nb) {
   if (!ctx) return RSG_ERR_ARG;
   RSG_HIP(hipSetDevice(ctx->device));

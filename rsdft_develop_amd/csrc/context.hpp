@@ -5,6 +5,7 @@
 #include <rocblas/rocblas.h>
 
 #include <cstdint>
+#include <array>
 #include <string>
 #include <vector>
 
@@ -194,6 +195,9 @@ struct rsg_ctx {
   rsg::DevBuf dotpart;             // multidot partial sums
   rsg::DevBuf grampart;            // per-chunk Gram matrices of the K-split U^T X products
   rsg::DevBuf pcbuf;               // work vectors of the stand-alone preconditioner call
+  // K-split of the U^T X Gram products chosen by the autotuner, per
+  // (N, nb): points per chunk, 0 = one GEMM over all of K
+  std::vector<std::array<int64_t, 3>> gram_split;
   rsg::DevBuf dead;                // store target of padding lanes (64 doubles)
   rsg::ScfState *scf = nullptr;    // density/potential state of the SCF step
   rsg::LpcState lpc;               // learned CG preconditioner

@@ -950,14 +950,25 @@ __global__ void k_sum_batches(const double *__restrict__ part, int nbat, int cou
 // workgroup walks all of K when nb is small), so for nb <= 192 K is split
 // into chunks: one strided-batched GEMM into per-chunk partial matrices,
 // then a deterministic sum over the chunks.
-int gemm_tn_gram(rsg_ctx *ctx, int nb, int64_t N, double alpha, const double *A, const double *B, double *C) {
+// Points per K chunk for an (N, nb) product: the autotuned choice if there
+// is one, else at most 256 chunks of >= 1024 points for nb <= 192 and one
+// GEMM above; 0 = one GEMM.
+int64_t gram_chunk(const rsg_ctx *ctx, int nb, int64_t N) {
+  for (const auto &e : ctx->gram_split)
+    if (e[0] == N && e[1] == nb) return e[2];
+  if (nb > 192 || N < 4096) return 0;
+  return std::max<int64_t>(1024, (N + 255) / 256);
+}
+
+int gemm_tn_gram(rsg_ctx *ctx, int nb, int64_t N, double alpha, const double *A, const double *B, double *C,
+                 int64_t kc_force = -1) {
   const double zero = 0.0, one = 1.0;
-  if (nb > 192 || N < 4096) {
+  const int64_t kc = kc_force >= 0 ? kc_force : gram_chunk(ctx, nb, N);
+  if (kc <= 0 || kc >= N) {
     RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, int(N), &alpha, A,
                            int(N), B, int(N), &zero, C, nb));
     return RSG_OK;
   }
-  const int64_t kc = std::max<int64_t>(1024, (N + 255) / 256);  // at most 256 chunks of >= 1024 points
   const int nfull = int(N / kc);
   const int64_t krem = N - int64_t(nfull) * kc;
   const int nbat = nfull + (krem > 0 ? 1 : 0);
@@ -978,6 +989,60 @@ int gemm_tn_gram(rsg_ctx *ctx, int nb, int64_t N, double alpha, const double *A,
   return RSG_OK;
 }
 }  // namespace
+
+// Time one GEMM over all of K against K-splits into 8..512 chunks of at
+// least 512 points, for nb bands of the current grid (scratch bands, reps
+// calls each), and keep the fastest for (N, nb).  The products are the same
+// up to summation order.
+int autotune_gram(rsg_ctx *ctx, int nb, int reps, float *best_ms, int64_t *best_kc) {
+  if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "autotune_gram: grid not set");
+  if (nb < 1 || reps < 1) return ctx->fail(RSG_ERR_ARG, "autotune_gram: nb and reps must be >= 1");
+  const int64_t N = ctx->N;
+  double *A = nullptr, *C = nullptr;
+  if (hipMalloc(&A, size_t(N) * nb * 8) != hipSuccess) return ctx->fail(RSG_ERR_HIP, "autotune_gram: allocation failed");
+  if (hipMalloc(&C, size_t(nb) * nb * 8) != hipSuccess) {
+    (void)hipFree(A);
+    return ctx->fail(RSG_ERR_HIP, "autotune_gram: allocation failed");
+  }
+  (void)hipMemsetAsync(A, 0x3f, size_t(N) * nb * 8, ctx->stream);
+  std::vector<int64_t> cand{0};
+  for (int c = 8; c <= 512; c *= 2)
+    if (N / c >= 512) cand.push_back((N + c - 1) / c);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = RSG_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = RSG_ERR_HIP;
+  float tbest = 0.f;
+  int64_t kbest = 0;
+  for (size_t i = 0; i < cand.size() && !rc; ++i) {
+    rc = gemm_tn_gram(ctx, nb, N, 1.0, A, A, C, cand[i]);  // warm-up (and scratch sizing)
+    if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = RSG_ERR_HIP;
+    for (int r = 0; r < reps && !rc; ++r) rc = gemm_tn_gram(ctx, nb, N, 1.0, A, A, C, cand[i]);
+    float ms = 0.f;
+    if (!rc && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = RSG_ERR_HIP;
+    if (!rc && (i == 0 || ms < tbest)) {
+      tbest = ms;
+      kbest = cand[i];
+    }
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(A);
+  (void)hipFree(C);
+  if (rc) return rc == RSG_ERR_HIP ? ctx->fail(rc, "autotune_gram: timing failed") : rc;
+  bool found = false;
+  for (auto &e : ctx->gram_split)
+    if (e[0] == N && e[1] == nb) {
+      e[2] = kbest;
+      found = true;
+    }
+  if (!found) ctx->gram_split.push_back({N, int64_t(nb), kbest});
+  if (best_ms) *best_ms = tbest / reps;
+  if (best_kc) *best_kc = kbest;
+  return RSG_OK;
+}
 
 int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
   int rc = check_store(ctx, "gram_schmidt");

@@ -8,6 +8,7 @@ int subspace_diag(rsg_ctx *ctx, double *esp);
 int conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res);
 int conjugate_gradient_f32(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *res);
 int precond_hprsdft_dev(rsg_ctx *ctx, int mloop, const double *g, double *pg, int64_t ld, int nb);
+int autotune_gram(rsg_ctx *ctx, int nb, int reps, float *best_ms, int64_t *best_kc);
 int precond_hprsdft_dev(rsg_ctx *ctx, int mloop, const float *g, float *pg, int64_t ld, int nb);
 int axpby_f32(rsg_ctx *ctx, float *y, const float *x, float a, float b, int64_t n);
 int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double shift, double vweight);

@@ -83,6 +83,7 @@ def load_library():
         "rsg_set_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_precond_hprsdft_f64": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
         "rsg_lpc_set_hybrid_scale": (_c_int, [_vp, _c_dbl]),
+        "rsg_autotune_gram": (_c_int, [_vp, _c_int, _c_int, _P(ctypes.c_float), _P(_c_i64)]),
         "rsg_axpby_f32": (_c_int, [_vp, _vp, _vp, ctypes.c_float, ctypes.c_float, _c_i64]),
         "rsg_precond_hprsdft_f32": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
         "rsg_get_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int, _P(_c_int)]),
@@ -274,6 +275,13 @@ class Context:
         """pg = P g, the hprsdft preconditioner on device arrays (precond_hprsdft)."""
         fn = self.L.rsg_precond_hprsdft_f64 if np.dtype(dtype) == np.float64 else self.L.rsg_precond_hprsdft_f32
         self._check(fn(self.h, int(mloop), _vp(d_g), _vp(d_pg), int(ld), int(nb)))
+
+    def autotune_gram(self, nb: int, reps: int = 3) -> tuple:
+        """Pick the K-split of the nb-band Gram products on this grid; returns
+        (ms per product, points per chunk, 0 = one GEMM)."""
+        ms, kc = ctypes.c_float(), _c_i64()
+        self._check(self.L.rsg_autotune_gram(self.h, int(nb), int(reps), ctypes.byref(ms), ctypes.byref(kc)))
+        return float(ms.value), int(kc.value)
 
     def lpc_set_hybrid_scale(self, scale: float):
         """pc_kind 6: P g = scale * P_hprsdft g + network(g)."""

@@ -202,3 +202,27 @@ def test_precond_hprsdft_early_exits(setup):
     tiny = 1e-17 * g  # |r|^2 ~ 1e-26 per band
     np.testing.assert_array_equal(ham.precond(tiny, mloop=3), tiny)
     np.testing.assert_array_equal(_precond_gpu(ctx, tiny, N, 3, np.float64), tiny)
+
+
+@pytest.mark.parametrize("nb", [21, 40])
+def test_gram_split_choices_agree(orc, nb):
+    """Every K-split the Gram autotuner may pick gives the same orthonormal
+    bands (1e-12) as one GEMM over all of K; the tuned choice is kept."""
+    from rsdft_develop_amd import gpu
+
+    cell = systems.silicon_supercell(1, ngrid_per_cell=48, nband=nb, displacement=0.05, seed=nb)
+    ctx = gpu.Context()
+    ctx.set_grid(cell.ngrid, cell.aa, cell.md)
+    ctx.alloc_bands(nb)
+    u0 = systems.random_bands(cell.npoints, nb, seed=3)
+    ctx.upload_bands(u0)
+    ctx.gram_schmidt()
+    ref = ctx.download_bands()
+    ms, kc = ctx.autotune_gram(nb, reps=2)
+    print(f"case: gram autotune nb {nb}: chunk {kc}, {ms:.3f} ms", flush=True)
+    assert ms > 0 and (kc == 0 or 512 <= kc < cell.npoints)
+    ctx.upload_bands(u0)
+    ctx.gram_schmidt()
+    got = ctx.download_bands()
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    np.testing.assert_allclose(got, orc.Ham(cell).gram_schmidt(u0, "lusl"), atol=1e-10)
