@@ -182,6 +182,35 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     return out
 
 
+def scf_config0_leg():
+    """The Si8 configuration (36^3, 20 bands, EKBT 0.01 Ha, IMIX 30, device
+    start bands) solved to dE < 1e-6 Ha/atom twice: by the library on the
+    GPU and by the oracle's FP64 CPU restatement on all host cores, with the
+    same start bands.  Wall times of whole SCF runs and their energy
+    difference."""
+    from oracle import pyoracle, scf_oracle
+    from rsdft_develop_amd import gpu, scf, systems
+
+    cell = systems.silicon_supercell(1, nband=20, displacement=0.0, seed=0)
+    params = dict(ekbt=0.01, etol_per_atom=1e-6, max_iter=80, imix=30, start="uniform")
+    ctx = gpu.Context(0)
+    scf.SCF(cell, scf.ScfParams(**params), ctx=ctx).run()  # first use of this size, untimed
+    t0 = time.perf_counter()
+    g = scf.SCF(cell, scf.ScfParams(**params), ctx=ctx).run()
+    gpu_s = time.perf_counter() - t0
+    ctx.close()
+    cores = os.cpu_count() or 1
+    pyoracle.set_threads(cores)
+    t0 = time.perf_counter()
+    c = scf_oracle.run_scf(cell, **params)
+    cpu_s = time.perf_counter() - t0
+    return {"workload": "Si8 36^3, 20 bands, EKBT 0.01 Ha, IMIX 30, to dE < 1e-6 Ha/atom",
+            "gpu_seconds": round(gpu_s, 3), "gpu_iterations": g.niter,
+            "cpu_seconds": round(cpu_s, 3), "cpu_cores": cores, "cpu_kind": "port",
+            "cpu_iterations": len(c.get("history", [])),
+            "etot_diff_Ha_per_atom": abs(float(g.etot) - float(c["etot"])) / cell.natom}
+
+
 def scf_warmup(nrep: int):
     """One FP32 and one FP64 SCF iteration of the same cell, untimed, so that
     neither timed run pays the process's first use of the kernels and
@@ -334,6 +363,8 @@ def main() -> int:
     lpc_out = lpc_leg()
     progress("CPU baseline")
     cpu = None if args.no_cpu else cpu_leg(cell, vloc)
+    progress("Si8 SCF on the GPU and on the host")
+    scf0 = None if args.no_cpu else scf_config0_leg()
     scf_out = None
     if args.scf > 0:
         progress("scf warm-up")
@@ -380,6 +411,7 @@ def main() -> int:
                 "kernel_avg_us": round(r32["kernel_us"], 1), "kernel_gbs": round(r32["kernel_gbs"], 1),
                 "kernel_traffic": kernel_traffic("f32")},
         "cpu_baseline": cpu,
+        "scf_si8_gpu_vs_cpu": scf0,
         "scf": scf_out,
         "si512_step": step_out,
         "batch": batch_out,
