@@ -523,6 +523,13 @@ __global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W> A, PzGeom q, W c0, 
   const int64_t exr = (x + VP == q.n1 ? 0 : x + VP) + int64_t(y) * q.n1;
   const int64_t eyu = x + int64_t(y == q.n2 - 1 ? 0 : y + 1) * q.n1;
   const int64_t eyd = x + int64_t(y == 0 ? q.n2 - 1 : y - 1) * q.n1;
+  // the groups left and right of this one (periodic in x) are the groups
+  // of lanes sl_l and sr_l when those lie in the same wave: their combined
+  // centre values are taken by lane shuffle, and only the other lanes load
+  const int lane = int(threadIdx.x & 63);
+  const int sl_l = lane + (x == 0 ? hv - 1 : -1), sr_l = lane + (x + VP == q.n1 ? 1 - hv : 1);
+  const bool sl_in = sl_l >= 0 && sl_l < 64, sr_in = sr_l >= 0 && sr_l < 64;
+  const int sl_src = sl_in ? sl_l : lane, sr_src = sr_in ? sr_l : lane;
   const int z0 = zr * q.zlen, z1 = min(z0 + q.zlen, q.n3);
   const int64_t off = int64_t(b) * q.N;
   const W *gb = A.g + int64_t(b) * q.ld, *pb = A.p + off;
@@ -557,8 +564,6 @@ __global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W> A, PzGeom q, W c0, 
       const V cn = ldv(src, int64_t(zn) * q.n12 + e0);
       V dn{};
       if (OP == kPzDir) dn = ldv(pb, int64_t(zn) * q.n12 + e0);
-      const W sl = comb(src[o + exl], OP == kPzDir ? pb[o + exl] : W(0));
-      const W sr = comb(src[o + exr], OP == kPzDir ? pb[o + exr] : W(0));
       const V yu = ldv(src, o + eyu), yd = ldv(src, o + eyd);
       V mu{}, md{};
       if (OP == kPzDir) {
@@ -568,6 +573,10 @@ __global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W> A, PzGeom q, W c0, 
       W v[VP], av[VP];
 #pragma unroll
       for (int i = 0; i < VP; ++i) v[i] = comb(cz.v[i], dz.v[i]);
+      // every lane of the wave is at plane z here (z0, z1 are block-uniform)
+      W sl = __shfl(v[VP - 1], sl_src, 64), sr = __shfl(v[0], sr_src, 64);
+      if (!sl_in) sl = comb(src[o + exl], OP == kPzDir ? pb[o + exl] : W(0));
+      if (!sr_in) sr = comb(src[o + exr], OP == kPzDir ? pb[o + exr] : W(0));
 #pragma unroll
       for (int i = 0; i < VP; ++i) {
         const W xp = i + 1 < VP ? v[i + 1 < VP ? i + 1 : i] : sr;
