@@ -192,6 +192,37 @@ def test_precond_hprsdft_matches_oracle(setup, mloop):
                 assert err <= rel, (ld - N, b, err)
 
 
+@pytest.mark.parametrize("ngrid", [(2, 5, 7), (4, 6, 10), (12, 9, 8), (20, 3, 5), (8, 16, 6), (72, 36, 24)])
+def test_precond_hprsdft_small_and_ragged_grids(orc, ngrid):
+    """Rows of 1 to 36 x-point groups (2 points per group in FP64, 4 or 2 in
+    FP32): the periodic x-neighbour of a row end then sits in the same wave,
+    or a row straddles two waves.  The grids are not cubic, so h1, h2, h3
+    all differ on the cubic cell and a coefficient applied along the wrong
+    axis fails.  Device preconditioner against the oracle's, with a leading
+    dimension that admits the vector passes and an odd one (one point per
+    thread): FP64 1e-12, FP32 5e-5 of each band's largest value."""
+    import dataclasses
+
+    from rsdft_develop_amd import gpu
+
+    cell = dataclasses.replace(systems.silicon_supercell(1, nband=4), ngrid=ngrid)
+    ham = orc.Ham(cell, vloc=None, projectors=False)
+    ctx = gpu.Context()
+    ctx.set_grid(cell.ngrid, cell.aa, cell.md)
+    N = cell.npoints
+    g = np.random.default_rng(sum(ngrid)).standard_normal((N, 3))
+    g[:, 1] *= 1e-3
+    want = ham.precond(g, mloop=3)
+    ld4 = N + (-N) % 4
+    for ld in (ld4, ld4 + 1):
+        for dt, rel in ((np.float64, 1e-12), (np.float32, 5e-5)):
+            print(f"case: precond grid {ngrid} ld N+{ld - N} {np.dtype(dt).name}", flush=True)
+            got = _precond_gpu(ctx, g, ld, 3, dt)
+            for b in range(g.shape[1]):
+                err = np.abs(got[:, b] - want[:, b]).max() / np.abs(want[:, b]).max()
+                assert err <= rel, (ngrid, ld - N, b, err)
+
+
 def test_precond_hprsdft_early_exits(setup):
     """mloop 0 and residual norms all below 1e-24 both return P g = g."""
     cell, ham, ctx, _ = setup
