@@ -530,6 +530,12 @@ __global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W> A, PzGeom q, W c0, 
   const int sl_l = lane + (x == 0 ? hv - 1 : -1), sr_l = lane + (x + VP == q.n1 ? 1 - hv : 1);
   const bool sl_in = sl_l >= 0 && sl_l < 64, sr_in = sr_l >= 0 && sr_l < 64;
   const int sl_src = sl_in ? sl_l : lane, sr_src = sr_in ? sr_l : lane;
+  // the same for the rows above and below (periodic in y) when a row holds
+  // at most half a wave of groups, so that most lanes find them in-wave
+  const bool ysh = hv <= 32;
+  const int yu_l = lane + (y == q.n2 - 1 ? (1 - q.n2) * hv : hv), yd_l = lane + (y == 0 ? (q.n2 - 1) * hv : -hv);
+  const bool yu_in = yu_l >= 0 && yu_l < 64, yd_in = yd_l >= 0 && yd_l < 64;
+  const int yu_src = yu_in ? yu_l : lane, yd_src = yd_in ? yd_l : lane;
   const int z0 = zr * q.zlen, z1 = min(z0 + q.zlen, q.n3);
   const int64_t off = int64_t(b) * q.N;
   const W *gb = A.g + int64_t(b) * q.ld, *pb = A.p + off;
@@ -564,25 +570,37 @@ __global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W> A, PzGeom q, W c0, 
       const V cn = ldv(src, int64_t(zn) * q.n12 + e0);
       V dn{};
       if (OP == kPzDir) dn = ldv(pb, int64_t(zn) * q.n12 + e0);
-      const V yu = ldv(src, o + eyu), yd = ldv(src, o + eyd);
-      V mu{}, md{};
-      if (OP == kPzDir) {
-        mu = ldv(pb, o + eyu);
-        md = ldv(pb, o + eyd);
-      }
-      W v[VP], av[VP];
+      W v[VP], wu[VP], wd[VP], av[VP];
 #pragma unroll
       for (int i = 0; i < VP; ++i) v[i] = comb(cz.v[i], dz.v[i]);
+      auto ldc = [&](int64_t e, W *w) {  // combined input of the group at e
+        const V a = ldv(src, e);
+        V m{};
+        if (OP == kPzDir) m = ldv(pb, e);
+#pragma unroll
+        for (int i = 0; i < VP; ++i) w[i] = comb(a.v[i], m.v[i]);
+      };
       // every lane of the wave is at plane z here (z0, z1 are block-uniform)
       W sl = __shfl(v[VP - 1], sl_src, 64), sr = __shfl(v[0], sr_src, 64);
       if (!sl_in) sl = comb(src[o + exl], OP == kPzDir ? pb[o + exl] : W(0));
       if (!sr_in) sr = comb(src[o + exr], OP == kPzDir ? pb[o + exr] : W(0));
+      if (ysh) {  // uniform over the launch
+#pragma unroll
+        for (int i = 0; i < VP; ++i) {
+          wu[i] = __shfl(v[i], yu_src, 64);
+          wd[i] = __shfl(v[i], yd_src, 64);
+        }
+        if (!yu_in) ldc(o + eyu, wu);
+        if (!yd_in) ldc(o + eyd, wd);
+      } else {
+        ldc(o + eyu, wu);
+        ldc(o + eyd, wd);
+      }
 #pragma unroll
       for (int i = 0; i < VP; ++i) {
         const W xp = i + 1 < VP ? v[i + 1 < VP ? i + 1 : i] : sr;
         const W xm = i > 0 ? v[i > 0 ? i - 1 : i] : sl;
-        av[i] = c0 * v[i] + c1 * (xp + xm) + c2 * (comb(yu.v[i], mu.v[i]) + comb(yd.v[i], md.v[i])) +
-                c3 * (comb(cp.v[i], dp.v[i]) + comb(cm.v[i], dm.v[i]));
+        av[i] = c0 * v[i] + c1 * (xp + xm) + c2 * (wu[i] + wd[i]) + c3 * (comb(cp.v[i], dp.v[i]) + comb(cm.v[i], dm.v[i]));
       }
       const int64_t k = o + e0;
       if (act) {
