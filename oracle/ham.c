@@ -301,11 +301,26 @@ void orc_hpsi(const orc_ham *h, const double *psi, double *hpsi, int64_t ld,
   orc_op_nonlocal(h, psi, hpsi, ld, nb);
 }
 
-/* ext0/dotp.f90:1-27 */
+/* ext0/dotp.f90:1-27.  Fixed-order sum: partials over fixed 4096-point
+ * chunks, added in chunk order, so the result does not depend on the thread
+ * count or on the order in which threads finish (an OpenMP reduction's
+ * combining order is unspecified, and the CG's rounding-level branches then
+ * made repeated runs differ). */
 double orc_dot(const double *a, const double *b, int64_t n, double alpha) {
+  enum { kChunk = 4096 };
+  const int64_t nc = (n + kChunk - 1) / kChunk;
+  if (nc <= 0) return 0.0;
+  double *part = (double *)malloc(sizeof(double) * (size_t)nc);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < nc; ++c) {
+    const int64_t i1 = c * kChunk, i2 = i1 + kChunk < n ? i1 + kChunk : n;
+    double s = 0.0;
+    for (int64_t i = i1; i < i2; ++i) s += a[i] * b[i];
+    part[c] = s;
+  }
   double tmp = 0.0;
-#pragma omp parallel for reduction(+ : tmp) schedule(static)
-  for (int64_t i = 0; i < n; ++i) tmp += a[i] * b[i];
+  for (int64_t c = 0; c < nc; ++c) tmp += part[c];
+  free(part);
   return tmp * alpha;
 }
 

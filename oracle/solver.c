@@ -77,10 +77,7 @@ static void pc_mat(const orc_ham *h, const double *g, double *f) {
 }
 
 static double plain_dot(const double *a, const double *b, int64_t n) {
-  double s = 0.0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-  for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
-  return s;
+  return orc_dot(a, b, n, 1.0); /* fixed summation order */
 }
 
 /* precond_hprsdft: cgpc_hprsdft_module.f90:133-270 -- mloop CG steps on
