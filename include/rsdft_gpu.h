@@ -190,6 +190,12 @@ int rsg_subspace_diag(rsg_ctx *ctx, double *esp);
  * mb). */
 int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop,
                            double *esp, double *res);
+/* The SWGS keyword of the CG (read_cg, cg_module.f90:42-50): 0 (default)
+ * leaves the preconditioned gradients as they are; 1 and 2 project each
+ * one against the other bands of the store before the direction update,
+ * cggs (cggs_module.f90:14-127, called at cg_module.f90:249-251): 1 against
+ * the bands before it, 2 against all bands but itself. */
+int rsg_set_cg_swgs(rsg_ctx *ctx, int swgs);
 /* pc_kind 5 = the learned preconditioner declared below; pc_kind 6 = the
  * hybrid of both, P g = scale * P_hprsdft g + net(g)
  * (rsg_lpc_set_hybrid_scale), with the same descent check. */

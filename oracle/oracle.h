@@ -113,6 +113,11 @@ void orc_precond_hprsdft(const orc_ham *h, int mloop, const double *gk,
 int orc_conjugate_gradient(const orc_ham *h, double *u, int64_t ld, int mb,
                            int mbd, int ncg, int pc_kind, int mloop,
                            double *esp, double *res);
+/* The same with the SWGS keyword (cg_module.f90:42-50): 1 or 2 applies
+ * cggs (cggs_module.f90:14-127) to each preconditioned gradient. */
+int orc_conjugate_gradient_gs(const orc_ham *h, double *u, int64_t ld, int mb,
+                              int mbd, int ncg, int pc_kind, int mloop, int swgs,
+                              double *esp, double *res);
 
 /* d_esp_calc: esp_calc_module.f90:17-68 (dv = 1/|psi_1|^2 on first use) */
 void orc_esp_calc(const orc_ham *h, const double *u, int64_t ld, int mb,

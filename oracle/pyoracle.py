@@ -62,6 +62,7 @@ def lib():
         L.orc_subspace_diag.argtypes = [_vp, _dp, _i64, _int, _dpc]
         L.orc_precond_hprsdft.argtypes = [_vp, _int, _dp, _dp, _i64, _int]
         L.orc_conjugate_gradient.argtypes = [_vp, _dp, _i64, _int, _int, _int, _int, _int, _dpc, _dpc]
+        L.orc_conjugate_gradient_gs.argtypes = [_vp, _dp, _i64, _int, _int, _int, _int, _int, _int, _dpc, _dpc]
         L.orc_esp_calc.argtypes = [_vp, _dp, _i64, _int, _dpc]
         L.orc_dense_h.argtypes = [_vp, _dp]
         L.orc_set_threads.argtypes = [_int]
@@ -190,12 +191,13 @@ class Ham:
         lib().orc_precond_hprsdft(self.h, mloop, gk, pgk, gk.shape[0], gk.shape[1])
         return pgk
 
-    def conjugate_gradient(self, u, ncg=4, mbd=None, pc_kind=4, mloop=3):
+    def conjugate_gradient(self, u, ncg=4, mbd=None, pc_kind=4, mloop=3, swgs=0):
         u = np.array(u, dtype=np.float64, order="F", copy=True)
         mb = u.shape[1]
         esp = np.zeros(mb)
         res = np.zeros(mb)
-        rc = lib().orc_conjugate_gradient(self.h, u, u.shape[0], mb, mbd or mb, ncg, pc_kind, mloop, esp, res)
+        rc = lib().orc_conjugate_gradient_gs(self.h, u, u.shape[0], mb, mbd or mb, ncg, pc_kind, mloop, swgs,
+                                             esp, res)
         if rc:
             raise ValueError(f"oracle CG failed ({rc})")
         return u, esp, res

@@ -111,12 +111,14 @@ void orc_precond_hprsdft(const orc_ham *h, int mloop, const double *gk,
         pc_mat(h, pb, f);
         double pap = plain_dot(pb, f, n);
         rr0[b] = rr1[b];
-        double a = rr0[b] / pap;
+        /* a band at rest (zero residual, so pAp = 0) takes a zero step
+         * rather than 0/0; the reference has no such band in practice */
+        double a = pap != 0.0 ? rr0[b] / pap : 0.0;
         for (int64_t i = 0; i < n; ++i) x[i] += a * pb[i];
         if (it == mloop) continue;
         for (int64_t i = 0; i < n; ++i) rb[i] -= a * f[i];
         rr1[b] = plain_dot(rb, rb, n);
-        double bb = rr1[b] / rr0[b];
+        double bb = rr0[b] != 0.0 ? rr1[b] / rr0[b] : 0.0;
         for (int64_t i = 0; i < n; ++i) pb[i] = rb[i] + bb * pb[i];
       }
     }
@@ -128,10 +130,43 @@ void orc_precond_hprsdft(const orc_ham *h, int mloop, const double *gk,
   free(rr0);
 }
 
-/* d_conjugate_gradient_1: cg_module.f90:112-431 */
+/* cggs: cggs_module.f90:14-127 -- classical Gram-Schmidt of one
+ * preconditioned gradient g against the store bands u(:, 0..mb-1): swgs 1
+ * projects out bands 0..nt-1 (cggs_1 with ns = nt - 1, 1-based nt = this
+ * band's store index + 1), swgs 2 every band but nt (cggs_2); any other
+ * value does nothing.  All coefficients dV <u_m, g> come from the incoming
+ * g, then g -= sum_m u_m c_m, as the reference's two loops do. */
+static void cggs(const orc_ham *h, int swgs, const double *u, int64_t ld, int mb, int nt, double *g) {
+  if (swgs != 1 && swgs != 2) return;
+  const int64_t n = h->N;
+  const int hi = swgs == 1 ? nt : mb;
+  if (hi <= 0) return;
+  double *c = (double *)calloc((size_t)hi, sizeof(double));
+  for (int m = 0; m < hi; ++m) {
+    if (m == nt) continue;
+    const double *um = u + (int64_t)m * ld;
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += um[i] * g[i];
+    c[m] = s * h->dv;
+  }
+  for (int m = 0; m < hi; ++m) {
+    if (m == nt) continue;
+    const double *um = u + (int64_t)m * ld;
+    for (int64_t i = 0; i < n; ++i) g[i] -= um[i] * c[m];
+  }
+  free(c);
+}
+
 int orc_conjugate_gradient(const orc_ham *h, double *u, int64_t ld, int mb,
                            int mbd, int ncg, int pc_kind, int mloop,
                            double *esp, double *res) {
+  return orc_conjugate_gradient_gs(h, u, ld, mb, mbd, ncg, pc_kind, mloop, 0, esp, res);
+}
+
+/* d_conjugate_gradient_1: cg_module.f90:112-431 */
+int orc_conjugate_gradient_gs(const orc_ham *h, double *u, int64_t ld, int mb,
+                              int mbd, int ncg, int pc_kind, int mloop, int swgs,
+                              double *esp, double *res) {
   const int64_t n = h->N;
   const double dv = h->dv;
   if (ld != n) return 1; /* work arrays share the leading dimension */
@@ -171,13 +206,16 @@ int orc_conjugate_gradient(const orc_ham *h, double *u, int64_t ld, int mb,
       if (conv) break;
       if (icg == ncg + 1) break;
       if (pc_kind == 4) orc_precond_hprsdft(h, mloop, gk, pgk, n, nn);
+      for (int b = 0; b < nn; ++b) cggs(h, swgs, u, ld, mb, ns + b, pgk + (int64_t)b * n); /* :249-251 */
       for (int b = 0; b < nn; ++b)
         rb[b] = orc_dot(pgk + (int64_t)b * n, gk + (int64_t)b * n, n, dv);
       if (icg == 1) {
         memcpy(pk, pgk, sizeof(double) * n * nn);
       } else {
         for (int b = 0; b < nn; ++b) {
-          double bk = rb[b] / gkgk[b];
+          /* a zero previous <Pg, g> (vanished gradient) restarts the
+           * direction instead of forming 0/0 */
+          double bk = gkgk[b] != 0.0 ? rb[b] / gkgk[b] : 0.0;
           double *p = pk + (int64_t)b * n, *q = pgk + (int64_t)b * n;
           for (int64_t i = 0; i < n; ++i) p[i] = q[i] + bk * p[i];
         }
@@ -193,7 +231,14 @@ int orc_conjugate_gradient(const orc_ham *h, double *u, int64_t ld, int mb,
                w3 = orc_dot(p, p, n, dv), w4 = orc_dot(x, hx, n, dv),
                w5 = orc_dot(p, hx, n, dv), w6 = orc_dot(p, hp, n, dv);
         double bm[4] = {w1, w2, w2, w3}, um[4] = {w4, w5, w5, w6}, W[2];
-        orc_sygv2(um, bm, W); /* info ignored as in :316 */
+        /* info is ignored at :316; here B not positive definite (p zero or
+         * parallel to x) keeps the band: u = (1, 0), E unchanged */
+        if (orc_sygv2(um, bm, W) != 0 || !isfinite(um[0]) || !isfinite(um[2]) || !isfinite(W[0]) ||
+            !isfinite(W[1])) {
+          um[0] = um[2] = 1.0;
+          um[1] = um[3] = 0.0;
+          W[0] = W[1] = E[b];
+        }
         if (fabs(W[0] - E[b]) > 1e-1 && fabs(W[1] - E[b]) <= 1e-1) {
           um[0] = um[2];
           um[1] = um[3];
@@ -213,7 +258,7 @@ int orc_conjugate_gradient(const orc_ham *h, double *u, int64_t ld, int mb,
         rb[b] = orc_dot(g, g, n, dv);
       }
       for (int b = 0; b < nn; ++b) {
-        if (rb[b] / res[ns + b] > 1.0e8) {
+        if (!isfinite(rb[b]) || rb[b] / res[ns + b] > 1.0e8) {
           E[b] = E1[b];
           continue;
         }

@@ -45,7 +45,7 @@ int rsg_destroy(rsg_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   DevBuf *bufs[] = {&ctx->vloc64, &ctx->vloc32, &ctx->uvunk, &ctx->unk, &ctx->work,
                     &ctx->hsub, &ctx->esp_dev, &ctx->info, &ctx->cgbuf, &ctx->scal,
-                    &ctx->hostio, &ctx->dead};
+                    &ctx->hostio, &ctx->dead, &ctx->cgcoef};
   for (DevBuf *b : bufs) b->release();
   ctx->uvpart.release();
   ctx->cproj.release();
@@ -472,6 +472,13 @@ int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double
   if (!ctx || !esp || !res) return RSG_ERR_ARG;
   RSG_HIP(hipSetDevice(ctx->device));
   return rsg::conjugate_gradient(ctx, ncg, pc_kind, mloop, esp, res);
+}
+
+int rsg_set_cg_swgs(rsg_ctx *ctx, int swgs) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (swgs < 0 || swgs > 2) return ctx->fail(RSG_ERR_ARG, "set_cg_swgs: SWGS must be 0, 1 or 2");
+  ctx->cg_swgs = swgs;
+  return RSG_OK;
 }
 
 int rsg_autotune_gram(rsg_ctx *ctx, int nb, int reps, float *best_ms, int64_t *chunk) {

@@ -195,6 +195,8 @@ struct rsg_ctx {
   rsg::DevBuf dotpart;             // multidot partial sums
   rsg::DevBuf grampart;            // per-chunk Gram matrices of the K-split U^T X products
   rsg::DevBuf pcbuf;               // work vectors of the stand-alone preconditioner call
+  rsg::DevBuf cgcoef;              // cggs coefficients dV U^T Pg (mb x block)
+  int cg_swgs = 0;                 // SWGS keyword: 0 none, 1/2 cggs projections in CG
   // K-split of the U^T X Gram products chosen by the autotuner, per
   // (N, nb): points per chunk, 0 = one GEMM over all of K
   std::vector<std::array<int64_t, 3>> gram_split;

@@ -133,7 +133,9 @@ __global__ void k_cg_grad(const W *__restrict__ hx, const W *__restrict__ x, con
   g[k] = W(-2) * (hx[k] - e * x[k]);
 }
 
-// y = x (first step) or y = x + (rb/gkgk) y   (cg_module.f90:266-279)
+// y = x (first step) or y = x + (rb/gkgk) y   (cg_module.f90:266-279);
+// a zero previous <Pg, g> (a band whose gradient vanished) restarts the
+// direction instead of forming 0/0
 template <typename W>
 __global__ void k_cg_dir(const W *__restrict__ pg, W *__restrict__ pk, const double *__restrict__ rb,
                          const double *__restrict__ gkgk, int first, int64_t N) {
@@ -141,7 +143,8 @@ __global__ void k_cg_dir(const W *__restrict__ pg, W *__restrict__ pk, const dou
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int64_t k = int64_t(b) * N + i;
-  pk[k] = first ? pg[k] : pg[k] + W(rb[b] / gkgk[b]) * pk[k];
+  const double bk = gkgk[b] != 0.0 ? rb[b] / gkgk[b] : 0.0;
+  pk[k] = first ? pg[k] : pg[k] + W(bk) * pk[k];
 }
 
 // 2x2 generalised Ritz step per band (cg_module.f90:305-334): DSYGV on
@@ -187,6 +190,14 @@ __global__ void k_cg_ritz(const double *__restrict__ d6, int nb, double *__restr
     v1 = -v1;
     v2 = -v2;
   }
+  // B not positive definite (p parallel to x, or zero) leaves non-finite
+  // coefficients: keep the band as it is (u = (1, 0), E unchanged); the
+  // guard below then rejects the step
+  if (!(std::isfinite(v1) && std::isfinite(v2) && std::isfinite(W))) {
+    v1 = 1.0;
+    v2 = 0.0;
+    W = e;
+  }
   uc[2 * b] = v1;
   uc[2 * b + 1] = v2;
   E1[b] = e;
@@ -208,13 +219,16 @@ __global__ void k_cg_update(W *__restrict__ hx, const W *__restrict__ hp, const 
   g[k] = W(-2) * (h - ev * (u1 * x[k] + u2 * p[k]));
 }
 
-// divergence guard (cg_module.f90:358-364): reject when rb/res > 1e8
+// divergence guard (cg_module.f90:358-364): reject when rb/res > 1e8, and
+// also when the new gradient norm is not finite (the reference would carry
+// a NaN into the band); an exactly zero gradient before and after (0/0)
+// keeps the band, which the Ritz step left unchanged
 __global__ void k_cg_guard(const double *__restrict__ rb, const double *__restrict__ res,
                            double *__restrict__ E, const double *__restrict__ E1,
                            int *__restrict__ accept, int nb) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
-  const int ok = !(rb[b] / res[b] > 1.0e8);
+  const int ok = std::isfinite(rb[b]) && !(rb[b] / res[b] > 1.0e8);
   accept[b] = ok;
   if (!ok) E[b] = E1[b];
 }
@@ -246,6 +260,24 @@ template <typename W>
 __global__ void k_to_work(const double *__restrict__ x, W *__restrict__ x32, int64_t total) {
   const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k < total) x32[k] = W(x[k]);
+}
+
+// x64 = (double) x for an nn-band block
+template <typename W>
+__global__ void k_to_double(const W *__restrict__ x, double *__restrict__ x64, int64_t total) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < total) x64[k] = double(x[k]);
+}
+
+// cggs coefficient mask (cggs_module.f90:35-127): column j of C (mb x nn)
+// belongs to store band nt = n0 + j; swgs 1 keeps rows m < nt, swgs 2
+// every row but nt
+__global__ void k_cggs_mask(double *__restrict__ C, int mb, int nn, int n0, int swgs) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (m >= mb || j >= nn) return;
+  const int nt = n0 + j;
+  const bool keep = swgs == 1 ? m < nt : m != nt;
+  if (!keep) C[m + int64_t(mb) * j] = 0.0;
 }
 
 __global__ void k_scale_copy(const double *__restrict__ a, double s, double *__restrict__ b, int n) {
@@ -292,14 +324,18 @@ __global__ void k_pc_init(const W *__restrict__ g, const W *__restrict__ f, W *_
   p[k] = v;
 }
 
-// a = rr1/pAp, rr0 = rr1 (per band)
+// a = rr1/pAp, rr0 = rr1 (per band).  A band whose residual is exactly
+// zero (its direction too, so pAp = 0) takes a zero step instead of 0/0.
 __global__ void k_pc_alpha(const double *__restrict__ rr1, const double *__restrict__ pap,
                            double *__restrict__ rr0, double *__restrict__ a, int nb) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   rr0[b] = rr1[b];
-  a[b] = rr1[b] / pap[b];
+  a[b] = pap[b] != 0.0 ? rr1[b] / pap[b] : 0.0;
 }
+
+// rr1/rr0 with a zero previous residual (a band already at rest) taken as 0
+__device__ inline double pc_beta(double rr1, double rr0) { return rr0 != 0.0 ? rr1 / rr0 : 0.0; }
 
 // x += a p ; optionally r -= a f
 template <typename W>
@@ -322,7 +358,7 @@ __global__ void k_pc_dir(const W *__restrict__ r, W *__restrict__ p, const doubl
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int64_t k = int64_t(b) * N + i;
-  p[k] = r[k] + W(rr1[b] / rr0[b]) * p[k];
+  p[k] = r[k] + W(pc_beta(rr1[b], rr0[b])) * p[k];
 }
 
 // ---- hprsdft preconditioner, z-streaming passes ----
@@ -408,7 +444,7 @@ __global__ __launch_bounds__(kThreads) void k_pcz(PzArgs<W> A, PzGeom q, W c0, W
   const W *gb = A.g + int64_t(b) * q.ld, *pb = A.p + off;
   W *xb = A.x + int64_t(b) * q.ld, *rb = A.r + off;
   const W al = (kStep || OP == kPzStepL || OP == kPzStepFL) ? W(A.a[b]) : W(0);
-  const W beta = OP == kPzDir ? W(A.rr1[b] / A.rr0[b]) : W(0);
+  const W beta = OP == kPzDir ? W(pc_beta(A.rr1[b], A.rr0[b])) : W(0);
   const W *src = OP == kPzFirst ? gb : (OP == kPzDir ? rb : pb);  // stencil input (r for the direction pass)
   auto wrapz = [&](int z) { return ((z % q.n3) + q.n3) % q.n3; };  // any n3 >= 1
   double acc = 0.0;
@@ -541,7 +577,7 @@ __global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W> A, PzGeom q, W c0, 
   const W *gb = A.g + int64_t(b) * q.ld, *pb = A.p + off;
   W *xb = A.x + int64_t(b) * q.ld, *rb = A.r + off;
   const W al = (kStep || OP == kPzStepL || OP == kPzStepFL) ? W(A.a[b]) : W(0);
-  const W beta = OP == kPzDir ? W(A.rr1[b] / A.rr0[b]) : W(0);
+  const W beta = OP == kPzDir ? W(pc_beta(A.rr1[b], A.rr0[b])) : W(0);
   const W *src = OP == kPzFirst ? gb : (OP == kPzDir ? rb : pb);
   auto ldv = [](const W *base, int64_t e) { return *reinterpret_cast<const V *>(base + e); };
   auto stv = [](W *base, int64_t e, const W *w) {
@@ -1157,6 +1193,41 @@ int esp_calc(rsg_ctx *ctx, double *esp) {
 
 namespace {
 
+// cggs (cggs_module.f90:14-127) on the preconditioned gradients of store
+// bands n0 .. n0+nn-1: C = dV U^T Pg over the whole store, rows masked per
+// column by the SWGS rule, Pg <- Pg - U C (classical Gram-Schmidt: every
+// coefficient from the incoming Pg, as the reference's two loops).  FP32
+// blocks go through an FP64 copy in tmp (N x nn doubles).
+template <typename W>
+int cggs_block(rsg_ctx *ctx, int swgs, int n0, int nn, W *pg, double *tmp) {
+  const int mb = ctx->mb;
+  const int64_t N = ctx->N;
+  if (ctx->cgcoef.ensure(size_t(mb) * nn * sizeof(double)) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "conjugate_gradient: cggs allocation failed");
+  double *C = ctx->cgcoef.as<double>();
+  const double *U = ctx->unk.as<double>();
+  const int64_t tot = N * nn;
+  const dim3 g1(unsigned((tot + kThreads - 1) / kThreads)), b1(kThreads);
+  double *P = nullptr;
+  if constexpr (sizeof(W) == 8) {
+    P = reinterpret_cast<double *>(pg);
+    (void)tmp;
+  } else {
+    P = tmp;
+    hipLaunchKernelGGL((k_to_double<W>), g1, b1, 0, ctx->stream, pg, P, tot);
+  }
+  const double zero = 0.0, one = 1.0, mone = -1.0, dv = ctx->dv;
+  RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, mb, nn, int(N), &dv, U,
+                         int(N), P, int(N), &zero, C, mb));
+  hipLaunchKernelGGL(k_cggs_mask, dim3((mb + 255) / 256, nn), dim3(256), 0, ctx->stream, C, mb, nn, n0, swgs);
+  RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, int(N), nn, mb, &mone, U,
+                         int(N), C, mb, &one, P, int(N)));
+  if constexpr (sizeof(W) != 8)
+    hipLaunchKernelGGL((k_to_work<W>), g1, b1, 0, ctx->stream, P, pg, tot);
+  RSG_HIP(hipGetLastError());
+  return RSG_OK;
+}
+
 // d_conjugate_gradient_1 (cg_module.f90:112-431) with the band block MB_d
 // as large as the CG work arrays allow.  W = double is the plain FP64
 // method on the store itself.  W = float keeps every CG and preconditioner
@@ -1256,6 +1327,17 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
           ++ctx->lpc.fallbacks;
           if ((rc = precond_hprsdft_fused<W>(ctx, mloop, gk, pgk, nn, pr, pp, pf, s, dv, s.rb))) return rc;
         }
+      }
+      if (ctx->cg_swgs == 1 || ctx->cg_swgs == 2) {
+        // orthogonalisation of Pg against the other bands (:249-251), then
+        // rb = dV <Pg, g> of the projected Pg (:255-257); the
+        // preconditioner's r and p vectors are free here
+        // (the FP64 copy of an FP32 block needs 2 N nn floats, 8-byte
+        // aligned: r, p and f of the preconditioner hold 3 N mbd floats)
+        double *tmp = reinterpret_cast<double *>((reinterpret_cast<uintptr_t>(pr) + 7) & ~uintptr_t(7));
+        if ((rc = cggs_block<W>(ctx, ctx->cg_swgs, ns, nn, pgk, tmp))) return rc;
+        const W *xs[1] = {pgk}, *ys[1] = {gk};
+        if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
       }
       hipLaunchKernelGGL((k_cg_dir<W>), gN, blkd, 0, ctx->stream, pgk, pk, s.rb, s.gkgk, icg == 1 ? 1 : 0, N);
       RSG_HIP(hipMemcpyAsync(s.gkgk, s.rb, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));

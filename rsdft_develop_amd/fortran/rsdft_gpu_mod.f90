@@ -5,8 +5,9 @@
 !   gram_schmidt(n0, n1, k, s)                   gram_schmidt/gram_schmidt_module.f90:35-79
 !   subspace_diag(k, s, unk, esp)                subspace_diag/subspace_diag_module.f90:17-19
 !   conjugate_gradient(n1, n2, k, s, unk, esp, res)
-!                                                conjugate_gradient/cg_module.f90:53-61
-!   esp_calc(k, s, unk, esp)                     esp_calc_module.f90:17-68
+!     unk(n1:,:,:,:), esp(:,:,:), res(:,:,:)    conjugate_gradient/cg_module.f90:53-61
+!     (also unk(:,:), esp(:), res(:) of one k, s slice)
+!   esp_calc(wf(:,n_in:), e(n_in:), n_in, k, s)  esp_calc_module.f90:17-68
 ! and call the C ABI of librsdft_gpu (include/rsdft_gpu.h) through
 ! ISO_C_BINDING.  Gamma point, one spin: k and s must be 1.
 !
@@ -209,7 +210,7 @@ module rsdft_gpu
     module procedure subspace_diag_gpu
   end interface
   interface conjugate_gradient
-    module procedure conjugate_gradient_gpu
+    module procedure conjugate_gradient_gpu, conjugate_gradient_gpu2
   end interface
   interface esp_calc
     module procedure esp_calc_gpu
@@ -377,26 +378,56 @@ contains
     call to_host(unk)
   end subroutine subspace_diag_gpu
 
-  ! unk(n1:n2, :) in the reference is the local grid slice; on one rank it
-  ! is the whole grid, so n1 = 1 and n2 = ML.
+  ! conjugate_gradient(n1, n2, k, s, unk, esp, res) with the reference's
+  ! whole arrays unk(n1:, MB, k, s), esp(MB, k, s), res(MB, k, s)
+  ! (cg_module.f90:53-61, called as conjugate_gradient(ML_0,ML_1,k,s,unk,
+  ! esp,res) at scf_module.f90:254).  n1:n2 is the rank's grid slice; on one
+  ! rank it is the whole grid.
   subroutine conjugate_gradient_gpu(n1, n2, k, s, unk, esp, res)
+    integer, intent(in) :: n1, n2, k, s
+    real(8), intent(inout) :: unk(n1:, :, :, :)
+    real(8), intent(inout) :: esp(:, :, :), res(:, :, :)
+    call gamma_only(k, s, "conjugate_gradient")
+    if (n2 - n1 + 1 /= size(unk, 1)) error stop "conjugate_gradient: one rank owns the whole grid"
+    if (size(esp, 1) /= size(unk, 2) .or. size(res, 1) /= size(unk, 2)) &
+      error stop "conjugate_gradient: esp and res need one entry per band"
+    call to_device(unk(:, :, k, s))
+    call check(rsg_conjugate_gradient(ctx, ncg_, pc_kind_, mloop_, esp(:, k, s), res(:, k, s)), &
+               "conjugate_gradient")
+    call to_host(unk(:, :, k, s))
+  end subroutine conjugate_gradient_gpu
+
+  ! The same on one (k, s) slice: unk(ML, MB), esp(MB), res(MB).
+  subroutine conjugate_gradient_gpu2(n1, n2, k, s, unk, esp, res)
     integer, intent(in) :: n1, n2, k, s
     real(8), intent(inout) :: unk(:, :)
     real(8), intent(inout) :: esp(:), res(:)
     call gamma_only(k, s, "conjugate_gradient")
-    if (n1 /= 1 .or. n2 /= size(unk, 1)) error stop "conjugate_gradient: one rank owns the whole grid"
+    if (n2 - n1 + 1 /= size(unk, 1)) error stop "conjugate_gradient: one rank owns the whole grid"
     call to_device(unk)
     call check(rsg_conjugate_gradient(ctx, ncg_, pc_kind_, mloop_, esp, res), "conjugate_gradient")
     call to_host(unk)
-  end subroutine conjugate_gradient_gpu
+  end subroutine conjugate_gradient_gpu2
 
-  subroutine esp_calc_gpu(k, s, unk, esp)
-    integer, intent(in) :: k, s
-    real(8), intent(in) :: unk(:, :)
-    real(8), intent(inout) :: esp(:)
+  ! esp_calc(wf, e, n_in, k, s) with wf(:, n_in:) and e(n_in:) the bands
+  ! n_in .. n_in+size(e)-1 of the store (d_esp_calc, esp_calc_module.f90:
+  ! 17-68, called as esp_calc(unk(:,MB_0:MB_1,k,s), esp(MB_0:MB_1,k,s),
+  ! MB_0,k,s) at scf_module.f90:268).  e_n = dV <wf_n|H wf_n>.
+  subroutine esp_calc_gpu(wf, e, n_in, k, s)
+    integer, intent(in) :: n_in, k, s
+    real(8), intent(in) :: wf(:, n_in:)
+    real(8), intent(inout) :: e(n_in:)
+    real(8), allocatable :: all_e(:)
+    integer :: nn
     call gamma_only(k, s, "esp_calc")
-    call to_device(unk)
-    call check(rsg_esp_calc(ctx, esp), "esp_calc")
+    nn = size(e)
+    if (size(wf, 2) /= nn) error stop "esp_calc: wf and e hold different band counts"
+    if (n_in < 1 .or. n_in + nn - 1 > mb_store) error stop "esp_calc: bands outside the store"
+    if (.not. resident) &
+      call check(rsg_upload_bands(ctx, wf, int(size(wf, 1), c_int64_t), n_in - 1, nn), "esp_calc")
+    allocate (all_e(mb_store))
+    call check(rsg_esp_calc(ctx, all_e), "esp_calc")
+    e(n_in:n_in + nn - 1) = all_e(n_in:n_in + nn - 1)
   end subroutine esp_calc_gpu
 
   ! --- the SCF half on the device (density, Hartree, XC, mixing, energies)

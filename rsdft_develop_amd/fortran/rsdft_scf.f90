@@ -22,7 +22,11 @@ program rsdft_scf
   integer :: nrep, npc, nb, max_iter, natom, n, it, i, j, k, a, niter
   real(8) :: ekbt, etol, aa(3, 3), eew, nel, eh_in, exc_in, eh, exc, eloc, eion, eloc_in, eion_in
   real(8) :: eeig, etot, eharris, eprev, de, fene
-  real(8), allocatable :: frac(:, :), unk(:, :), esp(:), res(:), occ(:), u1(:), u2(:), hunk(:, :)
+  real(8), allocatable :: frac(:, :), occ(:), u1(:), u2(:), hunk(:, :)
+  ! band storage as wf_module keeps it: unk(ML_0:ML_1, MB, k, s),
+  ! esp(MB, k, s), res(MB, k, s); Gamma point, one spin
+  real(8), allocatable :: unk(:, :, :, :), esp(:, :, :), res(:, :, :)
+  integer :: ML_0, ML_1, MB_0, MB_1, Nband, kk, ss
   real(8) :: dv, hmax
   integer, allocatable :: kind(:), seed(:)
   integer :: norb(1), lo(4, 1), no(4, 1)
@@ -75,7 +79,8 @@ program rsdft_scf
   call rsdft_gpu_update_vloc()
 
   ! N(0,1) starting bands (Box-Muller on the intrinsic generator, fixed seed)
-  allocate (unk(n**3, nb), esp(nb), res(nb), occ(nb), u1(n**3), u2(n**3))
+  ML_0 = 1; ML_1 = n**3; MB_0 = 1; MB_1 = nb; Nband = nb; kk = 1; ss = 1
+  allocate (unk(ML_0:ML_1, nb, 1, 1), esp(nb, 1, 1), res(nb, 1, 1), occ(nb), u1(n**3), u2(n**3))
   call random_seed(size=i)
   allocate (seed(i))
   seed = 20240607
@@ -83,11 +88,11 @@ program rsdft_scf
   do k = 1, nb
     call random_number(u1)
     call random_number(u2)
-    unk(:, k) = sqrt(-2d0 * log(1d0 - u1)) * cos(6.283185307179586d0 * u2)
+    unk(:, k, 1, 1) = sqrt(-2d0 * log(1d0 - u1)) * cos(6.283185307179586d0 * u2)
   end do
-  call rsdft_gpu_upload_unk(unk)
+  call rsdft_gpu_upload_unk(unk(:, :, 1, 1))
   call rsdft_gpu_set_resident(.true.)
-  call gram_schmidt(1, nb, 1, 1)
+  call gram_schmidt(1, Nband, kk, ss)
   call rsdft_gpu_mix_init(mmix, beta)
 
   conv = .false.
@@ -95,12 +100,13 @@ program rsdft_scf
   niter = 0
   do it = 1, max_iter
     niter = it
-    call subspace_diag(1, 1, unk, esp)
-    call conjugate_gradient(1, n**3, 1, 1, unk, esp, res)
-    call gram_schmidt(1, nb, 1, 1)
-    call esp_calc(1, 1, unk, esp)
-    call fermi(esp, nel, ekbt, occ)
-    eeig = sum(occ * esp)
+    ! the reference's own call lines (scf_module.f90:231, 254, 258, 268)
+    call subspace_diag( kk, ss, unk(:,:,kk,ss), esp(:,kk,ss) )
+    call conjugate_gradient(ML_0,ML_1,kk,ss,unk,esp,res)
+    call gram_schmidt(1,Nband,kk,ss)
+    call esp_calc( unk(:,MB_0:MB_1,kk,ss), esp(MB_0:MB_1,kk,ss), MB_0,kk,ss )
+    call fermi(esp(:, 1, 1), nel, ekbt, occ)
+    eeig = sum(occ * esp(:, 1, 1))
     call rsdft_gpu_density_integrals(RSG_FIELD_RHO, eloc_in, eion_in)
     eharris = eeig - eloc_in + eh_in + exc_in + eion_in + eew
     call rsdft_gpu_calc_density(occ)
@@ -121,17 +127,17 @@ program rsdft_scf
     call rsdft_gpu_hartree_xc(RSG_FIELD_RHO, eh_in, exc_in)
     call rsdft_gpu_update_vloc()
   end do
-  call rsdft_gpu_download_unk(unk)
+  call rsdft_gpu_download_unk(unk(:, :, 1, 1))
   allocate (hunk(n**3, nb))
-  call hamiltonian(unk, hunk, 1, 1, 1)
+  call hamiltonian(unk(:, :, 1, 1), hunk, 1, 1, 1)
   dv = rsdft_gpu_dv()
   hmax = 0d0
   do k = 1, nb
-    hmax = max(hmax, abs(dv * dot_product(unk(:, k), hunk(:, k)) - esp(k)))
+    hmax = max(hmax, abs(dv * dot_product(unk(:, k, 1, 1), hunk(:, k)) - esp(k, 1, 1)))
   end do
-  call sort(esp)
+  call sort(esp(:, 1, 1))
   write (*, '("ETOT ",es25.16,"  FENE ",es25.16,"  NITER ",i0,"  CONVERGED ",l1)') etot, fene, niter, conv
-  write (*, '("ESP",*(1x,es25.16))') esp
+  write (*, '("ESP",*(1x,es25.16))') esp(:, 1, 1)
   write (*, '("HCHECK ",es12.4)') hmax
   call rsdft_gpu_finalize()
 

@@ -83,6 +83,7 @@ def load_library():
         "rsg_set_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_precond_hprsdft_f64": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
         "rsg_lpc_set_hybrid_scale": (_c_int, [_vp, _c_dbl]),
+        "rsg_set_cg_swgs": (_c_int, [_vp, _c_int]),
         "rsg_autotune_gram": (_c_int, [_vp, _c_int, _c_int, _P(ctypes.c_float), _P(_c_i64)]),
         "rsg_axpby_f32": (_c_int, [_vp, _vp, _vp, ctypes.c_float, ctypes.c_float, _c_i64]),
         "rsg_precond_hprsdft_f32": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
@@ -286,6 +287,11 @@ class Context:
     def lpc_set_hybrid_scale(self, scale: float):
         """pc_kind 6: P g = scale * P_hprsdft g + network(g)."""
         self._check(self.L.rsg_lpc_set_hybrid_scale(self.h, float(scale)))
+
+    def set_cg_swgs(self, swgs: int):
+        """SWGS keyword of the CG: 1/2 project each preconditioned gradient
+        against the bands before it / all other bands (cggs)."""
+        self._check(self.L.rsg_set_cg_swgs(self.h, int(swgs)))
 
     def axpby_f32(self, d_y: int, d_x: int, a: float, b: float, n: int):
         """y = a y + b x on n device floats (context stream)."""

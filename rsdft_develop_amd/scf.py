@@ -33,7 +33,8 @@ class ScfParams:
     seed: int = 0           # seed of the initial bands
     start: str = "normal"   # initial bands: "normal" N(0,1) from the host, "uniform" [0, 1) drawn on the device
                             # (random_initial_wf, wf_module.f90:166-191)
-    mixed: bool = False     # FP32 CG/hpsi while dE/atom >= fp64_below, FP64 after
+    mixed: bool = False     # FP32 CG/hpsi while dE/atom >= fp64_below, FP64 after; the stop test
+                            # needs dE/atom < etol between two FP64 iterations
     fp64_below: float = 0.0  # 0: twice etol_per_atom (FP32 energies follow FP64 to well below 1e-6 Ha/atom)
     fp32_stall: int = 3      # ... or after this many FP32 iterations without a new lowest dE/atom
 
@@ -126,7 +127,7 @@ class SCF:
             c.set_field(gpu.FIELD_RHO, rho)
         self.eh_in, self.exc_in = c.hartree_xc(gpu.FIELD_RHO)
         c.update_vloc()
-        if self.p.pc_kind == 5:
+        if self.p.pc_kind in (5, 6):
             c.lpc_set_potential()  # the network's potential input follows Vloc
         c.alloc_bands(cell.nband)
         if bands is None and self.p.start == "uniform":
@@ -217,7 +218,7 @@ class SCF:
         c.normalize_density()
         self.eh_in, self.exc_in = c.hartree_xc(gpu.FIELD_RHO)
         c.update_vloc()
-        if self.p.pc_kind == 5:
+        if self.p.pc_kind in (5, 6):
             c.lpc_set_potential()
         return time.perf_counter() - t0
 
@@ -229,6 +230,7 @@ class SCF:
         prec = 32 if self.p.mixed else 64
         switch_below = self.p.fp64_below if self.p.fp64_below > 0 else 2.0 * self.p.etol_per_atom
         best, stall = np.inf, 0
+        prev_prec = None
         for it in range(1, self.p.max_iter + 1):
             rec = self.step(it, e_prev, prec)
             # FP32 is never trusted for the stop test: switch first, also
@@ -240,7 +242,10 @@ class SCF:
                     stall += 1
                 if rec.de_per_atom < switch_below or stall >= self.p.fp32_stall:
                     prec = 64
-            conv = rec.precision == 64 and rec.de_per_atom < self.p.etol_per_atom
+            # the stop test compares two FP64 energies: the first FP64
+            # iteration's dE is measured against an FP32 one
+            conv = rec.precision == 64 and prev_prec == 64 and rec.de_per_atom < self.p.etol_per_atom
+            prev_prec = rec.precision
             if not conv and it < self.p.max_iter:
                 rec.times["pot"] = self.update_potential()
             its.append(rec)

@@ -46,6 +46,7 @@ class GTHSpecies:
     rloc: float
     cloc: tuple
     projectors: tuple  # ((l, n, r_l, h), ...)
+    znuc: float = 0.0  # nuclear charge (Znuc of the file's first line)
 
 
 # Silicon GTH (LDA) parameters as given for the benchmark cells.
@@ -55,6 +56,7 @@ SI_GTH = GTHSpecies(
     rloc=0.44,
     cloc=(-7.336103, 0.0, 0.0, 0.0),
     projectors=((0, 1, 0.422738, 5.906928), (1, 1, 0.484278, 2.727013)),
+    znuc=14.0,
 )
 
 
@@ -121,12 +123,14 @@ def silicon_supercell(
     seed: int = 0,
     scale: float = 1.0,
     ngrid: int | None = None,
+    species: GTHSpecies | None = None,
 ) -> Cell:
     """``nrep^3`` supercell of the 8-atom diamond Si cell.
 
     ``displacement`` is the standard deviation (bohr) of i.i.d. Gaussian atom
     displacements drawn from ``seed``.  ``nband`` defaults to 2.5 bands per
-    atom (20 for Si8, 160 for Si64, 480 for Si216).
+    atom (20 for Si8, 160 for Si64, 480 for Si216).  ``species`` replaces
+    the built-in Si GTH parameters (e.g. ``pseudo.read_ps_gth(path)``).
     """
     a = BOHR_PER_SI8 * scale * nrep
     aa = np.eye(3) * a
@@ -147,7 +151,7 @@ def silicon_supercell(
         aa=aa,
         frac=frac,
         kind=np.zeros(natom, dtype=np.int32),
-        species=(SI_GTH,),
+        species=(species or SI_GTH,),
         ngrid=(n, n, n),
         nband=nband,
     )

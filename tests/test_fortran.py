@@ -34,6 +34,9 @@ def test_bind_c_names_are_declared_in_header():
     assert not missing, missing
 
 
+FLANG = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
+
+
 def test_generic_names_keep_reference_interfaces():
     src = open(MOD).read()
     for generic in ("hamiltonian", "gram_schmidt", "subspace_diag", "conjugate_gradient", "esp_calc"):
@@ -41,6 +44,29 @@ def test_generic_names_keep_reference_interfaces():
     # d_hamiltonian1(tpsi, htpsi, n, k, s): hamiltonian_module.f90:118-122
     assert "subroutine d_hamiltonian1(tpsi, htpsi, n, k, s)" in src
     assert "subroutine conjugate_gradient_gpu(n1, n2, k, s, unk, esp, res)" in src
+    assert "subroutine esp_calc_gpu(wf, e, n_in, k, s)" in src
+
+
+@pytest.mark.skipif(FLANG is None, reason="AMD flang not installed")
+def test_reference_call_lines_compile_against_the_module(tmp_path):
+    """The reference's call lines for the five generic names, with its band
+    arrays (4-D unk, 3-D esp/res), compile against `use rsdft_gpu`; a call
+    with the esp_calc argument order of the old shim does not."""
+    moddir = tmp_path / "mod"
+    moddir.mkdir()
+    r = subprocess.run([FLANG, "-c", "-module-dir", str(moddir), "-o", str(tmp_path / "m.o"), MOD],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    src = os.path.join(ROOT, "tests", "fortran_io", "reference_calls.f90")
+    r = subprocess.run([FLANG, "-c", "-I", str(moddir), "-o", str(tmp_path / "c.o"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    bad = tmp_path / "bad.f90"
+    bad.write_text("subroutine bad(unk, esp)\n  use rsdft_gpu\n  real(8) :: unk(:, :), esp(:)\n"
+                   "  call esp_calc(1, 1, unk, esp)\nend subroutine bad\n")
+    r = subprocess.run([FLANG, "-c", "-I", str(moddir), "-o", str(tmp_path / "b.o"), str(bad)],
+                       capture_output=True, text=True)
+    assert r.returncode != 0
 
 
 @pytest.mark.skipif(shutil.which("amdflang") is None and not os.path.exists("/opt/rocm/bin/amdflang"),

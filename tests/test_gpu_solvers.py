@@ -257,3 +257,61 @@ def test_gram_split_choices_agree(orc, nb):
     got = ctx.download_bands()
     assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
     np.testing.assert_allclose(got, orc.Ham(cell).gram_schmidt(u0, "lusl"), atol=1e-10)
+
+
+@pytest.mark.parametrize("swgs", [1, 2])
+def test_conjugate_gradient_cggs_matches_oracle(setup, swgs):
+    """SWGS 1/2 (cggs, cggs_module.f90:14-127) through the C ABI against the
+    oracle, FP64 and the FP32-storage CG, plus the one-step invariant that
+    pins the projection (new band j orthogonal to the old bands it was
+    projected against)."""
+    cell, ham, ctx, u0 = setup
+    q = ham.gram_schmidt(u0, "lusl")
+    u1, _ = ham.subspace_diag(q)
+    try:
+        ctx.set_cg_swgs(swgs)
+        _load(ctx, u1)
+        esp, res = ctx.conjugate_gradient(ncg=4, pc_kind=4, mloop=3)
+        u_want, esp_want, res_want = ham.conjugate_gradient(u1, ncg=4, pc_kind=4, swgs=swgs)
+        np.testing.assert_allclose(esp, esp_want, atol=1e-9)
+        np.testing.assert_allclose(res, res_want, rtol=1e-7, atol=1e-12)
+        _load(ctx, u1)
+        esp32, _ = ctx.conjugate_gradient(ncg=4, pc_kind=4, mloop=3, precision=32)
+        np.testing.assert_allclose(esp32, esp_want, atol=2e-5)
+        _load(ctx, u1)
+        ctx.conjugate_gradient(ncg=1, pc_kind=4, mloop=3)
+        s = ctx.download_bands().T @ u1 * ham.dv  # s[j, i] = dV <new j, old i>
+        mask = np.array([[(i < j) if swgs == 1 else (i != j) for i in range(cell.nband)]
+                         for j in range(cell.nband)])
+        assert np.abs(s[mask]).max() < 1e-12
+    finally:
+        ctx.set_cg_swgs(0)
+
+
+def test_conjugate_gradient_keeps_a_band_with_zero_gradient(orc):
+    """An exact eigenvector (constant band of the free h = 0.5, md = 1 grid,
+    gradient exactly zero) passes through the CG untouched, in FP64 and
+    FP32, with finite results for every band, as the oracle's guarded CG."""
+    from rsdft_develop_amd import gpu
+    from test_oracle_solvers import _exact_free_cell, zero_gradient_bands
+
+    cell = _exact_free_cell()
+    ham = orc.Ham(cell, projectors=False)
+    u = zero_gradient_bands(ham, 3)
+    ctx = gpu.Context()
+    try:
+        ctx.set_grid(cell.ngrid, cell.aa, cell.md)
+        ctx.set_vloc(np.zeros(cell.npoints))
+        for prec in (64, 32):
+            for pc in (0, 4):
+                _load(ctx, u)
+                esp, res = ctx.conjugate_gradient(ncg=4, pc_kind=pc, mloop=3, precision=prec)
+                got = ctx.download_bands()
+                assert np.all(np.isfinite(got)) and np.all(np.isfinite(esp)) and np.all(np.isfinite(res))
+                np.testing.assert_array_equal(got[:, 0], u[:, 0])
+                assert esp[0] == 0.0 and res[0] == 0.0
+                if prec == 64:
+                    _, esp_want, _ = ham.conjugate_gradient(u, ncg=4, pc_kind=pc)
+                    np.testing.assert_allclose(esp, esp_want, atol=1e-10)
+    finally:
+        ctx.close()

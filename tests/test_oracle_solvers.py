@@ -156,3 +156,87 @@ def test_inner_loop_converges_to_dense_spectrum(orc):
         u = ham.gram_schmidt(u)
     u, esp = ham.subspace_diag(u)
     np.testing.assert_allclose(esp[:6], w[:6], atol=1e-7)
+
+
+def _exact_free_cell(n=8, nb=3):
+    """A cubic cell with h = 0.5 bohr and md = 1: the FD weights
+    (1, -2, 1) * (-1/2)/h^2 are exact binary numbers, so the kinetic operator
+    maps a constant band to exactly zero and the constant band has an exactly
+    zero CG gradient."""
+    return systems.Cell(aa=np.eye(3) * 0.5 * n, frac=np.zeros((1, 3)), kind=np.zeros(1, dtype=np.int32),
+                        species=(systems.SI_GTH,), ngrid=(n, n, n), nband=nb, md=1)
+
+
+def zero_gradient_bands(ham, nb, seed=0):
+    """Band 0 constant (an exact eigenvector of T with eigenvalue 0), the rest
+    random, orthonormalised with band 0 first."""
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal((ham.N, nb))
+    u[:, 0] = 1.0
+    return ham.gram_schmidt(u, "lusl")
+
+
+def test_cg_keeps_a_band_with_zero_gradient(orc):
+    """A band whose gradient is exactly zero takes no step: no 0/0 in the
+    preconditioner, the direction update or the 2x2 Ritz step, and the
+    other bands still converge (their energies drop)."""
+    cell = _exact_free_cell()
+    ham = orc.Ham(cell, projectors=False)
+    u = zero_gradient_bands(ham, 3)
+    c = u[:, 0].copy()
+    assert np.abs(ham.hpsi(u[:, :1])).max() == 0.0
+    e0 = np.array([u[:, k] @ ham.hpsi(u[:, k:k + 1])[:, 0] * ham.dv for k in range(3)])
+    for pc in (0, 4):
+        for mbd in (1, 3):
+            v, esp, res = ham.conjugate_gradient(u, ncg=4, mbd=mbd, pc_kind=pc)
+            assert np.all(np.isfinite(v)) and np.all(np.isfinite(esp)) and np.all(np.isfinite(res))
+            np.testing.assert_array_equal(v[:, 0], c)
+            assert esp[0] == 0.0 and res[0] == 0.0
+            assert np.all(esp[1:] < e0[1:])
+
+
+def _overlaps(a, b, dv):
+    return a.T @ b * dv
+
+
+def test_cggs_swgs2_one_step_keeps_bands_orthogonal_to_the_others(orc):
+    """SWGS 2 (cggs_2) removes every other band from each preconditioned
+    gradient, so after one CG step of a whole block each new band is
+    orthogonal to every other band's old vector; without it they are not."""
+    _, ham, u = _tiny_system(orc, nb=6)
+    v2, _, _ = ham.conjugate_gradient(u, ncg=1, pc_kind=4, swgs=2)
+    v0, _, _ = ham.conjugate_gradient(u, ncg=1, pc_kind=4, swgs=0)
+    s2 = _overlaps(v2, u, ham.dv)
+    s0 = _overlaps(v0, u, ham.dv)
+    off = ~np.eye(6, dtype=bool)
+    assert np.abs(s2[off]).max() < 1e-12
+    assert np.abs(s0[off]).max() > 1e-4
+
+
+def test_cggs_swgs1_one_step_projects_out_the_earlier_bands(orc):
+    """SWGS 1 (cggs_1) removes only the bands before each one: after one CG
+    step of a whole block, new band j is orthogonal to old bands i < j but not
+    to later ones, and band 0 (nothing before it) moves exactly as without
+    the projection."""
+    _, ham, u = _tiny_system(orc, nb=6)
+    v1, e1, _ = ham.conjugate_gradient(u, ncg=1, pc_kind=4, swgs=1)
+    v0, e0, _ = ham.conjugate_gradient(u, ncg=1, pc_kind=4, swgs=0)
+    s1 = _overlaps(v1, u, ham.dv)  # s1[j, i] = dV <new j, old i>
+    below = np.array([[i < j for i in range(6)] for j in range(6)])
+    assert np.abs(s1[below]).max() < 1e-12
+    above = np.array([[i > j for i in range(6)] for j in range(6)])
+    assert np.abs(s1[above]).max() > 1e-4
+    np.testing.assert_array_equal(v1[:, 0], v0[:, 0])
+    assert e1[0] == e0[0]
+
+
+def test_cggs_swgs2_band_by_band_keeps_the_set_orthonormal(orc):
+    """With one band per block (MB_d = 1) and SWGS 2, each band's CG moves
+    only within the complement of all the others, so an orthonormal start
+    stays orthonormal through four CG steps."""
+    _, ham, u = _tiny_system(orc, nb=6)
+    v2, _, _ = ham.conjugate_gradient(u, ncg=4, mbd=1, pc_kind=4, swgs=2)
+    v0, _, _ = ham.conjugate_gradient(u, ncg=4, mbd=1, pc_kind=4, swgs=0)
+    off = ~np.eye(6, dtype=bool)
+    assert np.abs(_overlaps(v2, v2, ham.dv)[off]).max() < 1e-11
+    assert np.abs(_overlaps(v0, v0, ham.dv)[off]).max() > 1e-4

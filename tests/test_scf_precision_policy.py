@@ -26,8 +26,15 @@ def _run(des, **kw):
 
 def test_switch_below_twice_the_tolerance_by_default():
     precs, conv = _run([1.0, 1e-2, 1e-4, 3e-6, 1.5e-6, 9e-7, 5e-7])
-    # 1.5e-6 < 2e-6 switches after that iteration; the first FP64 one converges
-    assert precs == [32, 32, 32, 32, 32, 64] and conv
+    # 1.5e-6 < 2e-6 switches after that iteration; the first FP64 one is
+    # below the tolerance but measured against an FP32 energy, so the second
+    # FP64 one ends the loop
+    assert precs == [32, 32, 32, 32, 32, 64, 64] and conv
+
+
+def test_first_fp64_iteration_never_stops():
+    precs, conv = _run([1.0, 1e-2, 1.5e-6, 9e-7])
+    assert precs == [32, 32, 32, 64] and not conv
 
 
 def test_explicit_threshold():
@@ -36,9 +43,9 @@ def test_explicit_threshold():
 
 
 def test_fp32_never_ends_the_loop():
-    # below the stop tolerance in FP32: one FP64 iteration must confirm it
-    precs, conv = _run([1.0, 5e-7, 4e-7])
-    assert precs == [32, 32, 64] and conv
+    # below the stop tolerance in FP32: two FP64 iterations must confirm it
+    precs, conv = _run([1.0, 5e-7, 4e-7, 3e-7])
+    assert precs == [32, 32, 64, 64] and conv
 
 
 def test_stalled_fp32_switches():
