@@ -110,34 +110,52 @@ def kernel_traffic(tag: str):
     return None
 
 
-def cpu_leg(cell, vloc, target_s: float = 12.0):
-    """FP64 oracle hpsi on a band sample of the same cell, all host cores."""
+def physical_cores() -> int:
+    """Physical cores this process may run on: lscpu's cores per socket times
+    sockets, capped by the CPU affinity mask (a shared host gives a share)."""
+    import subprocess
+
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        info = dict(l.split(":", 1) for l in subprocess.run(["lscpu"], capture_output=True, text=True,
+                                                           timeout=10).stdout.splitlines() if ":" in l)
+        phys = int(info["Core(s) per socket"].strip()) * int(info["Socket(s)"].strip())
+        threads_per_core = int(info.get("Thread(s) per core", "1").strip())
+        return max(1, min(phys, allowed // max(1, threads_per_core) if allowed >= threads_per_core else allowed))
+    except Exception:
+        return max(1, allowed)
+
+
+def cpu_leg(cell, vloc):
+    """FP64 oracle hpsi over all 160 bands of the same cell on the host: the
+    median of 10 calls on all physical cores and of 3 calls on one thread."""
     from oracle import pyoracle
     from rsdft_develop_amd import systems
 
-    cores = os.cpu_count() or 1
-    pyoracle.set_threads(cores)
     ham = pyoracle.Ham(cell, vloc=vloc)
     E = int(ham.tables()["mjj"].sum())
-    nbs = 8
-    psi = systems.random_bands(cell.npoints, nbs, seed=7)
-    ham.hpsi(psi)  # warm-up
-    t = time.perf_counter()
-    calls = 0
-    while True:
-        ham.hpsi(psi)
-        calls += 1
-        el = time.perf_counter() - t
-        if el >= target_s or calls >= 5000:
-            break
-    per_call = el / calls
-    # scale the per-call time to the full 160-band workload (the oracle's
-    # cost is linear in bands; tables and Vloc are read once per call)
-    full = per_call * cell.nband / nbs
+    psi = systems.random_bands(cell.npoints, cell.nband, seed=7)
     B = hpsi_bytes(cell.npoints, cell.nband, E, 8)
-    return dict(value=B / full / 1e9, unit="GB/s", cores=cores, kind="port",
-                sample=f"oracle hpsi FP64, {nbs} of {cell.nband} bands, {calls} calls in {el:.1f} s, "
-                       f"scaled linearly to {cell.nband} bands ({full * 1e3:.1f} ms/call)")
+
+    def median_call(threads: int, calls: int) -> float:
+        pyoracle.set_threads(threads)
+        ham.hpsi(psi)  # warm-up
+        ts = []
+        for _ in range(calls):
+            t = time.perf_counter()
+            ham.hpsi(psi)
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
+
+    cores = physical_cores()
+    t_all = median_call(cores, 10)
+    t_one = median_call(1, 3)
+    pyoracle.set_threads(cores)
+    return dict(value=B / t_all / 1e9, unit="GB/s", cores=cores, kind="port",
+                sample=f"oracle hpsi FP64 over all {cell.nband} bands of the same Si64 cell, median of 10 calls on "
+                       f"{cores} physical cores ({t_all * 1e3:.1f} ms/call)",
+                one_thread={"value": B / t_one / 1e9, "unit": "GB/s", "cores": 1,
+                            "sample": f"median of 3 calls on one thread ({t_one * 1e3:.1f} ms/call)"})
 
 
 NBANDS = {1: 20, 2: 160, 3: 480, 4: 1100}
@@ -147,11 +165,15 @@ def progress(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_kind: int = 4, params=None):
+def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_kind: int = 4, weights=None,
+            refine_etol: float | None = None):
     """Wall seconds of a whole SCF run to dE < etol Ha/atom (the run's own
     set-up on the device included: tables, initial density and GS).  Pulay
-    mixing in G space with Kerker weights (IMIX 30), EKBT 0.01 Ha."""
-    from rsdft_develop_amd import gpu, scf, systems
+    mixing in G space with Kerker weights (IMIX 30), EKBT 0.01 Ha.  With
+    pc_kind 5 or 6 the learned preconditioner's weights come from `weights`.
+    With refine_etol the same run continues, untimed, to dE < refine_etol
+    and its eigenvalues and energy are returned as `reference`."""
+    from rsdft_develop_amd import gpu, lpc, scf, systems
 
     cell = systems.silicon_supercell(nrep, nband=NBANDS[nrep], displacement=0.05, seed=0)
     ctx = gpu.Context(0)
@@ -160,10 +182,13 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     ctx.autotune_hpsi(64, cell.nband, reps=2)
     ctx.autotune_hpsi(32, cell.nband, reps=2)
     gram = ctx.autotune_gram(cell.nband)
-    if params is not None:
-        ctx.lpc_set_params(params)
+    if weights is not None:
+        ctx.lpc_set_params(lpc.load(weights))
+        if pc_kind == 6:
+            ctx.lpc_set_hybrid_scale(lpc.load_hybrid_scale(weights))
     tag = f"scf Si{cell.natom} {'mixed' if mixed else 'fp64'} pc{pc_kind}"
     progress(f"{tag}: start")
+    fb0 = ctx.lpc_fallbacks()
     t0 = time.perf_counter()
     run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=etol, max_iter=max_iter, mixed=mixed, imix=30,
                                       pc_kind=pc_kind, start="uniform"), ctx=ctx)
@@ -172,12 +197,20 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     wall = time.perf_counter() - t0
     its = res.iterations
     out = dict(seconds=round(wall, 3), setup_s=round(res.setup, 3), iterations=len(its), converged=bool(res.converged),
-               gram_chunk=gram[1], gram_ms=round(gram[0], 3),
+               pc_kind=pc_kind, gram_chunk=gram[1], gram_ms=round(gram[0], 3),
                etot=res.etot,
                fp32_iterations=sum(1 for r in its if r.precision == 32),
                per_phase_s={k: round(sum(r.times.get(k, 0.0) for r in its), 3)
                             for k in ("sd", "cg", "gs", "esp", "eto", "pot")},
                esp=np.sort(res.esp).tolist())
+    if pc_kind in (5, 6):
+        out["descent_check_fallbacks"] = ctx.lpc_fallbacks() - fb0
+    if refine_etol is not None:
+        run.p.etol_per_atom = refine_etol
+        run.p.mixed = False
+        ref = run.run(callback=lambda r: progress(f"{tag} refine: iter {r.iter} dE/atom {r.de_per_atom:.2e}"))
+        out["reference"] = dict(etol_per_atom=refine_etol, extra_iterations=ref.niter, converged=bool(ref.converged),
+                                etot=ref.etot, esp=np.sort(ref.esp).tolist())
     ctx.close()
     return out
 
@@ -199,7 +232,7 @@ def scf_config0_leg():
     g = scf.SCF(cell, scf.ScfParams(**params), ctx=ctx).run()
     gpu_s = time.perf_counter() - t0
     ctx.close()
-    cores = os.cpu_count() or 1
+    cores = physical_cores()
     pyoracle.set_threads(cores)
     t0 = time.perf_counter()
     c = scf_oracle.run_scf(cell, **params)
@@ -216,14 +249,18 @@ def scf_warmup(nrep: int):
     neither timed run pays the process's first use of the kernels and
     library paths at this size (rocSOLVER's eigensolver for its band count
     among them)."""
-    from rsdft_develop_amd import gpu, scf, systems
+    from rsdft_develop_amd import gpu, lpc, scf, systems
 
     cell = systems.silicon_supercell(nrep, nband=NBANDS[nrep], displacement=0.05, seed=0)
     ctx = gpu.Context(0)
     ctx.set_grid(cell.ngrid, cell.aa, cell.md)
     ctx.build_gth_projectors(cell)
-    scf.SCF(cell, scf.ScfParams(ekbt=0.01, max_iter=2, mixed=True, fp64_below=1e9, imix=30, start="uniform"),
-            ctx=ctx).run()
+    for pc_kind, weights in ((4, None), (6, LPC_HYBRID)):
+        if weights is not None:
+            ctx.lpc_set_params(lpc.load(weights))
+            ctx.lpc_set_hybrid_scale(lpc.load_hybrid_scale(weights))
+        scf.SCF(cell, scf.ScfParams(ekbt=0.01, max_iter=2, mixed=True, fp64_below=1e9, imix=30, start="uniform",
+                                    pc_kind=pc_kind), ctx=ctx).run()
     ctx.close()
 
 
@@ -288,14 +325,20 @@ def batch_leg(ncells: int = 64, workers: int = 8, seq_cells: int | None = None):
     # rocSOLVER, hipFFT code objects load on first use) before either timing
     firsts = list({c.cell.ngrid: c for c in cells}.values())
     batch.run_batch(firsts, workers=1, shapes=shapes)
-    progress(f"batch: {ncells} cells, {workers} workers")
-    par = batch.run_batch(cells, workers=workers, shapes=shapes)
-    progress(f"batch: {par.wall:.2f} s; sequential sample of {seq_cells}")
-    seq = batch.run_batch(cells[:seq_cells], workers=1, shapes=shapes)
+    progress(f"batch: {ncells} cells, {workers} workers, CG from captured graphs")
+    par = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=True)
+    progress(f"batch: {par.wall:.2f} s; the same without graphs")
+    par_direct = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=False)
+    progress(f"batch: {par_direct.wall:.2f} s; sequential plain solves of {seq_cells}")
+    seq = batch.run_batch(cells[:seq_cells], workers=1, shapes=shapes, graphs=False)
     its = [r.iterations for r in par.results]
+    same = all(a.iterations == b.iterations and a.etot == b.etot for a, b in zip(par.results, par_direct.results))
     return dict(workload=f"{ncells} Si8 cells, scale U(0.95,1.10), displacement N(0,0.1), grids {{24,32,40,48}}^3, "
                          f"U{{20..40}} bands, FP64 SCF to dE < 1e-6 Ha/atom",
+                gpu_max_hw_queues=os.environ.get("GPU_MAX_HW_QUEUES"),
                 workers=workers, batched_seconds=round(par.wall, 3), batched_cells_per_s=round(par.cells_per_s, 3),
+                batched_direct_launch_cells_per_s=round(par_direct.cells_per_s, 3),
+                graph_results_equal_direct=same,
                 sequential_cells=seq_cells, sequential_seconds=round(seq.wall, 3),
                 sequential_cells_per_s=round(seq.cells_per_s, 3),
                 speedup=round(par.cells_per_s / seq.cells_per_s, 3),
@@ -369,20 +412,31 @@ def main() -> int:
     if args.scf > 0:
         progress("scf warm-up")
         scf_warmup(args.scf)
-        plain = scf_leg(args.scf, mixed=False)
-        mixed = scf_leg(args.scf, mixed=True)
         natom = 8 * args.scf ** 3
-        deig = np.abs(np.array(mixed.pop("esp")) - np.array(plain.pop("esp")))
         nocc = 2 * natom  # 4 valence electrons per Si: the lowest 2 natom bands are the occupied ones
-        de = float(deig.max())
+        # plain FP64, then mixed precision with hprsdft, with the learned
+        # preconditioner (pc_kind 5) and with the hybrid (pc_kind 6); the FP64
+        # run continues untimed to dE < 1e-9 Ha/atom as the reference the
+        # eigenvalue and energy errors of every run are measured against
+        plain = scf_leg(args.scf, mixed=False, refine_etol=1e-9)
+        runs = {"fp64": plain,
+                "mixed": scf_leg(args.scf, mixed=True),
+                "mixed_learned": scf_leg(args.scf, mixed=True, pc_kind=5, weights=LPC_WEIGHTS),
+                "mixed_hybrid": scf_leg(args.scf, mixed=True, pc_kind=6, weights=LPC_HYBRID)}
+        ref = plain.pop("reference")
+        e_ref = np.array(ref.pop("esp"))
+        for name, r in runs.items():
+            d = np.abs(np.array(r.pop("esp")) - e_ref)
+            r["max_eig_err_vs_reference_Ha"] = float(d.max())
+            r["max_eig_err_vs_reference_occupied_Ha"] = float(d[:nocc].max())
+            r["etot_err_vs_reference_Ha_per_atom"] = abs(r["etot"] - ref["etot"]) / natom
+            if name != "fp64":
+                r["speedup_vs_fp64"] = round(plain["seconds"] / r["seconds"], 3)
         scf_out = {"workload": f"Si{natom} {36 * args.scf}^3 grid, {NBANDS[args.scf]} bands, EKBT 0.01 Ha, "
                                f"IMIX 30, uniform random start bands drawn on the device, to dE < 1e-6 Ha/atom",
-                   "fp64": plain, "mixed": mixed,
-                   "speedup_mixed_vs_fp64": round(plain["seconds"] / mixed["seconds"], 3),
-                   "mixed_max_eig_diff_vs_fp64_Ha": de,
-                   "mixed_max_eig_diff_vs_fp64_occupied_Ha": float(deig[:nocc].max()),
-                   "occupied_bands": nocc,
-                   "mixed_etot_diff_vs_fp64_Ha_per_atom": abs(mixed["etot"] - plain["etot"]) / natom}
+                   "reference": ref, "occupied_bands": nocc, **runs,
+                   "speedup_mixed_learned_vs_fp64": runs["mixed_learned"]["speedup_vs_fp64"],
+                   "speedup_mixed_vs_fp64": runs["mixed"]["speedup_vs_fp64"]}
     step_out = step_leg(4, args.si512) if args.si512 > 0 else None
     out = {
         "metric": METRIC,

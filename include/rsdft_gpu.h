@@ -190,6 +190,16 @@ int rsg_subspace_diag(rsg_ctx *ctx, double *esp);
  * mb). */
 int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop,
                            double *esp, double *res);
+/* enable != 0: rsg_conjugate_gradient(_f32) with pc_kind 0 or 4 and SWGS 0
+ * replays each band block's kernel sequence from a captured HIP graph
+ * (captured on the second call with the same block, buffers and launch
+ * parameters, replayed from the third on; any change of those captures
+ * anew).  Results equal the direct launches bit for bit.  0 (default)
+ * launches directly and frees the captured graphs.  No reference
+ * counterpart (launch-overhead reduction for many small cells). */
+int rsg_set_cg_graphs(rsg_ctx *ctx, int enable);
+/* CG band blocks run from a captured graph since the context was created. */
+int rsg_cg_graph_launches(rsg_ctx *ctx, int64_t *launches);
 /* The SWGS keyword of the CG (read_cg, cg_module.f90:42-50): 0 (default)
  * leaves the preconditioned gradients as they are; 1 and 2 project each
  * one against the other bands of the store before the direction update,
@@ -205,8 +215,8 @@ int rsg_set_cg_swgs(rsg_ctx *ctx, int swgs);
  * x0 = g.  Replaces precond_hprsdft (cgpc_hprsdft_module.f90:133-270) with
  * Pgk entering as gk (its caller, cg_module.f90, sets Pgk = gk first); E
  * is not an argument, since the reference's E0 is zero.  Asynchronous on
- * rsg_stream(ctx) apart from one read-back of the first residual norms (the
- * reference's all(rr1 < 1e-24) early exit). */
+ * rsg_stream(ctx): the reference's all(rr1 < 1e-24) early exit is decided
+ * on the device (every step then takes a zero step, leaving pg = g). */
 int rsg_precond_hprsdft_f64(rsg_ctx *ctx, int mloop, const double *g, double *pg,
                             int64_t ld, int nb);
 int rsg_precond_hprsdft_f32(rsg_ctx *ctx, int mloop, const float *g, float *pg,

@@ -6,6 +6,11 @@ cells from a queue.  The library calls release the GIL, so the streams'
 small launches overlap on the GPU, where one cell alone leaves most of the
 chip idle.
 
+Each context replays the CG band blocks of its SCF steps from captured HIP
+graphs once their buffers and launch parameters have settled (two calls
+with the same block), so a small cell's CG step costs one graph launch
+instead of a hundred kernel launches.
+
 The hpsi launch shape is tuned once per grid size by the library's
 autotuner (`rsg_autotune_hpsi`) and handed to every context that meets that
 grid.  The band chunk is left to the library, so it follows each cell's
@@ -103,8 +108,11 @@ class ShapeCache:
 
 
 def run_batch(cells: list, workers: int = 8, params: scf.ScfParams | None = None, shapes: ShapeCache | None = None,
-              log=None) -> BatchResult:
-    """SCF of every cell, `workers` at a time.  Results in input order."""
+              log=None, graphs: bool = True) -> BatchResult:
+    """SCF of every cell, `workers` at a time.  Results in input order.
+    With `graphs` every context replays its CG band blocks from captured
+    HIP graphs (rsg_set_cg_graphs), which removes the per-kernel launch cost
+    that dominates small cells."""
     params = params or scf.ScfParams(ekbt=0.01, etol_per_atom=1e-6, max_iter=80, imix=30, start="uniform")
     shapes = shapes or ShapeCache()
     todo = queue.Queue()
@@ -121,6 +129,7 @@ def run_batch(cells: list, workers: int = 8, params: scf.ScfParams | None = None
         # (a free waits for the whole device, stalling every other worker)
         ctx = gpu.Context(0)
         try:
+            ctx.set_cg_graphs(graphs)
             while True:
                 try:
                     i = todo.get_nowait()

@@ -474,6 +474,23 @@ int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double
   return rsg::conjugate_gradient(ctx, ncg, pc_kind, mloop, esp, res);
 }
 
+int rsg_set_cg_graphs(rsg_ctx *ctx, int enable) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto &g : ctx->cg_graph_cache)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  ctx->cg_graph_cache.clear();
+  ctx->cg_graphs = enable != 0;
+  return RSG_OK;
+}
+
+int rsg_cg_graph_launches(rsg_ctx *ctx, int64_t *launches) {
+  if (!ctx || !launches) return RSG_ERR_ARG;
+  *launches = ctx->cg_graph_launches;
+  return RSG_OK;
+}
+
 int rsg_set_cg_swgs(rsg_ctx *ctx, int swgs) {
   if (!ctx) return RSG_ERR_ARG;
   if (swgs < 0 || swgs > 2) return ctx->fail(RSG_ERR_ARG, "set_cg_swgs: SWGS must be 0, 1 or 2");

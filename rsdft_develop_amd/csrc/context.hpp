@@ -130,6 +130,16 @@ struct LpcState {
   void release();
 };
 
+// One captured CG band block: the key bytes (every size, pointer and launch
+// parameter the block's kernels are built from) and the executable graph;
+// seen counts eager runs of the key before its capture.
+struct CgGraph {
+  std::vector<unsigned char> key;
+  hipGraphExec_t exec = nullptr;
+  int seen = 0;
+  bool failed = false;  // capture not possible here: stay eager
+};
+
 struct HostTables {
   int nzlma = 0, maxmjj = 0;
   std::vector<int> mjj, jjp, iuv, amap;  // jjp/uvk column-major (maxmjj,nzlma)
@@ -197,6 +207,9 @@ struct rsg_ctx {
   rsg::DevBuf pcbuf;               // work vectors of the stand-alone preconditioner call
   rsg::DevBuf cgcoef;              // cggs coefficients dV U^T Pg (mb x block)
   int cg_swgs = 0;                 // SWGS keyword: 0 none, 1/2 cggs projections in CG
+  bool cg_graphs = false;          // replay CG band blocks from captured HIP graphs
+  int64_t cg_graph_launches = 0;   // band blocks run from a captured graph
+  std::vector<rsg::CgGraph> cg_graph_cache;
   // K-split of the U^T X Gram products chosen by the autotuner, per
   // (N, nb): points per chunk, 0 = one GEMM over all of K
   std::vector<std::array<int64_t, 3>> gram_split;

@@ -151,9 +151,17 @@ __global__ void k_cg_dir(const W *__restrict__ pg, W *__restrict__ pk, const dou
 // (<x,p|H|x,p>, <x,p|x,p>), root switch, sign fix.  d6 holds
 // (xx, px, pp, xHx, pHx, pHp) * dV as [q*nb + b].
 __global__ void k_cg_ritz(const double *__restrict__ d6, int nb, double *__restrict__ E,
-                          double *__restrict__ E1, double *__restrict__ uc) {
+                          double *__restrict__ E1, double *__restrict__ uc, const int *__restrict__ done) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
+  if (*done) {
+    // the block has converged (the reference leaves the CG loop here): the
+    // remaining steps keep every band, u = (1, 0), E unchanged
+    uc[2 * b] = 1.0;
+    uc[2 * b + 1] = 0.0;
+    E1[b] = E[b];
+    return;
+  }
   const double b11 = d6[0 * nb + b], b12 = d6[1 * nb + b], b22 = d6[2 * nb + b];
   const double a11 = d6[3 * nb + b], a12 = d6[4 * nb + b], a22 = d6[5 * nb + b];
   // B = U^T U, C = U^-T A U^-1, C = Y diag(w) Y^T, X = U^-1 Y
@@ -327,11 +335,46 @@ __global__ void k_pc_init(const W *__restrict__ g, const W *__restrict__ f, W *_
 // a = rr1/pAp, rr0 = rr1 (per band).  A band whose residual is exactly
 // zero (its direction too, so pAp = 0) takes a zero step instead of 0/0.
 __global__ void k_pc_alpha(const double *__restrict__ rr1, const double *__restrict__ pap,
-                           double *__restrict__ rr0, double *__restrict__ a, int nb) {
+                           double *__restrict__ rr0, double *__restrict__ a, int nb,
+                           const int *__restrict__ skip) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   rr0[b] = rr1[b];
-  a[b] = pap[b] != 0.0 ? rr1[b] / pap[b] : 0.0;
+  // skip: every first residual below the early-exit bound, so the result
+  // stays x = g (the reference returns before its first step)
+  a[b] = (skip && *skip) ? 0.0 : (pap[b] != 0.0 ? rr1[b] / pap[b] : 0.0);
+}
+
+// flag = all(v[b] < bound) over nb values (one block); the hprsdft early
+// exit all(rr1 < 1e-24) (cgpc_hprsdft_module.f90:160) decided on the device
+__global__ void k_all_below(const double *__restrict__ v, int nb, double bound, int *__restrict__ flag) {
+  __shared__ int any_above;
+  if (threadIdx.x == 0) any_above = 0;
+  __syncthreads();
+  int above = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) above |= !(v[b] < bound);
+  if (above) atomicOr(&any_above, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) *flag = any_above ? 0 : 1;
+}
+
+// flag |= all(|E - E1| < 1e-15): the CG block's convergence test
+// (cg_module.f90:236), sticky for the rest of the call
+__global__ void k_cg_conv(const double *__restrict__ E, const double *__restrict__ E1, int nb,
+                          int *__restrict__ flag) {
+  __shared__ int any_moving;
+  if (threadIdx.x == 0) any_moving = 0;
+  __syncthreads();
+  int moving = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) moving |= !(fabs(E[b] - E1[b]) < 1e-15);
+  if (moving) atomicOr(&any_moving, 1);
+  __syncthreads();
+  if (threadIdx.x == 0 && !any_moving) *flag = 1;
+}
+
+__global__ void k_fill_int(int *__restrict__ a, int v, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
 }
 
 // rr1/rr0 with a zero previous residual (a band already at rest) taken as 0
@@ -710,6 +753,9 @@ struct Scratch {
   // per-band device scalars, each nb doubles
   double *E, *E1, *rb, *gkgk, *res, *d6, *uc, *rr0, *rr1, *a, *pap;
   int *accept;
+  // device-side decisions: flags[0] the CG block has converged, flags[1]
+  // the preconditioner's first residuals are all below its exit bound
+  int *flags;
 };
 
 int ensure_scalars(rsg_ctx *ctx, int nb, int nseg, Scratch &s) {
@@ -731,6 +777,7 @@ int ensure_scalars(rsg_ctx *ctx, int nb, int nseg, Scratch &s) {
   s.d6 = p; p += 6 * per;
   s.uc = p; p += 2 * per;
   s.accept = reinterpret_cast<int *>(p);
+  s.flags = s.accept + per;  // 4 ints inside the 256-byte tail
   return RSG_OK;
 }
 
@@ -870,7 +917,7 @@ int precond_hprsdft(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W *r, W 
     const W *xs[1] = {p}, *ys[1] = {f};
     int rc = multidot(ctx, 1, xs, ys, nn, 1.0, s.pap);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pc_alpha, gb, b64, 0, ctx->stream, s.rr1, s.pap, s.rr0, s.a, nn);
+    hipLaunchKernelGGL(k_pc_alpha, gb, b64, 0, ctx->stream, s.rr1, s.pap, s.rr0, s.a, nn, nullptr);
     const int last = it == mloop;
     hipLaunchKernelGGL((k_pc_step<W>), gN, blk, 0, ctx->stream, pg, p, r, f, s.a, last ? 0 : 1, N);
     if (last) break;
@@ -959,14 +1006,9 @@ int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W 
   };
   pass(std::integral_constant<int, kPzFirst>{});
   reduce(1.0, s.rr1);
-  {
-    std::vector<double> h(static_cast<size_t>(nn));
-    RSG_HIP(hipMemcpyAsync(h.data(), s.rr1, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    RSG_HIP(hipStreamSynchronize(ctx->stream));
-    bool small = true;
-    for (double v : h) small = small && (v < 1e-24);
-    if (small) return plain();
-  }
+  // the early exit all(rr1 < 1e-24) decided on the device: every step then
+  // takes a zero step, leaving P g = g, with no read-back to the host
+  hipLaunchKernelGGL(k_all_below, dim3(1), dim3(256), 0, ctx->stream, s.rr1, nn, 1e-24, s.flags + 1);
   W *pcur = p, *pnext = f;
   for (int it = 1; it <= mloop; ++it) {
     A.p = pcur;
@@ -979,7 +1021,7 @@ int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W 
       A.p = pcur;
     }
     reduce(1.0, s.pap);
-    hipLaunchKernelGGL(k_pc_alpha, gb, b64, 0, ctx->stream, s.rr1, s.pap, s.rr0, s.a, nn);
+    hipLaunchKernelGGL(k_pc_alpha, gb, b64, 0, ctx->stream, s.rr1, s.pap, s.rr0, s.a, nn, s.flags + 1);
     const bool first = it == 1, last = it == mloop;
     if (first && last)
       pass(std::integral_constant<int, kPzStepFL>{});
@@ -1228,6 +1270,98 @@ int cggs_block(rsg_ctx *ctx, int swgs, int n0, int nn, W *pg, double *tmp) {
   return RSG_OK;
 }
 
+// Key of a CG band block's kernel sequence: the block, the solver
+// parameters, every device buffer address and the host-side launch
+// parameters the kernels are built from.  A block replays a captured graph
+// only while all of these are unchanged (a buffer that grows moves, and
+// so changes the key); the buffers' contents (Vloc, bands) may change.
+template <typename V> void key_put(std::vector<unsigned char> &k, const V &v) {
+  const unsigned char *p = reinterpret_cast<const unsigned char *>(&v);
+  k.insert(k.end(), p, p + sizeof(V));
+}
+
+template <typename T> void key_analytic(std::vector<unsigned char> &k, const AnalyticDev<T> &a) {
+  for (const DevBuf *b : {&a.es, &a.ep, &a.dd, &a.amp, &a.scale, &a.pts, &a.seg_beg, &a.seg_end, &a.seg_grp,
+                          &a.seg_order, &a.grp_seg_off, &a.hits, &a.tile_off, &a.tile_g})
+    key_put(k, b->p);
+  for (int v : {a.ngroup, a.nseg, a.S, a.ntile}) key_put(k, v);
+}
+
+template <typename T> void key_tables(std::vector<unsigned char> &k, const NonlocalDev<T> &t) {
+  for (const DevBuf *b : {&t.grp_lma0, &t.grp_nl, &t.grp_seg_off, &t.seg_beg, &t.seg_end, &t.seg_grp, &t.grp_pts,
+                          &t.grp_coef, &t.lma_grp, &t.scale, &t.sl_off, &t.sl_len, &t.sl_lma, &t.sl_uv})
+    key_put(k, b->p);
+  for (int v : {t.ngroup, t.nseg}) key_put(k, v);
+}
+
+std::vector<unsigned char> cg_graph_key(const rsg_ctx *ctx, int wsize, int ns, int nn, int mbd, int ncg, int pc_kind,
+                                        int mloop) {
+  std::vector<unsigned char> k;
+  for (int v : {wsize, ns, nn, mbd, ncg, pc_kind, mloop, ctx->n[0], ctx->n[1], ctx->n[2], ctx->md, ctx->nl_mode,
+                int(ctx->have_ana), int(ctx->have_nl), ctx->tables.nzlma})
+    key_put(k, v);
+  key_put(k, ctx->N);
+  for (const DevBuf *b : {&ctx->unk, &ctx->cgbuf, &ctx->scal, &ctx->dotpart, &ctx->cproj, &ctx->uvpart, &ctx->uvunk,
+                          &ctx->vloc64, &ctx->vloc32, &ctx->dead})
+    key_put(k, b->p);
+  key_analytic(k, ctx->ana64);
+  key_analytic(k, ctx->ana32);
+  key_tables(k, ctx->nl64);
+  key_tables(k, ctx->nl32);
+  for (const HpsiShape *h : {&ctx->shape64, &ctx->shape32})
+    for (int v : {h->ry, h->ty, h->zlen, h->chunk, h->bt, h->order, h->rx}) key_put(k, v);
+  for (double v : ctx->coef_lap) key_put(k, v);
+  for (double v : {ctx->coef_lap0, ctx->dv, ctx->hgrid[0], ctx->hgrid[1], ctx->hgrid[2], ctx->ggg[0], ctx->ggg[1],
+                   ctx->ggg[2]})
+    key_put(k, v);
+  return k;
+}
+
+// Run the work `enqueue` queues on the context stream through a captured
+// HIP graph: the first call of a key runs eagerly (its buffers reach their
+// final size), the second captures and launches the graph, later ones
+// replay it.  A capture the runtime refuses leaves the key eager.
+template <typename F> int run_graphed(rsg_ctx *ctx, const std::vector<unsigned char> &key, F &&enqueue) {
+  CgGraph *e = nullptr;
+  for (auto &g : ctx->cg_graph_cache)
+    if (g.key == key) e = &g;
+  if (!e) {
+    constexpr size_t kMaxGraphs = 16;
+    if (ctx->cg_graph_cache.size() >= kMaxGraphs) {
+      if (ctx->cg_graph_cache.front().exec) (void)hipGraphExecDestroy(ctx->cg_graph_cache.front().exec);
+      ctx->cg_graph_cache.erase(ctx->cg_graph_cache.begin());
+    }
+    ctx->cg_graph_cache.push_back(CgGraph());
+    e = &ctx->cg_graph_cache.back();
+    e->key = key;
+  }
+  if (e->exec) {
+    RSG_HIP(hipGraphLaunch(e->exec, ctx->stream));
+    ++ctx->cg_graph_launches;
+    return RSG_OK;
+  }
+  if (e->failed || e->seen++ == 0) return enqueue();
+  RSG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue();
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc == RSG_OK && ce == hipSuccess && graph &&
+      hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    RSG_HIP(hipGraphLaunch(e->exec, ctx->stream));
+    ++ctx->cg_graph_launches;
+    return RSG_OK;
+  }
+  // the capture failed (or the work reported an error while recording):
+  // nothing ran, so run the block eagerly and keep this key eager
+  if (graph) (void)hipGraphDestroy(graph);
+  e->exec = nullptr;
+  e->failed = true;
+  (void)hipGetLastError();
+  ctx->err.clear();
+  return enqueue();
+}
+
 // d_conjugate_gradient_1 (cg_module.f90:112-431) with the band block MB_d
 // as large as the CG work arrays allow.  W = double is the plain FP64
 // method on the store itself.  W = float keeps every CG and preconditioner
@@ -1259,9 +1393,10 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
   W *hxk = base, *hpk = base + blk, *gk = base + 2 * blk, *pgk = base + 3 * blk, *pk = base + 4 * blk;
   W *pr = base + 5 * blk, *pp = base + 6 * blk, *pf = base + 7 * blk;
   W *x32 = kMixed ? base + 8 * blk : nullptr;
-  std::vector<double> hE(static_cast<size_t>(mbd)), hE1(static_cast<size_t>(mbd));
-  for (int ns = 0; ns < mb; ns += mbd) {
-    const int nn = std::min(mbd, mb - ns);
+  // everything one band block queues on the stream; no host read-back for
+  // pc_kind 0 and 4 (the learned ones read back their descent check)
+  auto enqueue_block = [&](int ns, int nn) -> int {
+    int rc = RSG_OK;
     double *xs64 = ctx->unk.as<double>() + size_t(ns) * N;
     const dim3 gN = grid_nb(N, nn), blkd(kThreads), gb((nn + 63) / 64), b64(64);
     W *x;
@@ -1274,6 +1409,7 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
       x = reinterpret_cast<W *>(xs64);
     }
     hipLaunchKernelGGL(k_fill, gb, b64, 0, ctx->stream, s.E1, 1.0e10, nn);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, s.flags, 0, 2);
     rc = launch_hpsi<W>(ctx, x, hxk, N, nn);
     if (rc) return rc;
     {
@@ -1287,12 +1423,11 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
     }
     for (int icg = 1; icg <= ncg + 1; ++icg) {
       hipLaunchKernelGGL(k_scale_copy, gb, b64, 0, ctx->stream, s.rb, 0.25, s.res, nn);
-      RSG_HIP(hipMemcpyAsync(hE.data(), s.E, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
-      RSG_HIP(hipMemcpyAsync(hE1.data(), s.E1, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
-      RSG_HIP(hipStreamSynchronize(ctx->stream));
-      bool conv = true;
-      for (int b = 0; b < nn; ++b) conv = conv && (std::fabs(hE[b] - hE1[b]) < 1e-15);
-      if (conv || icg == ncg + 1) break;
+      if (icg == ncg + 1) break;
+      // convergence test on the device: once the whole block has
+      // |E - E1| < 1e-15 the remaining steps leave it as it is (the
+      // reference exits the loop), without a read-back per step
+      hipLaunchKernelGGL(k_cg_conv, dim3(1), dim3(256), 0, ctx->stream, s.E, s.E1, nn, s.flags);
       // pgk = P gk and rb = dV <pgk, gk>
       if (pc_kind == 4 || pc_kind == 6) {
         if ((rc = precond_hprsdft_fused<W>(ctx, mloop, gk, pgk, nn, pr, pp, pf, s, dv, s.rb))) return rc;
@@ -1343,7 +1478,7 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
       RSG_HIP(hipMemcpyAsync(s.gkgk, s.rb, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
       if ((rc = launch_hpsi<W>(ctx, pk, hpk, N, nn))) return rc;
       if ((rc = ritz_dots<W>(ctx, x, pk, hxk, hpk, nn, dv, s.d6))) return rc;
-      hipLaunchKernelGGL(k_cg_ritz, gb, b64, 0, ctx->stream, s.d6, nn, s.E, s.E1, s.uc);
+      hipLaunchKernelGGL(k_cg_ritz, gb, b64, 0, ctx->stream, s.d6, nn, s.E, s.E1, s.uc, s.flags);
       hipLaunchKernelGGL((k_cg_update<W>), gN, blkd, 0, ctx->stream, hxk, hpk, x, pk, gk, s.uc, s.E, N);
       {
         const W *xs[1] = {gk}, *ys[1] = {gk};
@@ -1353,6 +1488,18 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
       hipLaunchKernelGGL((k_cg_accept<W>), gN, blkd, 0, ctx->stream, xs64, x32, pk, s.uc, s.accept, N);
       RSG_HIP(hipGetLastError());
     }
+    return RSG_OK;
+  };
+  const bool graphable = ctx->cg_graphs && (pc_kind == 0 || pc_kind == 4) && ctx->cg_swgs == 0;
+  for (int ns = 0; ns < mb; ns += mbd) {
+    const int nn = std::min(mbd, mb - ns);
+    if (graphable) {
+      const std::vector<unsigned char> key = cg_graph_key(ctx, int(sizeof(W)), ns, nn, mbd, ncg, pc_kind, mloop);
+      rc = run_graphed(ctx, key, [&]() { return enqueue_block(ns, nn); });
+    } else {
+      rc = enqueue_block(ns, nn);
+    }
+    if (rc) return rc;
     RSG_HIP(hipMemcpyAsync(esp + ns, s.E, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
     RSG_HIP(hipMemcpyAsync(res + ns, s.res, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
     RSG_HIP(hipStreamSynchronize(ctx->stream));
@@ -1500,6 +1647,9 @@ void solver_release(rsg_ctx *ctx) {
   ctx->dotpart.release();
   ctx->grampart.release();
   ctx->pcbuf.release();
+  for (auto &g : ctx->cg_graph_cache)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  ctx->cg_graph_cache.clear();
 }
 
 }  // namespace rsg

@@ -84,6 +84,8 @@ def load_library():
         "rsg_precond_hprsdft_f64": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
         "rsg_lpc_set_hybrid_scale": (_c_int, [_vp, _c_dbl]),
         "rsg_set_cg_swgs": (_c_int, [_vp, _c_int]),
+        "rsg_set_cg_graphs": (_c_int, [_vp, _c_int]),
+        "rsg_cg_graph_launches": (_c_int, [_vp, _P(_c_i64)]),
         "rsg_autotune_gram": (_c_int, [_vp, _c_int, _c_int, _P(ctypes.c_float), _P(_c_i64)]),
         "rsg_axpby_f32": (_c_int, [_vp, _vp, _vp, ctypes.c_float, ctypes.c_float, _c_i64]),
         "rsg_precond_hprsdft_f32": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
@@ -287,6 +289,15 @@ class Context:
     def lpc_set_hybrid_scale(self, scale: float):
         """pc_kind 6: P g = scale * P_hprsdft g + network(g)."""
         self._check(self.L.rsg_lpc_set_hybrid_scale(self.h, float(scale)))
+
+    def set_cg_graphs(self, enable: bool = True):
+        """Replay CG band blocks (pc_kind 0/4, SWGS 0) from captured HIP graphs."""
+        self._check(self.L.rsg_set_cg_graphs(self.h, int(bool(enable))))
+
+    def cg_graph_launches(self) -> int:
+        n = _c_i64()
+        self._check(self.L.rsg_cg_graph_launches(self.h, ctypes.byref(n)))
+        return n.value
 
     def set_cg_swgs(self, swgs: int):
         """SWGS keyword of the CG: 1/2 project each preconditioned gradient

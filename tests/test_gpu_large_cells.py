@@ -1,0 +1,151 @@
+"""Configurations 3 and 4 at full size: Si216 (3x3x3, 108^3 grid, 480
+bands) and Si512 (4x4x4, 144^3 grid, 1100 bands), atoms displaced by
+N(0, 0.05 bohr), analytic GTH projectors.
+
+Direct parity on sampled bands and full-store properties that do not
+depend on the oracle's cost at this size:
+
+- hpsi of sampled bands (inside a 64-band call) against the oracle, 1e-12
+  of max|H psi| in FP64 and 2e-5 in FP32;
+- Gram-Schmidt of the whole device-drawn store: sampled bands orthonormal to
+  1e-12, and a second Gram-Schmidt leaves the store unchanged to 1e-11
+  (it would move every band by about |dV U^T U - I| otherwise);
+- subspace_diag of the whole store: eigenvalues ascending and equal to
+  esp_calc's dV <u_n|H u_n> over all bands to 1e-9 Ha (a non-symmetric
+  Hsub or a wrong rotation breaks that), and the Rayleigh quotients of
+  sampled bands by the host-array hpsi agree;
+- two SCF steps in FP64 and in mixed precision give total energies within
+  1e-6 Ha/atom;
+- Si216: one CG call with the learned preconditioner (pc_kind 5) and with
+  the hybrid (pc_kind 6) lowers every band energy, with the descent-check
+  fallbacks printed.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from rsdft_develop_amd import systems
+
+pytestmark = pytest.mark.gpu
+
+NBANDS = {3: 480, 4: 1100}
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _cell(nrep):
+    return systems.silicon_supercell(nrep, nband=NBANDS[nrep], displacement=0.05, seed=0)
+
+
+@pytest.fixture(scope="module", params=[3, 4], ids=["si216", "si512"])
+def large(request, orc):
+    from rsdft_develop_amd import gpu
+
+    nrep = request.param
+    cell = _cell(nrep)
+    print(f"case: Si{cell.natom} {cell.ngrid[0]}^3 x {cell.nband} bands, set-up", flush=True)
+    vloc = systems.model_vloc(cell)
+    ctx = gpu.Context().setup(cell, vloc)
+    ham = orc.Ham(cell, vloc=vloc)
+    yield cell, ctx, ham, vloc
+    ctx.close()
+
+
+def _hpsi(ctx, psi, dtype):
+    from rsdft_develop_amd import hip
+
+    N, nb = psi.shape
+    d_in = hip.DeviceArray.from_numpy(psi, dtype=dtype)
+    d_out = hip.DeviceArray((N, nb), dtype=dtype)
+    ctx.hpsi_device(d_in.ptr, d_out.ptr, N, nb, dtype=dtype)
+    ctx.synchronize()
+    out = d_out.to_numpy().astype(np.float64)
+    d_in.free()
+    d_out.free()
+    return out
+
+
+def test_hpsi_sampled_bands_match_oracle(large):
+    cell, ctx, ham, _ = large
+    psi = systems.random_bands(cell.npoints, 64, seed=7)
+    sample = [0, 21, 42, 63]
+    want = ham.hpsi(psi[:, sample])
+    for dt, rel in ((np.float64, 1e-12), (np.float32, 2e-5)):
+        print(f"case: Si{cell.natom} hpsi {np.dtype(dt).name}", flush=True)
+        got = _hpsi(ctx, psi, dt)[:, sample]
+        err = np.abs(got - want).max() / np.abs(want).max()
+        assert err <= rel, err
+
+
+def test_gram_schmidt_and_subspace_diag_whole_store(large):
+    cell, ctx, ham, _ = large
+    nb = cell.nband
+    sample = [0, 1, nb // 3, nb // 2, nb - 2, nb - 1]
+    print(f"case: Si{cell.natom} Gram-Schmidt of {nb} bands", flush=True)
+    ctx.alloc_bands(nb)
+    ctx.random_bands(seed=3)
+    ctx.gram_schmidt(1, nb)
+    u1 = np.column_stack([ctx.download_bands(b, 1)[:, 0] for b in sample])
+    s = u1.T @ u1 * ham.dv
+    assert np.abs(s - np.eye(len(sample))).max() < 1e-12
+    ctx.gram_schmidt(1, nb)
+    u2 = np.column_stack([ctx.download_bands(b, 1)[:, 0] for b in sample])
+    assert np.abs(u2 - u1).max() < 1e-11 * np.abs(u1).max()
+    print(f"case: Si{cell.natom} subspace_diag of {nb} bands", flush=True)
+    esp = ctx.subspace_diag()
+    assert np.all(np.diff(esp) >= 0)
+    np.testing.assert_allclose(ctx.esp_calc(), esp, rtol=0, atol=1e-9)
+    u = np.column_stack([ctx.download_bands(b, 1)[:, 0] for b in sample])
+    hu = ctx.hamiltonian(u)
+    rq = np.sum(u * hu, axis=0) * ham.dv
+    np.testing.assert_allclose(rq, esp[sample], rtol=0, atol=1e-9)
+
+
+def test_scf_steps_mixed_vs_fp64(large):
+    from rsdft_develop_amd import gpu, scf
+
+    cell, ctx0, _, _ = large
+    etot = {}
+    for mixed in (False, True):
+        print(f"case: Si{cell.natom} two SCF steps {'mixed' if mixed else 'fp64'}", flush=True)
+        ctx = gpu.Context()
+        try:
+            run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=0.0, max_iter=2, mixed=mixed, imix=30,
+                                              fp64_below=0.0, start="uniform"), ctx=ctx)
+            res = run.run()
+            etot[mixed] = [r.etot for r in res.iterations]
+            assert [r.precision for r in res.iterations] == ([32, 32] if mixed else [64, 64])
+        finally:
+            ctx.close()
+    diff = [abs(a - b) / cell.natom for a, b in zip(etot[True], etot[False])]
+    print(f"mixed vs fp64 Etot difference per atom after each step: {diff}", flush=True)
+    assert max(diff) < 1e-6
+
+
+@pytest.mark.parametrize("pc_kind", [5, 6])
+def test_si216_cg_with_learned_preconditioner_lowers_energies(pc_kind):
+    from rsdft_develop_amd import gpu, lpc, scf
+
+    cell = _cell(3)
+    weights = os.path.join(ROOT, "rsdft_develop_amd", "data", "lpc_si64.npz" if pc_kind == 5 else
+                           "lpc_si64_hybrid.npz")
+    print(f"case: Si216 CG pc_kind {pc_kind}", flush=True)
+    ctx = gpu.Context()
+    try:
+        run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, imix=30, pc_kind=pc_kind, start="uniform"), ctx=ctx)
+        ctx.lpc_set_params(lpc.load(weights))
+        if pc_kind == 6:
+            ctx.lpc_set_hybrid_scale(lpc.load_hybrid_scale(weights))
+        ctx.lpc_set_potential()
+        e0 = ctx.subspace_diag()
+        before = ctx.lpc_fallbacks()
+        for prec, slack in ((64, 1e-10), (32, 2e-5)):  # FP32 hpsi rounds the energies at ~1e-6 relative
+            e1, res = ctx.conjugate_gradient(ncg=4, pc_kind=pc_kind, mloop=3, precision=prec)
+            assert np.all(np.isfinite(e1)) and np.all(np.isfinite(res))
+            assert np.all(e1 <= e0 + slack), np.max(e1 - e0)
+            print(f"pc_kind {pc_kind} FP{prec}: mean energy drop {np.mean(e0 - e1):.4e} Ha, "
+                  f"descent-check fallbacks so far {ctx.lpc_fallbacks() - before}", flush=True)
+            e0 = ctx.esp_calc()
+        del run
+    finally:
+        ctx.close()

@@ -315,3 +315,31 @@ def test_conjugate_gradient_keeps_a_band_with_zero_gradient(orc):
                     np.testing.assert_allclose(esp, esp_want, atol=1e-10)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_conjugate_gradient_graph_replays_equal_direct_launches(setup, precision):
+    """CG band blocks replayed from captured HIP graphs give the direct
+    launches' bands, energies and residuals bit for bit, on every replay,
+    with the program's own eager work (uploads, hpsi, esp_calc) and
+    synchronisations between the replays."""
+    cell, ham, ctx, u0 = setup
+    q = ham.gram_schmidt(u0, "lusl")
+    u1, _ = ham.subspace_diag(q)
+    _load(ctx, u1)
+    esp0, res0 = ctx.conjugate_gradient(ncg=4, pc_kind=4, mloop=3, precision=precision)
+    want = ctx.download_bands()
+    n0 = ctx.cg_graph_launches()
+    try:
+        ctx.set_cg_graphs(True)
+        for rep in range(6):
+            _load(ctx, u1)
+            ctx.hamiltonian(u0[:, :3])  # eager work between replays
+            esp, res = ctx.conjugate_gradient(ncg=4, pc_kind=4, mloop=3, precision=precision)
+            np.testing.assert_array_equal(esp, esp0)
+            np.testing.assert_array_equal(res, res0)
+            np.testing.assert_array_equal(ctx.download_bands(), want)
+            ctx.esp_calc()
+        assert ctx.cg_graph_launches() - n0 >= 3  # captured once, then replayed
+    finally:
+        ctx.set_cg_graphs(False)
