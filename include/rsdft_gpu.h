@@ -101,6 +101,19 @@ int rsg_get_projectors(rsg_ctx *ctx, int *mjj, int *jjp, double *uvk,
  * not overlap.  Asynchronous on the context stream. */
 int rsg_hpsi_f64(rsg_ctx *ctx, const double *d_psi, double *d_hpsi,
                  int64_t ld, int nb);
+/* rsg_hpsi_f64 that allocates nothing: no hipMalloc, hipFree or allocating
+ * library call, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  No counterpart
+ * in the reference.  The nonlocal projections go to a fixed 16 MiB device
+ * scratch array (one per GPU, held until the process ends) instead of
+ * context buffers; the bands run in chunks small enough to fit, and a call
+ * whose single band's projections exceed the scratch fails with
+ * RSG_ERR_ARG.  Results equal rsg_hpsi_f64's.  The scratch holds the last
+ * call's projections (every call writes what it reads); calls that overlap
+ * on one GPU, from any context, race on it.  The context must be fully set
+ * up (grid, Vloc, projectors and nonlocal mode) beforehand. */
+int rsg_hpsi_f64_noalloc(rsg_ctx *ctx, const double *d_psi, double *d_hpsi,
+                         int64_t ld, int nb);
 /* The same operator in FP32 storage with FP32 accumulation (tables rounded
  * to FP32).  No reference counterpart (mixed-precision addition). */
 int rsg_hpsi_f32(rsg_ctx *ctx, const float *d_psi, float *d_hpsi, int64_t ld,

@@ -1,6 +1,7 @@
 // capi.cpp -- the extern "C" entry points declared in include/rsdft_gpu.h.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "context.hpp"
@@ -8,6 +9,40 @@
 #include "solver.hpp"
 
 using rsg::DevBuf;
+
+namespace {
+
+// Device scratch of rsg_hpsi_f64_noalloc: the partial and reduced nonlocal
+// projections of a chunk of bands (16 MiB; the chunk shrinks to fit).  Every
+// GPU the library runs on holds its own copy until the process ends, and
+// every call on that GPU shares it.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in for the symbol): it
+// uses the device address resolved once per device at context creation.
+constexpr size_t kNoallocScratchBytes = size_t(16) << 20;
+__device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: resolve (once) the address of this device's copy.
+int resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return RSG_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return RSG_OK;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return RSG_ERR_HIP;
+  g_scratch_ptr[device] = p;
+  return RSG_OK;
+}
+
+void *scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -24,7 +59,8 @@ int rsg_create(rsg_ctx **out, int device) {
     delete ctx;
     return RSG_ERR_HIP;
   }
-  if (ctx->dead.ensure(64 * sizeof(double)) != hipSuccess) {
+  if (ctx->dead.ensure(64 * sizeof(double)) != hipSuccess || resolve_scratch(device) != RSG_OK) {
+    ctx->dead.release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RSG_ERR_HIP;
@@ -302,6 +338,15 @@ int rsg_hpsi_f64(rsg_ctx *ctx, const double *psi, double *hpsi, int64_t ld, int 
   if (!ctx) return RSG_ERR_ARG;
   RSG_HIP(hipSetDevice(ctx->device));
   return rsg::launch_hpsi<double>(ctx, psi, hpsi, ld, nb);
+}
+
+int rsg_hpsi_f64_noalloc(rsg_ctx *ctx, const double *psi, double *hpsi, int64_t ld, int nb) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  void *p = scratch_of(ctx->device);
+  if (!p) return ctx->fail(RSG_ERR_STATE, "hpsi_f64_noalloc: no scratch address for this device");
+  const rsg::HpsiScratch s{p, kNoallocScratchBytes};
+  return rsg::launch_hpsi<double>(ctx, psi, hpsi, ld, nb, &s);
 }
 
 int rsg_hpsi_f32(rsg_ctx *ctx, const float *psi, float *hpsi, int64_t ld, int nb) {

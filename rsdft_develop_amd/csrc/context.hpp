@@ -256,8 +256,15 @@ int upload_analytic(rsg_ctx *ctx, const HostAnalytic &h);
 
 // hpsi.hip
 namespace rsg {
+// Caller-provided device scratch for the projections of one hpsi call:
+// with it, launch_hpsi allocates nothing and runs the bands in chunks small
+// enough for the partial and reduced projections to fit in `bytes`.
+struct HpsiScratch {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
 template <typename T>
-int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb);
+int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const HpsiScratch *scr = nullptr);
 template <typename T>
 int launch_kinetic(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, int accumulate);
 int resolved_shape(rsg_ctx *ctx, int precision, int nb, HpsiShape *out);

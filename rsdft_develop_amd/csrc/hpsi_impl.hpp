@@ -955,23 +955,50 @@ int launch_kinetic(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, int 
   return RSG_OK;
 }
 
+// The two projection buffers of a chunk of bands: `nred` reduced and `npart`
+// partial values per band.  From the caller's scratch (the chunk shrinks
+// until both fit; RSG_ERR_ARG when one band does not) or from the
+// context's own buffers, grown on demand.
 template <typename T>
-int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb) {
+int projection_buffers(rsg_ctx *ctx, const HpsiScratch *scr, size_t nred, size_t npart, HpsiShape &sh,
+                       T **red, T **part) {
+  if (scr) {
+    const size_t per_band = (nred + npart) * sizeof(T) + 256;
+    if (per_band > scr->bytes)
+      return ctx->fail(RSG_ERR_ARG, "hpsi: the projections of one band need " + std::to_string(per_band) +
+                                        " bytes of scratch, more than the " + std::to_string(scr->bytes) +
+                                        " available");
+    const int fit = int(std::min<size_t>(65535, (scr->bytes - 256) / ((nred + npart) * sizeof(T))));
+    if (sh.chunk > fit) sh.chunk = fit;
+    const uintptr_t base = reinterpret_cast<uintptr_t>(scr->p);
+    *red = reinterpret_cast<T *>(base);
+    *part = reinterpret_cast<T *>((base + nred * sh.chunk * sizeof(T) + 255) & ~uintptr_t(255));
+    return RSG_OK;
+  }
+  if (ctx->cproj.ensure(nred * sh.chunk * sizeof(T)) != hipSuccess ||
+      ctx->uvpart.ensure(npart * sh.chunk * sizeof(T)) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "hpsi: projection scratch allocation failed");
+  *red = ctx->cproj.as<T>();
+  *part = ctx->uvpart.as<T>();
+  return RSG_OK;
+}
+
+template <typename T>
+int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const HpsiScratch *scr) {
   if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "hpsi: grid not set");
   if (nb < 0 || ld < ctx->N) return ctx->fail(RSG_ERR_ARG, "hpsi: bad nb or ld");
   if (nb == 0) return RSG_OK;
   if (nb > 65535) return ctx->fail(RSG_ERR_ARG, "hpsi: nb above 65535");
   if (!psi || !hpsi) return ctx->fail(RSG_ERR_ARG, "hpsi: null buffer");
-  const HpsiShape sh = resolve_shape(ctx, shape_of<T>(ctx), sizeof(T), nb, ctx->have_nl);
+  HpsiShape sh = resolve_shape(ctx, shape_of<T>(ctx), sizeof(T), nb, ctx->have_nl);
   if (!ctx->have_nl) {
     int rc = launch_stencil<T>(ctx, psi, hpsi, ld, nb, sh, 0);
     if (rc) return rc;
   } else if (ctx->nl_mode != 1 && ctx->have_ana) {
     AnalyticDev<T> &an = ana_of<T>(ctx);
-    if (ctx->cproj.ensure(size_t(an.ngroup) * 4 * sh.chunk * sizeof(T)) != hipSuccess ||
-        ctx->uvpart.ensure(size_t(an.nseg) * 4 * sh.chunk * sizeof(T)) != hipSuccess)
-      return ctx->fail(RSG_ERR_HIP, "hpsi: projection scratch allocation failed");
-    T *cp = ctx->cproj.as<T>(), *part = ctx->uvpart.as<T>();
+    T *cp = nullptr, *part = nullptr;
+    int rc0 = projection_buffers<T>(ctx, scr, size_t(an.ngroup) * 4, size_t(an.nseg) * 4, sh, &cp, &part);
+    if (rc0) return rc0;
     for (int c0 = 0; c0 < nb; c0 += sh.chunk) {
       const int nbc = std::min(sh.chunk, nb - c0);
       const T *pc = psi + int64_t(c0) * ld;
@@ -1005,10 +1032,17 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb) {
     NonlocalDev<T> &nl = nl_of<T>(ctx);
     const int nz = ctx->tables.nzlma;
     constexpr int BB = 8;
-    if (ctx->uvunk.ensure(size_t(nz) * sh.chunk * sizeof(T)) != hipSuccess ||
-        ctx->uvpart.ensure(size_t(nl.nseg) * 4 * sh.chunk * sizeof(T)) != hipSuccess)
-      return ctx->fail(RSG_ERR_HIP, "hpsi: projection scratch allocation failed");
-    T *uv = ctx->uvunk.as<T>(), *part = ctx->uvpart.as<T>();
+    T *uv = nullptr, *part = nullptr;
+    if (scr) {
+      int rc0 = projection_buffers<T>(ctx, scr, size_t(nz), size_t(nl.nseg) * 4, sh, &uv, &part);
+      if (rc0) return rc0;
+    } else {
+      if (ctx->uvunk.ensure(size_t(nz) * sh.chunk * sizeof(T)) != hipSuccess ||
+          ctx->uvpart.ensure(size_t(nl.nseg) * 4 * sh.chunk * sizeof(T)) != hipSuccess)
+        return ctx->fail(RSG_ERR_HIP, "hpsi: projection scratch allocation failed");
+      uv = ctx->uvunk.as<T>();
+      part = ctx->uvpart.as<T>();
+    }
     for (int c0 = 0; c0 < nb; c0 += sh.chunk) {
       const int nbc = std::min(sh.chunk, nb - c0);
       const T *pc = psi + int64_t(c0) * ld;
@@ -1193,7 +1227,8 @@ int autotune_hpsi(rsg_ctx *ctx, int precision, int nb, int reps, float *best_ms)
 
 #endif  // RSG_HPSI_TYPE_IS_DOUBLE
 
-template int launch_hpsi<RSG_HPSI_TYPE>(rsg_ctx *, const RSG_HPSI_TYPE *, RSG_HPSI_TYPE *, int64_t, int);
+template int launch_hpsi<RSG_HPSI_TYPE>(rsg_ctx *, const RSG_HPSI_TYPE *, RSG_HPSI_TYPE *, int64_t, int,
+                                        const HpsiScratch *);
 template int launch_kinetic<RSG_HPSI_TYPE>(rsg_ctx *, const RSG_HPSI_TYPE *, RSG_HPSI_TYPE *, int64_t, int, int);
 
 }  // namespace rsg

@@ -71,6 +71,7 @@ def load_library():
         "rsg_get_projectors": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
         "rsg_hpsi_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
         "rsg_hpsi_f32": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
+        "rsg_hpsi_f64_noalloc": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int]),
         "rsg_random_bands": (_c_int, [_vp, _c_int, _c_int, ctypes.c_uint64]),
         "rsg_op_kinetic_f64": (_c_int, [_vp, _vp, _vp, ctypes.c_int64, _c_int, _c_int]),
         "rsg_op_kinetic_f32": (_c_int, [_vp, _vp, _vp, ctypes.c_int64, _c_int, _c_int]),
