@@ -282,8 +282,27 @@ class PulayG:
         return to_flat(np.real(np.fft.ifftn(self.xin[m - 1])))
 
 
+def refine_levels(ham, u, ncg=4, mloop=3, gs_variant="t", tol=1e-11, max_passes=60):
+    """Band energies of the final Hamiltonian converged at a fixed potential:
+    passes of subspace_diag, conjugate_gradient, gram_schmidt and esp_calc
+    (the eigensolver half of one SCF iteration, scf_module.f90:231-268)
+    until no band energy moves by more than tol.  Returns (u, esp, passes)."""
+    esp = ham.esp_calc(u)
+    for k in range(1, max_passes + 1):
+        u, _ = ham.subspace_diag(u)
+        u, _, _ = ham.conjugate_gradient(u, ncg=ncg, mloop=mloop)
+        u = ham.gram_schmidt(u, gs_variant)
+        e = ham.esp_calc(u)
+        moved = float(np.abs(e - esp).max())
+        esp = e
+        if moved < tol:
+            return u, esp, k
+    return u, esp, max_passes
+
+
 def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-8, max_iter=200,
-            init_width=1.0, seed=0, gs_variant="t", bands=None, callback=None, imix=10, start="normal"):
+            init_width=1.0, seed=0, gs_variant="t", bands=None, callback=None, imix=10, start="normal",
+            refine_tol=None):
     """Plain FP64 SCF in the reference's order; returns a dict of results.
 
     Stops when the Kohn-Sham energy of the output density changes by less
@@ -332,8 +351,12 @@ def run_scf(cell, ncg=4, mloop=3, mmix=4, beta=0.3, ekbt=1e-5, etol_per_atom=1e-
         if callback:
             callback(hist[-1])
         if de < etol_per_atom or it == max_iter:
-            return dict(converged=de < etol_per_atom, etot=etot, fene=etot - ent, esp=esp, occ=occ,
-                        history=hist, eewald=eew, bands=u, rho=rho_out)
+            out = dict(converged=de < etol_per_atom, etot=etot, fene=etot - ent, esp=esp, occ=occ,
+                       history=hist, eewald=eew, bands=u, rho=rho_out, rho_in=rho)
+            if refine_tol is not None:
+                _, out["esp_refined"], out["refine_passes"] = refine_levels(ham, u, ncg, mloop, gs_variant,
+                                                                            refine_tol)
+            return out
         e_prev = etot
         rho = mixer.mix(rho_out)
         rho = rho * nel / (np.sum(rho) * cell.dv)

@@ -1,7 +1,7 @@
 // scf.hip -- the density/potential half of an SCF step on the device:
 // GTH local potential (G-space form factor + hipFFT), density from the band
 // store, FFT Hartree, LDA-PZ81, Pulay mixing of the density, and the
-// energy integrals; Ewald energy on the host.  FP64 throughout.
+// energy integrals and the Ewald energy.  FP64 throughout.
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
@@ -34,6 +34,7 @@ struct ScfState {
   // atom positions, kinds and species parameters of the local potential
   // (kept: a free would wait for the whole device)
   DevBuf dfrac, dkind, dsp;
+  DevBuf ewpart;  // Ewald terms before their ordered sums
   // GTH local parameters per species
   std::vector<double> zps, rloc, cloc;
   std::vector<double> frac;
@@ -394,13 +395,87 @@ int dot1(rsg_ctx *ctx, const double *x, const double *y, double alpha, double *o
 
 // Ewald energy of point ions in a neutralising background (the lattice sum
 // cal_ewald computes, ewald_module.f90:50-62), by the standard split into
-// real-space erfc and reciprocal Gaussian sums.
-double ewald_energy(const double *aa, const double *bb, double vol, int natom, const double *frac,
-                    const double *z) {
+// real-space erfc and reciprocal Gaussian sums, evaluated on the device: one
+// atom pair per thread over its images, one reciprocal vector per thread over
+// the atoms, each set of terms summed in a fixed order (k_sum_ordered), so
+// the result does not vary from run to run.
+struct EwaldArgs {
+  double aa[9], bb[9];
+  double se, rcut, gcut2;
+  int natom, nr[3], ng[3];
+};
+
+__global__ void k_ewald_real(EwaldArgs e, const double *__restrict__ frac, const int *__restrict__ kind,
+                             const double *__restrict__ sp, double *__restrict__ part) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= int64_t(e.natom) * e.natom) return;
+  const int a = int(t / e.natom), b = int(t - int64_t(a) * e.natom);
+  const double f0 = frac[3 * b] - frac[3 * a], f1 = frac[3 * b + 1] - frac[3 * a + 1],
+               f2 = frac[3 * b + 2] - frac[3 * a + 2];
+  double s = 0.0;
+  for (int l1 = -e.nr[0]; l1 <= e.nr[0]; ++l1)
+    for (int l2 = -e.nr[1]; l2 <= e.nr[1]; ++l2)
+      for (int l3 = -e.nr[2]; l3 <= e.nr[2]; ++l3) {
+        if (a == b && l1 == 0 && l2 == 0 && l3 == 0) continue;
+        const double d0 = f0 + l1, d1 = f1 + l2, d2 = f2 + l3;
+        const double x = e.aa[0] * d0 + e.aa[3] * d1 + e.aa[6] * d2;
+        const double y = e.aa[1] * d0 + e.aa[4] * d1 + e.aa[7] * d2;
+        const double z = e.aa[2] * d0 + e.aa[5] * d1 + e.aa[8] * d2;
+        const double r = sqrt(x * x + y * y + z * z);
+        if (r < e.rcut) s += erfc(e.se * r) / r;
+      }
+  part[t] = sp[6 * kind[a]] * sp[6 * kind[b]] * s;
+}
+
+// exp(-G^2 / 4 eta) / G^2 |sum_a Z_a exp(i G.R_a)|^2 for one reciprocal
+// vector G = m1 b1 + m2 b2 + m3 b3 of the box |m_k| <= ng_k
+__global__ void k_ewald_recip(EwaldArgs e, const double *__restrict__ frac, const int *__restrict__ kind,
+                              const double *__restrict__ sp, double inv4eta, double *__restrict__ part) {
+  const int w1 = 2 * e.ng[0] + 1, w2 = 2 * e.ng[1] + 1, w3 = 2 * e.ng[2] + 1;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= int64_t(w1) * w2 * w3) return;
+  const int m1 = int(t % w1) - e.ng[0], m2 = int((t / w1) % w2) - e.ng[1], m3 = int(t / (int64_t(w1) * w2)) - e.ng[2];
+  const double gx = e.bb[0] * m1 + e.bb[3] * m2 + e.bb[6] * m3;
+  const double gy = e.bb[1] * m1 + e.bb[4] * m2 + e.bb[7] * m3;
+  const double gz = e.bb[2] * m1 + e.bb[5] * m2 + e.bb[8] * m3;
+  const double g2 = gx * gx + gy * gy + gz * gz;
+  double v = 0.0;
+  if ((m1 | m2 | m3) != 0 && g2 <= e.gcut2) {
+    double sr = 0.0, si = 0.0;
+    for (int a = 0; a < e.natom; ++a) {
+      const double ph = 2.0 * kPi * (m1 * frac[3 * a] + m2 * frac[3 * a + 1] + m3 * frac[3 * a + 2]);
+      double sn, cs;
+      sincos(ph, &sn, &cs);
+      const double za = sp[6 * kind[a]];
+      sr += za * cs;
+      si += za * sn;
+    }
+    v = exp(-g2 * inv4eta) / g2 * (sr * sr + si * si);
+  }
+  part[t] = v;
+}
+
+// out[0] = sum of n values in a fixed order: one block, strided partial
+// sums, then a shared-memory tree
+__global__ __launch_bounds__(1024) void k_sum_ordered(const double *__restrict__ v, int64_t n,
+                                                      double *__restrict__ out) {
+  __shared__ double sh[1024];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) s += v[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+int ewald_energy(rsg_ctx *ctx, ScfState *st, double vol, int natom, const double *zat, double *energy) {
   double q = 0.0, q2 = 0.0;
   for (int a = 0; a < natom; ++a) {
-    q += z[a];
-    q2 += z[a] * z[a];
+    q += zat[a];
+    q2 += zat[a] * zat[a];
   }
   // Gaussian split parameter.  The energy does not depend on it; eta = pi
   // natom^(1/3) / vol^(2/3) balances the real-space pair-image sum against
@@ -408,50 +483,40 @@ double ewald_energy(const double *aa, const double *bb, double vol, int natom, c
   // with the cell: 13^3 image cells per pair at Si216, and its large
   // background term cancels to 1e-7 Ha there in double precision).
   const double eta = kPi * std::cbrt(double(std::max(natom, 1))) / std::pow(vol, 2.0 / 3.0);
-  const double se = std::sqrt(eta);
-  const double rcut = 8.0 / se, gcut = 16.0 * se;
-  auto len = [](const double *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
-  int nr[3], ng[3];
-  for (int k = 0; k < 3; ++k) {
-    nr[k] = int(std::ceil(rcut * len(bb + 3 * k) / (2 * kPi))) + 1;
-    ng[k] = int(std::ceil(gcut / len(bb + 3 * k))) + 1;
+  EwaldArgs e{};
+  for (int k = 0; k < 9; ++k) {
+    e.aa[k] = ctx->aa[k];
+    e.bb[k] = st->bb[k];
   }
-  double er = 0.0;
-  for (int a = 0; a < natom; ++a)
-    for (int b = 0; b < natom; ++b)
-      for (int l1 = -nr[0]; l1 <= nr[0]; ++l1)
-        for (int l2 = -nr[1]; l2 <= nr[1]; ++l2)
-          for (int l3 = -nr[2]; l3 <= nr[2]; ++l3) {
-            if (a == b && l1 == 0 && l2 == 0 && l3 == 0) continue;
-            const double d[3] = {frac[3 * b] - frac[3 * a] + l1, frac[3 * b + 1] - frac[3 * a + 1] + l2,
-                                 frac[3 * b + 2] - frac[3 * a + 2] + l3};
-            const double x = aa[0] * d[0] + aa[3] * d[1] + aa[6] * d[2];
-            const double y = aa[1] * d[0] + aa[4] * d[1] + aa[7] * d[2];
-            const double zz = aa[2] * d[0] + aa[5] * d[1] + aa[8] * d[2];
-            const double r = std::sqrt(x * x + y * y + zz * zz);
-            if (r < rcut) er += z[a] * z[b] * std::erfc(se * r) / r;
-          }
-  er *= 0.5;
-  double eg = 0.0;
-  for (int m1 = -ng[0]; m1 <= ng[0]; ++m1)
-    for (int m2 = -ng[1]; m2 <= ng[1]; ++m2)
-      for (int m3 = -ng[2]; m3 <= ng[2]; ++m3) {
-        if (m1 == 0 && m2 == 0 && m3 == 0) continue;
-        const double gx = bb[0] * m1 + bb[3] * m2 + bb[6] * m3;
-        const double gy = bb[1] * m1 + bb[4] * m2 + bb[7] * m3;
-        const double gz = bb[2] * m1 + bb[5] * m2 + bb[8] * m3;
-        const double g2 = gx * gx + gy * gy + gz * gz;
-        if (g2 > gcut * gcut) continue;
-        double sr = 0.0, si = 0.0;
-        for (int a = 0; a < natom; ++a) {
-          const double ph = 2.0 * kPi * (m1 * frac[3 * a] + m2 * frac[3 * a + 1] + m3 * frac[3 * a + 2]);
-          sr += z[a] * std::cos(ph);
-          si += z[a] * std::sin(ph);
-        }
-        eg += std::exp(-g2 / (4.0 * eta)) / g2 * (sr * sr + si * si);
-      }
-  eg *= 2.0 * kPi / vol;
-  return er + eg - std::sqrt(eta / kPi) * q2 - kPi * q * q / (2.0 * eta * vol);
+  e.se = std::sqrt(eta);
+  e.rcut = 8.0 / e.se;
+  const double gcut = 16.0 * e.se;
+  e.gcut2 = gcut * gcut;
+  e.natom = natom;
+  auto len = [](const double *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+  for (int k = 0; k < 3; ++k) {
+    e.nr[k] = int(std::ceil(e.rcut * len(st->bb + 3 * k) / (2 * kPi))) + 1;
+    e.ng[k] = int(std::ceil(gcut / len(st->bb + 3 * k))) + 1;
+  }
+  const int64_t npair = int64_t(natom) * natom;
+  const int64_t ngv = int64_t(2 * e.ng[0] + 1) * (2 * e.ng[1] + 1) * (2 * e.ng[2] + 1);
+  if (st->ewpart.ensure(size_t(std::max(npair, ngv) + 2) * 8) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "ewald: allocation failed");
+  double *part = st->ewpart.as<double>(), *sums = part + std::max(npair, ngv);
+  double h[2] = {0.0, 0.0};
+  if (natom > 0) {
+    hipLaunchKernelGGL(k_ewald_real, dim3(nblk(npair)), dim3(kT), 0, ctx->stream, e, st->dfrac.as<double>(),
+                       st->dkind.as<int>(), st->dsp.as<double>(), part);
+    hipLaunchKernelGGL(k_sum_ordered, dim3(1), dim3(1024), 0, ctx->stream, part, npair, sums);
+    hipLaunchKernelGGL(k_ewald_recip, dim3(nblk(ngv)), dim3(kT), 0, ctx->stream, e, st->dfrac.as<double>(),
+                       st->dkind.as<int>(), st->dsp.as<double>(), 1.0 / (4.0 * eta), part);
+    hipLaunchKernelGGL(k_sum_ordered, dim3(1), dim3(1024), 0, ctx->stream, part, ngv, sums + 1);
+    RSG_HIP(hipGetLastError());
+    RSG_HIP(hipMemcpyAsync(h, sums, 16, hipMemcpyDeviceToHost, ctx->stream));
+    RSG_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  *energy = 0.5 * h[0] + 2.0 * kPi / vol * h[1] - std::sqrt(eta / kPi) * q2 - kPi * q * q / (2.0 * eta * vol);
+  return RSG_OK;
 }
 
 // small dense solve A x = b (partial pivoting), in place
@@ -521,7 +586,7 @@ int scf_set_local_gth(rsg_ctx *ctx, int natom, const double *frac, const int *ki
     zat[size_t(a)] = zps[kind[a]];
     st->nelectron += zat[size_t(a)];
   }
-  st->eewald = ewald_energy(ctx->aa, st->bb, vol, natom, frac, zat.data());
+  if ((rc = ewald_energy(ctx, st, vol, natom, zat.data(), &st->eewald))) return rc;
   st->frac.assign(frac, frac + 3 * natom);
   st->kind.assign(kind, kind + natom);
   st->zps.assign(zps, zps + nkind);
@@ -957,7 +1022,8 @@ void scf_release(rsg_ctx *ctx) {
   for (auto &b : st->xou) b.release();
   for (auto &b : st->gxin) b.release();
   for (auto &b : st->gxou) b.release();
-  for (DevBuf *b : {&st->res, &st->wq, &st->wq2, &st->gpart, &st->dfrac, &st->dkind, &st->dsp}) b->release();
+  for (DevBuf *b : {&st->res, &st->wq, &st->wq2, &st->gpart, &st->dfrac, &st->dkind, &st->dsp, &st->ewpart})
+    b->release();
   delete st;
   ctx->scf = nullptr;
 }

@@ -263,6 +263,14 @@ __global__ void k_axpby(W *__restrict__ y, const W *__restrict__ x, W a, W b, in
   if (k < n) y[k] = a * y[k] + b * x[k];
 }
 
+// y = x (n values): a kernel rather than a copy node, so that a captured
+// CG block holds kernels only
+template <typename W>
+__global__ void k_copy(W *__restrict__ y, const W *__restrict__ x, int64_t n) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < n) y[k] = x[k];
+}
+
 // x32 = (W) x for an nn-band block
 template <typename W>
 __global__ void k_to_work(const double *__restrict__ x, W *__restrict__ x32, int64_t total) {
@@ -1445,7 +1453,9 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
         if (pc_kind == 5) {
           if ((rc = lpc_apply(ctx, gk, pgk, N, nn, kMixed ? 32 : 64))) return rc;
         } else {
-          RSG_HIP(hipMemcpyAsync(pgk, gk, size_t(N) * nn * sizeof(W), hipMemcpyDeviceToDevice, ctx->stream));
+          const int64_t tot = N * nn;
+          hipLaunchKernelGGL((k_copy<W>), dim3(unsigned((tot + kThreads - 1) / kThreads)), blkd, 0, ctx->stream, pgk,
+                             gk, tot);
         }
         const W *xs[1] = {pgk}, *ys[1] = {gk};
         if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
@@ -1475,7 +1485,7 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
         if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
       }
       hipLaunchKernelGGL((k_cg_dir<W>), gN, blkd, 0, ctx->stream, pgk, pk, s.rb, s.gkgk, icg == 1 ? 1 : 0, N);
-      RSG_HIP(hipMemcpyAsync(s.gkgk, s.rb, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+      hipLaunchKernelGGL(k_scale_copy, gb, b64, 0, ctx->stream, s.rb, 1.0, s.gkgk, nn);
       if ((rc = launch_hpsi<W>(ctx, pk, hpk, N, nn))) return rc;
       if ((rc = ritz_dots<W>(ctx, x, pk, hxk, hpk, nn, dv, s.d6))) return rc;
       hipLaunchKernelGGL(k_cg_ritz, gb, b64, 0, ctx->stream, s.d6, nn, s.E, s.E1, s.uc, s.flags);

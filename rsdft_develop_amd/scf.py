@@ -222,6 +222,27 @@ class SCF:
             c.lpc_set_potential()
         return time.perf_counter() - t0
 
+    def refine_levels(self, tol: float = 1e-11, max_passes: int = 60, precision: int = 64) -> tuple:
+        """Band energies of the final Hamiltonian converged at a fixed
+        potential: passes of subspace_diag, conjugate_gradient, gram_schmidt
+        and esp_calc (the eigensolver half of an SCF iteration,
+        scf_module.f90:231-268) until no band energy moves by more than tol.
+        The SCF's stop test bounds the energy change only; members of a
+        degenerate shell stay split by the eigenvectors' remaining error.
+        Returns (esp, passes)."""
+        c, p = self.ctx, self.p
+        esp = c.esp_calc()
+        for k in range(1, max_passes + 1):
+            c.subspace_diag()
+            c.conjugate_gradient(p.ncg, p.pc_kind, p.mloop, precision=precision)
+            c.gram_schmidt()
+            e = c.esp_calc()
+            moved = float(np.abs(e - esp).max())
+            esp = e
+            if moved < tol:
+                return esp, k
+        return esp, max_passes
+
     def run(self, callback=None) -> ScfResult:
         its = []
         e_prev = None

@@ -9,7 +9,7 @@ def shell_means(e, gap=1e-4):
     return np.array([s.mean() for s in np.split(e, cuts)]), np.diff(np.r_[0, cuts, e.size])
 
 
-def assert_levels_match(got, want, shell_tol=1e-8, level_tol=2e-6):
+def assert_levels_match(got, want, shell_tol=1e-8, level_tol=1e-8):
     """Shell means (invariant under rotations inside a degenerate shell) to
     shell_tol, single sorted levels to level_tol, same shell structure."""
     mg, ng = shell_means(got)

@@ -8,8 +8,9 @@ depend on the oracle's cost at this size:
 - hpsi of sampled bands (inside a 64-band call) against the oracle, 1e-12
   of max|H psi| in FP64 and 2e-5 in FP32;
 - Gram-Schmidt of the whole device-drawn store: sampled bands orthonormal to
-  1e-12, and a second Gram-Schmidt leaves the store unchanged to 1e-11
-  (it would move every band by about |dV U^T U - I| otherwise);
+  1e-12, and a second Gram-Schmidt leaves the store unchanged to 1e-10 of
+  its largest value (rounding: 1.0e-11 at Si512; without the first it would
+  move every band by about |dV U^T U - I|);
 - subspace_diag of the whole store: eigenvalues ascending and equal to
   esp_calc's dV <u_n|H u_n> over all bands to 1e-9 Ha (a non-symmetric
   Hsub or a wrong rotation breaks that), and the Rayleigh quotients of
@@ -90,7 +91,7 @@ def test_gram_schmidt_and_subspace_diag_whole_store(large):
     assert np.abs(s - np.eye(len(sample))).max() < 1e-12
     ctx.gram_schmidt(1, nb)
     u2 = np.column_stack([ctx.download_bands(b, 1)[:, 0] for b in sample])
-    assert np.abs(u2 - u1).max() < 1e-11 * np.abs(u1).max()
+    assert np.abs(u2 - u1).max() < 1e-10 * np.abs(u1).max()
     print(f"case: Si{cell.natom} subspace_diag of {nb} bands", flush=True)
     esp = ctx.subspace_diag()
     assert np.all(np.diff(esp) >= 0)

@@ -2,14 +2,21 @@
 mixing, energies) and of whole SCF runs against the CPU oracle.
 
 Tolerances (FP64): fields 1e-11 absolute (|Vion| ~ 10 Ha, FFT rounding
-~1e-14 relative); energies 1e-10 Ha.  Whole runs: rounding differences of
-1e-14 in the starting bands grow to at most 5e-9 Ha in eigenvalues and 2e-11
-Ha/atom in energy over a run converged to 1e-11 Ha/atom (measured on the
-oracle against itself), so the runs are held to 1e-7 Ha/atom in total
-energy and 1e-8 Ha in the eigenvalues of each degenerate shell taken
-together (their mean, which rotations inside the shell leave unchanged).
-Members of a degenerate shell stay split by up to ~1e-6 Ha at this
-convergence, so single levels are held to 2e-6 Ha.
+~1e-14 relative); energies 1e-10 Ha.  Whole runs: total energy within
+1e-7 Ha/atom, band energies within 1e-8 Ha.
+
+The band energies of the Si8 cell need a density converged to the rounding
+floor for that.  Its Gamma levels form six-fold shells, and a density left
+unconverged breaks their symmetry: at a stop of 1e-10 Ha/atom the members of
+a shell are split by about 1.3e-6 Ha, at 1e-12 by about 3e-8 Ha (the oracle
+against itself; more eigensolver passes at the final potential move no
+level by more than 1e-9 Ha, so the split belongs to the density, not to the
+eigenvectors).  Below about 1e-12 Ha/atom the energy change per iteration
+is rounding noise and no longer a usable stop test, so the SCF comparisons
+run a fixed number of iterations on both sides (the last ones at the noise
+floor) and converge the final levels at fixed potential.  The
+fixed-potential comparison gives the library and the oracle the same input
+density and start bands.
 """
 import numpy as np
 import pytest
@@ -44,6 +51,30 @@ def test_constants_and_vion(cell, ctx):
     assert nel == 32.0
     assert eew == pytest.approx(so.ewald(cell), abs=1e-10)
     np.testing.assert_allclose(ctx.get_field(gpu.FIELD_VION), so.vion(cell), rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("nrep", [2, 3])
+def test_ewald_on_device_larger_cells(nrep):
+    """The device Ewald sum (pair images and reciprocal vectors summed in a
+    fixed order) against the oracle's at Si64 and Si216 with displaced
+    atoms; the oracle uses the balanced split eta = pi natom^(1/3) /
+    V^(2/3), on which the energy does not depend."""
+    from rsdft_develop_amd import gpu
+
+    cell = systems.silicon_supercell(nrep, ngrid_per_cell=12, nband=8, displacement=0.05, seed=nrep)
+    c = gpu.Context()
+    try:
+        c.set_grid(cell.ngrid, cell.aa, cell.md)
+        c.set_local_gth(cell)
+        eew, _ = c.scf_constants()
+        eew2, _ = c.scf_constants()
+        c.set_local_gth(cell)
+        assert c.scf_constants()[0] == eew == eew2  # the same bits on a repeat
+    finally:
+        c.close()
+    vol = abs(np.linalg.det(cell.aa))
+    want = so.ewald(cell, eta=np.pi * cell.natom ** (1 / 3) / vol ** (2 / 3))
+    assert eew == pytest.approx(want, abs=1e-9 * cell.natom)
 
 
 def test_initial_density_hartree_xc_vloc(cell, ctx):
@@ -142,11 +173,18 @@ def test_field_errors(ctx):
         ctx.mix_init(0, 0.3)
 
 
+SI8_ITERS = 72  # about 20 iterations past a 1e-12 Ha/atom stop
+
+
+def _at_noise_floor(de_list):
+    return max(de_list[-5:]) < 1e-11
+
+
 @pytest.fixture(scope="module")
 def si8_ref():
     cell = systems.silicon_supercell(1, ngrid_per_cell=36)
-    ref = so.run_scf(cell, etol_per_atom=1e-10, ekbt=0.01, max_iter=80)
-    assert ref["converged"]
+    ref = so.run_scf(cell, etol_per_atom=0.0, ekbt=0.01, max_iter=SI8_ITERS, refine_tol=1e-11)
+    assert _at_noise_floor([h["de"] for h in ref["history"]])
     return cell, ref
 
 
@@ -154,17 +192,34 @@ def test_si8_scf_matches_oracle(si8_ref):
     from rsdft_develop_amd import scf
 
     cell, ref = si8_ref
-    print("case: Si8 36^3 SCF, EKBT 0.01, etol 1e-10 Ha/atom", flush=True)
-    run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=1e-10, max_iter=80))
+    print(f"case: Si8 36^3 SCF, EKBT 0.01, {SI8_ITERS} iterations", flush=True)
+    run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=0.0, max_iter=SI8_ITERS))
     res = run.run()
-    assert res.converged
+    assert res.niter == SI8_ITERS and _at_noise_floor([r.de_per_atom for r in res.iterations])
     # early iterations follow the oracle step for step
     for a, b in zip(res.iterations[:6], ref["history"][:6]):
         assert a.etot == pytest.approx(b["etot"], abs=1e-8 * cell.natom), a.iter
-    assert abs(res.niter - len(ref["history"])) <= 3
     assert abs(res.etot - ref["etot"]) / cell.natom < 1e-7
     assert abs(res.fene - ref["fene"]) / cell.natom < 1e-7
-    assert_levels_match(res.esp, ref["esp"])
+    esp, passes = run.refine_levels(tol=1e-11)
+    d = np.abs(np.sort(esp) - np.sort(ref["esp_refined"])).max()
+    print(f"levels after {passes} fixed-potential passes: max difference {d:.2e} Ha", flush=True)
+    assert_levels_match(esp, ref["esp_refined"], shell_tol=1e-8, level_tol=1e-8)
+
+
+def test_si8_levels_at_fixed_potential_match_oracle(si8_ref):
+    """The oracle's last input density and final bands handed to the
+    library: the band energies of that Hamiltonian, converged at fixed
+    potential on both sides, agree to 1e-10 Ha."""
+    from rsdft_develop_amd import scf
+
+    cell, ref = si8_ref
+    print("case: Si8 36^3 levels at the oracle's final potential", flush=True)
+    run = scf.SCF(cell, scf.ScfParams(ekbt=0.01), bands=ref["bands"], rho=ref["rho_in"])
+    esp, passes = run.refine_levels(tol=1e-12)
+    d = np.abs(np.sort(esp) - np.sort(ref["esp_refined"])).max()
+    print(f"levels after {passes} passes: max difference {d:.2e} Ha", flush=True)
+    np.testing.assert_allclose(np.sort(esp), np.sort(ref["esp_refined"]), rtol=0, atol=1e-10)
 
 
 def test_si8_mixed_precision_scf_reaches_fp64_result(si8_ref):
@@ -174,12 +229,15 @@ def test_si8_mixed_precision_scf_reaches_fp64_result(si8_ref):
 
     cell, ref = si8_ref
     print("case: Si8 36^3 mixed-precision SCF", flush=True)
-    res = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=1e-10, max_iter=80, mixed=True)).run()
-    assert res.converged
+    run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=0.0, max_iter=SI8_ITERS + 20, mixed=True,
+                                      fp64_below=1e-8))
+    res = run.run()
+    assert _at_noise_floor([r.de_per_atom for r in res.iterations])
     precs = [r.precision for r in res.iterations]
     assert precs[0] == 32 and precs[-1] == 64 and precs == sorted(precs)
     assert abs(res.etot - ref["etot"]) / cell.natom < 1e-7
-    assert_levels_match(res.esp, ref["esp"], shell_tol=1e-7, level_tol=5e-6)
+    esp, _ = run.refine_levels(tol=1e-11)
+    assert_levels_match(esp, ref["esp_refined"], shell_tol=1e-8, level_tol=1e-8)
 
 
 def test_device_start_bands_equal_host_twin():
@@ -209,10 +267,12 @@ def test_si8_scf_from_device_start_reaches_the_same_state(si8_ref):
 
     cell, ref = si8_ref
     print("case: Si8 36^3 SCF from device uniform start", flush=True)
-    res = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=1e-10, max_iter=80, start="uniform")).run()
-    assert res.converged
+    run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=0.0, max_iter=SI8_ITERS, start="uniform"))
+    res = run.run()
+    assert _at_noise_floor([r.de_per_atom for r in res.iterations])
     assert abs(res.etot - ref["etot"]) / cell.natom < 1e-7
-    assert_levels_match(res.esp, ref["esp"], shell_tol=3e-7, level_tol=5e-6)
+    esp, _ = run.refine_levels(tol=1e-11)
+    assert_levels_match(esp, ref["esp_refined"], shell_tol=1e-8, level_tol=1e-8)
 
 
 
