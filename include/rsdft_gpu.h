@@ -144,6 +144,13 @@ int rsg_set_hpsi_shape(rsg_ctx *ctx, int precision, int ry, int ty, int zlen,
  * writes and the nonlocal kernel adds.  No reference counterpart. */
 int rsg_set_hpsi_order(rsg_ctx *ctx, int precision, int order);
 int rsg_get_hpsi_order(rsg_ctx *ctx, int precision, int *order);
+/* Form of the analytic projections (d_ps_nloc2_op_module.f90:182-191):
+ * 0 = per sphere segment (one atom's points on one plane), 1 = by row
+ * tiles (grid rows staged once per plane, every atom meeting them).  The
+ * getter reports what an nb-band call would use (1 falls back to 0 where
+ * the row tiles do not fit).  No reference counterpart. */
+int rsg_set_hpsi_proj(rsg_ctx *ctx, int precision, int proj);
+int rsg_get_hpsi_proj(rsg_ctx *ctx, int precision, int *proj);
 /* Grid points along x per stencil thread: 1, or 2 for the pair kernel
  * (FP32 with md = 6, n1 even, tile rows dividing n2; elsewhere the call
  * falls back to 1).  The getter reports what an nb-band call would use.

@@ -413,6 +413,21 @@ int rsg_get_hpsi_order(rsg_ctx *ctx, int precision, int *order) {
   return RSG_OK;
 }
 
+int rsg_set_hpsi_proj(rsg_ctx *ctx, int precision, int proj) {
+  if (!ctx) return RSG_ERR_ARG;
+  if ((precision != 64 && precision != 32) || (proj != 0 && proj != 1))
+    return ctx->fail(RSG_ERR_ARG, "set_hpsi_proj: precision 64/32 and proj 0/1");
+  (precision == 64 ? ctx->shape64 : ctx->shape32).proj = proj;
+  return RSG_OK;
+}
+
+int rsg_get_hpsi_proj(rsg_ctx *ctx, int precision, int *proj) {
+  if (!ctx || !proj) return RSG_ERR_ARG;
+  if (precision != 64 && precision != 32) return ctx->fail(RSG_ERR_ARG, "get_hpsi_proj: precision 64/32");
+  *proj = rsg::effective_proj(ctx, precision);
+  return RSG_OK;
+}
+
 int rsg_set_hpsi_xpoints(rsg_ctx *ctx, int precision, int rx) {
   if (!ctx) return RSG_ERR_ARG;
   if ((precision != 64 && precision != 32) || (rx != 1 && rx != 2))

@@ -85,9 +85,16 @@ template <typename T> struct AnalyticDev {
   DevBuf hits;          // uint64 per grid point: 4 x uint16 (group + 1 | s << 14 | p << 15)
   DevBuf tile_off;      // int[ntile + 1]
   DevBuf tile_g;        // int[tile_off[ntile]]
+  // row-tile projection: blocks of rt_ty grid rows x rt_zr planes; per block
+  // its pieces (one atom's sphere points on one plane and in the block's
+  // rows: int4 beg, end, z * 256 + local slot, group), and per local slot
+  // the global slot of the partial projections (grouped by atom)
+  int rt_ty = 0, rt_zr = 0, rt_ny = 0, rt_nz = 0, rt_nslot_max = 0, rt_nslot = 0;
+  DevBuf rt_piece, rt_piece_off, rt_slot_off, rt_slot_gid, grp_slot_off;
   void release_all() {
     for (DevBuf *b : {&es, &ep, &dd, &amp, &scale, &pts, &seg_beg, &seg_end, &seg_grp, &seg_order, &grp_seg_off,
-                      &hits, &tile_off, &tile_g})
+                      &hits, &tile_off, &tile_g, &rt_piece, &rt_piece_off, &rt_slot_off, &rt_slot_gid,
+                      &grp_slot_off})
       b->release();
   }
 };
@@ -99,6 +106,8 @@ struct HostAnalytic {
   std::vector<int> seg_beg, seg_end, seg_grp, seg_order, grp_seg_off{0};
   std::vector<uint64_t> hits;
   std::vector<int> tile_off, tile_g;
+  int rt_ty = 0, rt_zr = 0, rt_ny = 0, rt_nz = 0, rt_nslot_max = 0;
+  std::vector<int> rt_piece, rt_piece_off, rt_slot_off, rt_slot_gid, grp_slot_off;
 };
 
 // Launch shape of the hpsi kernels for one grid and precision.
@@ -110,6 +119,7 @@ struct HpsiShape {
   int bt = 8;     // bands per scatter thread
   int order = 0;  // analytic projectors: 0 = nonlocal writes, stencil adds; 1 = stencil writes, nonlocal adds
   int rx = 1;     // x-points per stencil thread: 1, or 2 (pair kernel: FP32, Md 6, even n1, ty | n2)
+  int proj = 0;   // analytic projections: 0 = per sphere segment (k_proj_m), 1 = by row tiles (k_proj_rows)
 };
 
 // Learned preconditioner state: bf16 weight matrices for the kernels, the
@@ -268,5 +278,7 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const H
 template <typename T>
 int launch_kinetic(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, int accumulate);
 int resolved_shape(rsg_ctx *ctx, int precision, int nb, HpsiShape *out);
+// the projection form an hpsi call uses (HpsiShape::proj where it fits)
+int effective_proj(rsg_ctx *ctx, int precision);
 int autotune_hpsi(rsg_ctx *ctx, int precision, int nb, int reps, float *best_ms);
 }

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "context.hpp"
 
@@ -391,6 +392,115 @@ int upload_nonlocal(rsg_ctx *ctx) {
   return RSG_OK;
 }
 
+namespace {
+
+// Row-tile blocks of the projection: rt_ty grid rows by rt_zr planes (all
+// of x).  A piece is the run of one atom's sphere points on one plane that
+// falls in the block's rows; an atom's points are in (z, y, x) order, so the
+// run is contiguous.  Each block numbers the atoms it meets (local slots,
+// in atom order); the global slots of one atom are contiguous, in block
+// order, so that the reduction sums one range per atom.
+// f(begin, end) over [0, n) split among host threads (at most 16, the
+// host share of one GPU); each index belongs to exactly one call.
+template <typename F> void parallel_range(int n, F &&f) {
+  const int nt = int(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  if (n < 2 * nt || nt == 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) th.emplace_back([&, t]() { f(int(int64_t(n) * t / nt), int(int64_t(n) * (t + 1) / nt)); });
+  for (auto &x : th) x.join();
+}
+
+constexpr int kRowTilePoints = 288;  // rows per block ~ this many points / n1
+constexpr int kRowTileSlots = 32;    // at most this many atoms per block (LDS sums)
+
+void build_row_tiles(const int *ng, HostAnalytic &out) {
+  const int ty = std::max(1, std::min(8, kRowTilePoints / ng[0]));
+  const int ny = (ng[1] + ty - 1) / ty;
+  struct Piece {
+    int blk, z, g, beg, end;
+  };
+  auto collect = [&](int zr, std::vector<Piece> &pieces) {
+    pieces.clear();
+    for (int g = 0; g < out.ngroup; ++g) {
+      const int s0 = out.grp_seg_off[size_t(g)], s1 = out.grp_seg_off[size_t(g) + 1];
+      const int b = out.seg_beg[size_t(s0)], e = out.seg_end[size_t(s1) - 1];
+      for (int i = b; i < e;) {
+        const uint32_t w = out.pts[size_t(i)];
+        const int z = int(w >> 20 & 1023u), t = int(w >> 10 & 1023u) / ty;
+        int j = i + 1;
+        while (j < e && int(out.pts[size_t(j)] >> 20 & 1023u) == z && int(out.pts[size_t(j)] >> 10 & 1023u) / ty == t)
+          ++j;
+        pieces.push_back({t + ny * (z / zr), z, g, i, j});
+        i = j;
+      }
+    }
+    std::stable_sort(pieces.begin(), pieces.end(), [](const Piece &a, const Piece &b) {
+      return a.blk != b.blk ? a.blk < b.blk : (a.z != b.z ? a.z < b.z : a.g < b.g);
+    });
+  };
+  std::vector<Piece> pieces;
+  int zr = 8, nz = 0, maxslot = 0;
+  for (;;) {
+    nz = (ng[2] + zr - 1) / zr;
+    collect(zr, pieces);
+    // atoms per block
+    maxslot = 0;
+    for (size_t i = 0; i < pieces.size();) {
+      size_t j = i;
+      std::vector<int> gs;
+      while (j < pieces.size() && pieces[j].blk == pieces[i].blk) gs.push_back(pieces[j++].g);
+      std::sort(gs.begin(), gs.end());
+      maxslot = std::max(maxslot, int(std::unique(gs.begin(), gs.end()) - gs.begin()));
+      i = j;
+    }
+    if (maxslot <= kRowTileSlots || zr == 1) break;
+    zr /= 2;
+  }
+  const int nrt = ny * nz;
+  out.rt_ty = ty;
+  out.rt_zr = zr;
+  out.rt_ny = ny;
+  out.rt_nz = nz;
+  out.rt_nslot_max = maxslot;
+  out.rt_piece.clear();
+  out.rt_piece_off.assign(size_t(nrt) + 1, 0);
+  out.rt_slot_off.assign(size_t(nrt) + 1, 0);
+  std::vector<std::vector<int>> blk_groups(static_cast<size_t>(nrt));
+  std::vector<std::vector<int>> grp_blocks(static_cast<size_t>(out.ngroup));
+  size_t i = 0;
+  for (int blk = 0; blk < nrt; ++blk) {
+    size_t j = i;
+    while (j < pieces.size() && pieces[j].blk == blk) ++j;
+    std::vector<int> &gs = blk_groups[size_t(blk)];
+    for (size_t k = i; k < j; ++k) gs.push_back(pieces[k].g);
+    std::sort(gs.begin(), gs.end());
+    gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
+    for (size_t k = i; k < j; ++k) {
+      const int slot = int(std::lower_bound(gs.begin(), gs.end(), pieces[k].g) - gs.begin());
+      out.rt_piece.insert(out.rt_piece.end(), {pieces[k].beg, pieces[k].end, pieces[k].z * 256 + slot, pieces[k].g});
+    }
+    for (int g : gs) grp_blocks[size_t(g)].push_back(blk);
+    out.rt_piece_off[size_t(blk) + 1] = int(out.rt_piece.size() / 4);
+    out.rt_slot_off[size_t(blk) + 1] = out.rt_slot_off[size_t(blk)] + int(gs.size());
+    i = j;
+  }
+  // global slot ids: one contiguous range per atom, its blocks in order
+  out.grp_slot_off.assign(size_t(out.ngroup) + 1, 0);
+  for (int g = 0; g < out.ngroup; ++g)
+    out.grp_slot_off[size_t(g) + 1] = out.grp_slot_off[size_t(g)] + int(grp_blocks[size_t(g)].size());
+  out.rt_slot_gid.assign(size_t(out.rt_slot_off[size_t(nrt)]), 0);
+  std::vector<int> next(out.grp_slot_off.begin(), out.grp_slot_off.end() - 1);
+  for (int blk = 0; blk < nrt; ++blk) {
+    const std::vector<int> &gs = blk_groups[size_t(blk)];
+    for (size_t k = 0; k < gs.size(); ++k) out.rt_slot_gid[size_t(out.rt_slot_off[size_t(blk)]) + k] = next[size_t(gs[k])]++;
+  }
+}
+
+}  // namespace
+
 // Analytic description of GTH sets made only of (l = 0, n = 1) and
 // (l = 1, n = 1) projectors, at most one of each per species.  Returns
 // RSG_ERR_UNSUPPORTED (not an error for the caller) when the set, the cell
@@ -426,15 +536,28 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
   const int axoff[3] = {0, ng[0], ng[0] + ng[1]};
   const double c[3] = {1.0 / ng[0], 1.0 / ng[1], 1.0 / ng[2]};
   const int64_t n12 = int64_t(ng[0]) * ng[1];
-  std::vector<int> nhit(size_t(ctx->N), 0);
-  out.hits.assign(size_t(ctx->N), 0);
+  for (int a = 0; a < natom; ++a)
+    if (kind[a] < 0 || kind[a] >= nkind) return RSG_ERR_ARG;
+  // group (atom with projectors) numbering, in atom order
+  std::vector<int> atom_of;
+  for (int a = 0; a < natom; ++a)
+    if (qs[size_t(kind[a])] >= 0 || qp[size_t(kind[a])] >= 0) atom_of.push_back(a);
+  const int ngrp = int(atom_of.size());
   out.S = S;
-  int g = 0;
-  for (int a = 0; a < natom; ++a) {
+  out.amp.assign(size_t(ngrp) * 4, 0.0);
+  out.scale.assign(size_t(ngrp) * 4, 0.0);
+  out.es.assign(size_t(ngrp) * S, 0.0);
+  out.ep.assign(size_t(ngrp) * S, 0.0);
+  out.dd.assign(size_t(ngrp) * S, 0.0);
+  // Per group, independent of the others: amplitudes, the 1-D factor tables
+  // and the sphere points in grid order.  Host threads take groups in turn;
+  // everything shared (hit words, point list, segments) is merged afterwards
+  // in group order, so the result does not depend on the thread count.
+  std::vector<std::vector<std::pair<int64_t, uint32_t>>> gpts(static_cast<size_t>(ngrp));
+  auto build_group = [&](int g) {
+    const int a = atom_of[size_t(g)];
     const int ik = kind[a];
-    if (ik < 0 || ik >= nkind) return RSG_ERR_ARG;
     const int q0 = qs[size_t(ik)], q1 = qp[size_t(ik)];
-    if (q0 < 0 && q1 < 0) continue;
     const double *fa = frac + 3 * a;
     // same arithmetic as build_gth_tables, so sphere membership agrees
     const double R[3] = {aa[0] * fa[0] + aa[3] * fa[1] + aa[6] * fa[2],
@@ -453,15 +576,14 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
                                        (std::pow(rps[q1], 2.5) * std::sqrt(gamma_half(1, 1))) *
                                        std::sqrt(3.0 / pi4)
                                  : 0.0;
-    out.amp.insert(out.amp.end(), {amp_s, amp_p, amp_p, amp_p});
     const double sg_s = q0 >= 0 && hnl[q0] < 0.0 ? -1.0 : 1.0, sg_p = q1 >= 0 && hnl[q1] < 0.0 ? -1.0 : 1.0;
-    out.scale.insert(out.scale.end(), {q0 >= 0 ? sg_s * ctx->dv : 0.0, q1 >= 0 ? sg_p * ctx->dv : 0.0,
-                                       q1 >= 0 ? sg_p * ctx->dv : 0.0, q1 >= 0 ? sg_p * ctx->dv : 0.0});
+    const size_t g4 = size_t(g) * 4;
+    out.amp[g4] = amp_s;
+    out.amp[g4 + 1] = out.amp[g4 + 2] = out.amp[g4 + 3] = amp_p;
+    out.scale[g4] = q0 >= 0 ? sg_s * ctx->dv : 0.0;
+    out.scale[g4 + 1] = out.scale[g4 + 2] = out.scale[g4 + 3] = q1 >= 0 ? sg_p * ctx->dv : 0.0;
     // 1-D tables over each axis at the image nearest the atom
-    const size_t base = out.es.size();
-    out.es.resize(base + size_t(S));
-    out.ep.resize(base + size_t(S));
-    out.dd.resize(base + size_t(S));
+    const size_t base = size_t(g) * S;
     for (int t = 0; t < 3; ++t)
       for (int i = 0; i < ng[t]; ++i) {
         int id = i;
@@ -475,7 +597,7 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
     // sphere points, in grid order
     int mm[3];
     for (int t = 0; t < 3; ++t) mm[t] = int(std::ceil(rc / ctx->hgrid[t])) + 2;
-    std::vector<std::pair<int64_t, uint32_t>> pts;
+    std::vector<std::pair<int64_t, uint32_t>> &pts = gpts[size_t(g)];
     for (int o3 = -mm[2]; o3 <= mm[2]; ++o3)
       for (int o2 = -mm[1]; o2 <= mm[1]; ++o2)
         for (int o1 = -mm[0]; o1 <= mm[0]; ++o1) {
@@ -494,13 +616,26 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
           const uint32_t word = uint32_t(w[0]) | (uint32_t(w[1]) << 10) | (uint32_t(w[2]) << 20) |
                                 (uint32_t(in_s) << 30) | (uint32_t(in_p) << 31);
           pts.emplace_back(lin, word);
-          int &k = nhit[size_t(lin)];
-          if (k >= 4) return RSG_ERR_UNSUPPORTED;  // more than four spheres at one point
-          const uint64_t e = uint64_t(g + 1) | (uint64_t(in_s) << 14) | (uint64_t(in_p) << 15);
-          out.hits[size_t(lin)] |= e << (16 * k);
-          ++k;
         }
     std::sort(pts.begin(), pts.end());
+  };
+  parallel_range(ngrp, [&](int g0, int g1) {
+    for (int g = g0; g < g1; ++g) build_group(g);
+  });
+  // merge in group order: hit words (at most four spheres per point), the
+  // point list and its plane segments
+  std::vector<unsigned char> nhit(size_t(ctx->N), 0);
+  out.hits.assign(size_t(ctx->N), 0);
+  int g = 0;
+  for (; g < ngrp; ++g) {
+    const std::vector<std::pair<int64_t, uint32_t>> &pts = gpts[size_t(g)];
+    for (const auto &pw : pts) {
+      unsigned char &k = nhit[size_t(pw.first)];
+      if (k >= 4) return RSG_ERR_UNSUPPORTED;  // more than four spheres at one point
+      const uint64_t e = uint64_t(g + 1) | (uint64_t((pw.second >> 30) & 1u) << 14) | (uint64_t(pw.second >> 31) << 15);
+      out.hits[size_t(pw.first)] |= e << (16 * k);
+      ++k;
+    }
     const int pbase = int(out.pts.size());
     for (auto &pw : pts) out.pts.push_back(pw.second);
     // one segment per plane run (grid order is plane-major), capped in length
@@ -514,7 +649,7 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
       b = e;
     }
     out.grp_seg_off.push_back(int(out.seg_beg.size()));
-    ++g;
+    std::vector<std::pair<int64_t, uint32_t>>().swap(gpts[size_t(g)]);
   }
   out.ngroup = g;
   if (g == 0) return RSG_ERR_UNSUPPORTED;
@@ -524,27 +659,40 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
   std::stable_sort(out.seg_order.begin(), out.seg_order.end(), [&](int a, int b) {
     return (out.pts[size_t(out.seg_beg[size_t(a)])] >> 20 & 1023u) < (out.pts[size_t(out.seg_beg[size_t(b)])] >> 20 & 1023u);
   });
-  // atoms meeting each 16-point tile
+  // atoms meeting each 16-point tile: tile ranges in host threads, joined in
+  // tile order
   const int64_t ntile = (ctx->N + 15) / 16;
   out.tile_off.assign(size_t(ntile) + 1, 0);
   out.tile_g.clear();
-  for (int64_t t = 0; t < ntile; ++t) {
-    int list[64];
-    int nl = 0;
-    for (int64_t j = t * 16; j < std::min<int64_t>(ctx->N, t * 16 + 16); ++j) {
-      uint64_t hw = out.hits[size_t(j)];
-      while (hw) {
-        const int gg = int(hw & 0x3fffu) - 1;
-        hw >>= 16;
-        bool seen = false;
-        for (int i = 0; i < nl; ++i) seen |= list[i] == gg;
-        if (!seen) list[nl++] = gg;
+  const int nchunk = int(std::min<int64_t>(ntile, 256));
+  std::vector<std::vector<int>> chunk_g(static_cast<size_t>(nchunk));
+  std::vector<int> tcount(static_cast<size_t>(ntile), 0);
+  parallel_range(nchunk, [&](int c0, int c1) {
+    for (int c = c0; c < c1; ++c) {
+      const int64_t t0 = ntile * c / nchunk, t1 = ntile * (c + 1) / nchunk;
+      for (int64_t t = t0; t < t1; ++t) {
+        int list[64];
+        int nl = 0;
+        for (int64_t j = t * 16; j < std::min<int64_t>(ctx->N, t * 16 + 16); ++j) {
+          uint64_t hw = out.hits[size_t(j)];
+          while (hw) {
+            const int gg = int(hw & 0x3fffu) - 1;
+            hw >>= 16;
+            bool seen = false;
+            for (int i = 0; i < nl; ++i) seen |= list[i] == gg;
+            if (!seen) list[nl++] = gg;
+          }
+        }
+        std::sort(list, list + nl);
+        chunk_g[size_t(c)].insert(chunk_g[size_t(c)].end(), list, list + nl);
+        tcount[size_t(t)] = nl;
       }
     }
-    std::sort(list, list + nl);
-    out.tile_g.insert(out.tile_g.end(), list, list + nl);
-    out.tile_off[size_t(t) + 1] = int(out.tile_g.size());
-  }
+  });
+  for (int64_t t = 0; t < ntile; ++t) out.tile_off[size_t(t) + 1] = out.tile_off[size_t(t)] + tcount[size_t(t)];
+  out.tile_g.reserve(size_t(out.tile_off[size_t(ntile)]));
+  for (auto &v : chunk_g) out.tile_g.insert(out.tile_g.end(), v.begin(), v.end());
+  build_row_tiles(ng, out);
   return RSG_OK;
 }
 
@@ -580,6 +728,17 @@ template <typename T> int upload_analytic_t(rsg_ctx *ctx, AnalyticDev<T> &d, con
   rc |= up(d.tile_off, h.tile_off.data(), h.tile_off.size() * sizeof(int));
   rc |= up(d.tile_g, h.tile_g.data(), h.tile_g.size() * sizeof(int));
   d.ntile = int(h.tile_off.size()) - 1;
+  rc |= up(d.rt_piece, h.rt_piece.data(), h.rt_piece.size() * sizeof(int));
+  rc |= up(d.rt_piece_off, h.rt_piece_off.data(), h.rt_piece_off.size() * sizeof(int));
+  rc |= up(d.rt_slot_off, h.rt_slot_off.data(), h.rt_slot_off.size() * sizeof(int));
+  rc |= up(d.rt_slot_gid, h.rt_slot_gid.data(), h.rt_slot_gid.size() * sizeof(int));
+  rc |= up(d.grp_slot_off, h.grp_slot_off.data(), h.grp_slot_off.size() * sizeof(int));
+  d.rt_ty = h.rt_ty;
+  d.rt_zr = h.rt_zr;
+  d.rt_ny = h.rt_ny;
+  d.rt_nz = h.rt_nz;
+  d.rt_nslot_max = h.rt_nslot_max;
+  d.rt_nslot = h.grp_slot_off.empty() ? 0 : h.grp_slot_off.back();
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = RSG_ERR_HIP;
   return rc ? ctx->fail(RSG_ERR_HIP, "upload of analytic projector data failed") : RSG_OK;
 }

@@ -81,6 +81,8 @@ def load_library():
         "rsg_autotune_hpsi": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(ctypes.c_float)]),
         "rsg_set_hpsi_order": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_get_hpsi_order": (_c_int, [_vp, _c_int, _P(_c_int)]),
+        "rsg_set_hpsi_proj": (_c_int, [_vp, _c_int, _c_int]),
+        "rsg_get_hpsi_proj": (_c_int, [_vp, _c_int, _P(_c_int)]),
         "rsg_set_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_precond_hprsdft_f64": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
         "rsg_lpc_set_hybrid_scale": (_c_int, [_vp, _c_dbl]),
@@ -320,6 +322,15 @@ class Context:
         self._check(self.L.rsg_get_hpsi_order(self.h, int(precision), ctypes.byref(o)))
         return o.value
 
+    def set_hpsi_proj(self, precision=64, proj=0):
+        """Analytic projections per sphere segment (0) or by row tiles (1)."""
+        self._check(self.L.rsg_set_hpsi_proj(self.h, int(precision), int(proj)))
+
+    def hpsi_proj(self, precision=64) -> int:
+        o = _c_int()
+        self._check(self.L.rsg_get_hpsi_proj(self.h, int(precision), ctypes.byref(o)))
+        return o.value
+
     def set_hpsi_xpoints(self, precision=64, rx=1):
         """x-points per stencil thread: 2 selects the FP32 pair kernel where it fits."""
         self._check(self.L.rsg_set_hpsi_xpoints(self.h, int(precision), int(rx)))
@@ -332,6 +343,7 @@ class Context:
         rx = _c_int()
         self._check(self.L.rsg_get_hpsi_xpoints(self.h, int(precision), int(nb), ctypes.byref(rx)))
         d["rx"] = rx.value
+        d["proj"] = self.hpsi_proj(precision)
         return d
 
     def apply_hpsi_shape(self, precision, sh: dict, chunk=None):
@@ -340,6 +352,7 @@ class Context:
                             chunk=sh["chunk"] if chunk is None else chunk, bt=sh["bt"])
         self.set_hpsi_order(precision, sh.get("order", 0))
         self.set_hpsi_xpoints(precision, sh.get("rx", 1))
+        self.set_hpsi_proj(precision, sh.get("proj", 0))
 
     def autotune_hpsi(self, precision=64, nb=1, reps=5) -> float:
         ms = ctypes.c_float()

@@ -215,9 +215,11 @@ def test_hpsi_errors(gpu_mod):
         ctx.hamiltonian(np.zeros((8, 1)), k=2)
 
 
-@pytest.mark.parametrize("nrep,ng,disp,nb", [(1, 36, 0.1, 13), (2, 36, 0.05, 19)])
+@pytest.mark.parametrize("nrep,ng,disp,nb", [(1, 36, 0.1, 13), (2, 36, 0.05, 19), (1, 30, 0.1, 21)])
 def test_nonlocal_modes_agree_with_oracle(orc, gpu_mod, nrep, ng, disp, nb):
-    """Tables (mode 1) and on-the-fly GTH projectors (mode 2), FP64 and FP32."""
+    """Tables (mode 1) and on-the-fly GTH projectors (mode 2), FP64 and FP32;
+    the analytic projections per sphere segment (proj 0) and by row tiles
+    (proj 1, whose last tile of rows is partial on these grids)."""
     from rsdft_develop_amd import hip
 
     cell = systems.silicon_supercell(nrep, ngrid_per_cell=ng, displacement=disp, seed=nb)
@@ -225,12 +227,16 @@ def test_nonlocal_modes_agree_with_oracle(orc, gpu_mod, nrep, ng, disp, nb):
     assert ctx.nonlocal_mode() == (0, True)
     psi = systems.random_bands(cell.npoints, nb, seed=300 + nb)
     want = ham.hpsi(psi)
-    for mode, order, chunk in ((1, 0, 0), (2, 0, 0), (2, 1, 0), (2, 0, 4), (2, 1, 4)):
+    for mode, order, chunk, proj in ((1, 0, 0, 0), (2, 0, 0, 0), (2, 1, 0, 0), (2, 0, 4, 0), (2, 1, 4, 0),
+                                     (2, 0, 0, 1), (2, 1, 4, 1), (2, 0, 17, 1)):
         ctx.set_nonlocal_mode(mode)
         for prec in (64, 32):
             ctx.set_hpsi_shape(prec, chunk=chunk)
             ctx.set_hpsi_order(prec, order)
-        print(f"case: mode {mode} order {order} chunk {chunk}, Si{8 * nrep ** 3}, {nb} bands", flush=True)
+            ctx.set_hpsi_proj(prec, proj)
+            if mode == 2:
+                assert ctx.hpsi_shape(prec, nb)["proj"] == proj
+        print(f"case: mode {mode} order {order} chunk {chunk} proj {proj}, Si{8 * nrep ** 3}, {nb} bands", flush=True)
         _close(ctx.hamiltonian(psi), want, 1e-12)
         d_in = hip.DeviceArray.from_numpy(psi, dtype=np.float32)
         d_out = hip.DeviceArray(psi.shape, dtype=np.float32)
