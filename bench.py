@@ -340,7 +340,8 @@ def batch_leg(ncells: int = 64, workers: int = 8, seq_cells: int | None = None):
                 gpu_max_hw_queues=os.environ.get("GPU_MAX_HW_QUEUES"),
                 workers=workers, batched_seconds=round(par.wall, 3), batched_cells_per_s=round(par.cells_per_s, 3),
                 batched_direct_launch_cells_per_s=round(par_direct.cells_per_s, 3),
-                graph_results_equal_direct=same,
+                graph_results_equal_direct=same, graph_launches=par.graph_launches,
+                graph_capture_failures=par.graph_failures,
                 sequential_cells=seq_cells, sequential_seconds=round(seq.wall, 3),
                 sequential_cells_per_s=round(seq.cells_per_s, 3),
                 speedup=round(par.cells_per_s / seq.cells_per_s, 3),

@@ -220,6 +220,9 @@ int rsg_conjugate_gradient(rsg_ctx *ctx, int ncg, int pc_kind, int mloop,
 int rsg_set_cg_graphs(rsg_ctx *ctx, int enable);
 /* CG band blocks run from a captured graph since the context was created. */
 int rsg_cg_graph_launches(rsg_ctx *ctx, int64_t *launches);
+/* Captures the runtime refused since the context was created (those band
+ * blocks ran, and their keys keep running, with direct launches). */
+int rsg_cg_graph_failures(rsg_ctx *ctx, int64_t *failures);
 /* The SWGS keyword of the CG (read_cg, cg_module.f90:42-50): 0 (default)
  * leaves the preconditioned gradients as they are; 1 and 2 project each
  * one against the other bands of the store before the direction update,

@@ -67,6 +67,8 @@ class BatchResult:
     results: list
     wall: float
     workers: int
+    graph_launches: int = 0  # CG band blocks replayed from captured graphs, all workers
+    graph_failures: int = 0  # captures the runtime refused (those blocks ran directly)
 
     @property
     def cells_per_s(self) -> float:
@@ -122,6 +124,7 @@ def run_batch(cells: list, workers: int = 8, params: scf.ScfParams | None = None
         todo.put(i)
     results = [None] * len(cells)
     errors = []
+    graph_stats = []
 
     def worker():
         # one context per worker for all its cells: device buffers and, on
@@ -152,7 +155,10 @@ def run_batch(cells: list, workers: int = 8, params: scf.ScfParams | None = None
                     errors.append((i, e))
                     return
         finally:
-            ctx.close()
+            try:
+                graph_stats.append((ctx.cg_graph_launches(), ctx.cg_graph_failures()))
+            finally:
+                ctx.close()
 
     t0 = time.perf_counter()
     threads = [threading.Thread(target=worker, daemon=True) for _ in range(max(1, workers))]
@@ -164,4 +170,4 @@ def run_batch(cells: list, workers: int = 8, params: scf.ScfParams | None = None
     if errors:
         i, e = errors[0]
         raise RuntimeError(f"batch cell {i} failed: {e}") from e
-    return BatchResult(results, wall, workers)
+    return BatchResult(results, wall, workers, sum(g[0] for g in graph_stats), sum(g[1] for g in graph_stats))

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <shared_mutex>
 
 #include "context.hpp"
 #include "lpc.hpp"
@@ -42,13 +43,51 @@ void *scratch_of(int device) {
   return g_scratch_ptr[device];
 }
 
+// Process-wide gate between graph captures and the calls that may
+// invalidate them in other threads.  t_gate_depth counts this thread's
+// nested guards (and a capture in progress), so that only the outermost
+// one takes the lock: a shared lock taken twice by one thread can block
+// behind a waiting capture.
+std::shared_mutex g_capture_gate;
+thread_local int t_gate_depth = 0;
+
 }  // namespace
+
+namespace rsg {
+
+DeviceOpGuard::DeviceOpGuard() {
+  if (t_gate_depth++ == 0) {
+    g_capture_gate.lock_shared();
+    locked_ = true;
+  }
+}
+
+DeviceOpGuard::~DeviceOpGuard() {
+  --t_gate_depth;
+  if (locked_) g_capture_gate.unlock_shared();
+}
+
+CaptureGate::CaptureGate() {
+  if (t_gate_depth != 0) return;  // this thread holds a guard: capturing would deadlock
+  g_capture_gate.lock();
+  ++t_gate_depth;
+  usable_ = true;
+}
+
+CaptureGate::~CaptureGate() {
+  if (!usable_) return;
+  --t_gate_depth;
+  g_capture_gate.unlock();
+}
+
+}  // namespace rsg
 
 extern "C" {
 
 int rsg_create(rsg_ctx **out, int device) {
   if (!out) return RSG_ERR_ARG;
   *out = nullptr;
+  rsg::DeviceOpGuard guard;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RSG_ERR_HIP;
   if (device < 0 || device >= ndev) return RSG_ERR_ARG;
@@ -77,6 +116,7 @@ int rsg_create(rsg_ctx **out, int device) {
 
 int rsg_destroy(rsg_ctx *ctx) {
   if (!ctx) return RSG_OK;
+  rsg::DeviceOpGuard guard;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   DevBuf *bufs[] = {&ctx->vloc64, &ctx->vloc32, &ctx->uvunk, &ctx->unk, &ctx->work,
@@ -538,6 +578,7 @@ int rsg_set_cg_graphs(rsg_ctx *ctx, int enable) {
   if (!ctx) return RSG_ERR_ARG;
   RSG_HIP(hipSetDevice(ctx->device));
   RSG_HIP(hipStreamSynchronize(ctx->stream));
+  rsg::DeviceOpGuard guard;
   for (auto &g : ctx->cg_graph_cache)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   ctx->cg_graph_cache.clear();
@@ -548,6 +589,12 @@ int rsg_set_cg_graphs(rsg_ctx *ctx, int enable) {
 int rsg_cg_graph_launches(rsg_ctx *ctx, int64_t *launches) {
   if (!ctx || !launches) return RSG_ERR_ARG;
   *launches = ctx->cg_graph_launches;
+  return RSG_OK;
+}
+
+int rsg_cg_graph_failures(rsg_ctx *ctx, int64_t *failures) {
+  if (!ctx || !failures) return RSG_ERR_ARG;
+  *failures = ctx->cg_graph_failures;
   return RSG_OK;
 }
 

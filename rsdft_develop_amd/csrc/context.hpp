@@ -15,6 +15,34 @@ namespace rsg {
 
 struct ScfState;
 
+// A HIP graph capture on one context's stream must not overlap device
+// allocation, frees, stream creation or library calls that may allocate
+// (rocBLAS, rocSOLVER, hipFFT) made by other threads of the process: the
+// runtime can invalidate the capture in progress.  Such calls hold a
+// DeviceOpGuard (shared, re-entrant per thread); a capture holds the
+// CaptureGate (exclusive).  Inside a capture the calling thread's own
+// guards are no-ops.
+struct DeviceOpGuard {
+  DeviceOpGuard();
+  ~DeviceOpGuard();
+  DeviceOpGuard(const DeviceOpGuard &) = delete;
+  DeviceOpGuard &operator=(const DeviceOpGuard &) = delete;
+
+ private:
+  bool locked_ = false;
+};
+
+struct CaptureGate {
+  CaptureGate();  // blocks until no other thread holds a DeviceOpGuard
+  ~CaptureGate();
+  bool usable() const { return usable_; }  // false when this thread already holds a guard
+  CaptureGate(const CaptureGate &) = delete;
+  CaptureGate &operator=(const CaptureGate &) = delete;
+
+ private:
+  bool usable_ = false;
+};
+
 // Device buffer owned by the context (hipMalloc/hipFree).
 struct DevBuf {
   void *p = nullptr;
@@ -219,6 +247,7 @@ struct rsg_ctx {
   int cg_swgs = 0;                 // SWGS keyword: 0 none, 1/2 cggs projections in CG
   bool cg_graphs = false;          // replay CG band blocks from captured HIP graphs
   int64_t cg_graph_launches = 0;   // band blocks run from a captured graph
+  int64_t cg_graph_failures = 0;   // captures the runtime refused
   std::vector<rsg::CgGraph> cg_graph_cache;
   // K-split of the U^T X Gram products chosen by the autotuner, per
   // (N, nb): points per chunk, 0 = one GEMM over all of K

@@ -1269,6 +1269,7 @@ int time_shape(rsg_ctx *ctx, const T *psi, T *hpsi, int nb, const HpsiShape &sh,
 // Coordinate search over the launch shape on seeded scratch bands; keeps
 // the fastest shape in the context.
 template <typename T> int autotune_t(rsg_ctx *ctx, int nb, int reps, float *best_ms) {
+  DeviceOpGuard guard;
   const size_t bytes = size_t(ctx->N) * nb * sizeof(T);
   T *psi = nullptr, *hpsi = nullptr;
   if (hipMalloc(&psi, bytes) != hipSuccess) return ctx->fail(RSG_ERR_HIP, "autotune: allocation failed");

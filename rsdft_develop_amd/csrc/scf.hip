@@ -344,6 +344,7 @@ int ensure_state(rsg_ctx *ctx) {
       st->vxc.ensure(n8) || st->work.ensure(n8) || st->zbuf.ensure(2 * n8))
     return ctx->fail(RSG_ERR_HIP, "scf: field allocation failed");
   if (!st->have_plan) {
+    DeviceOpGuard guard;
     if (hipfftPlan3d(&st->plan, ctx->n[2], ctx->n[1], ctx->n[0], HIPFFT_Z2Z) != HIPFFT_SUCCESS)
       return ctx->fail(RSG_ERR_HIP, "scf: hipfftPlan3d failed");
     if (hipfftSetStream(st->plan, ctx->stream) != HIPFFT_SUCCESS)
@@ -377,6 +378,7 @@ int ensure_state(rsg_ctx *ctx) {
 int fft(rsg_ctx *ctx, int dir) {
   ScfState *st = ctx->scf;
   hipfftDoubleComplex *z = reinterpret_cast<hipfftDoubleComplex *>(st->zbuf.p);
+  DeviceOpGuard guard;
   if (hipfftExecZ2Z(st->plan, z, z, dir) != HIPFFT_SUCCESS) return ctx->fail(RSG_ERR_HIP, "scf: hipfftExecZ2Z failed");
   return RSG_OK;
 }
@@ -1016,7 +1018,10 @@ void scf_reset(rsg_ctx *ctx) {
 void scf_release(rsg_ctx *ctx) {
   ScfState *st = ctx->scf;
   if (!st) return;
-  if (st->have_plan) hipfftDestroy(st->plan);
+  if (st->have_plan) {
+    DeviceOpGuard guard;
+    hipfftDestroy(st->plan);
+  }
   for (DevBuf *b : {&st->rho, &st->rho_out, &st->vion, &st->vh, &st->vxc, &st->zbuf, &st->work, &st->occ}) b->release();
   for (auto &b : st->xin) b.release();
   for (auto &b : st->xou) b.release();

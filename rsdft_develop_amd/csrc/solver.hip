@@ -1075,6 +1075,7 @@ int64_t gram_chunk(const rsg_ctx *ctx, int nb, int64_t N) {
 
 int gemm_tn_gram(rsg_ctx *ctx, int nb, int64_t N, double alpha, const double *A, const double *B, double *C,
                  int64_t kc_force = -1) {
+  DeviceOpGuard guard;
   const double zero = 0.0, one = 1.0;
   const int64_t kc = kc_force >= 0 ? kc_force : gram_chunk(ctx, nb, N);
   if (kc <= 0 || kc >= N) {
@@ -1108,6 +1109,7 @@ int gemm_tn_gram(rsg_ctx *ctx, int nb, int64_t N, double alpha, const double *A,
 // calls each), and keep the fastest for (N, nb).  The products are the same
 // up to summation order.
 int autotune_gram(rsg_ctx *ctx, int nb, int reps, float *best_ms, int64_t *best_kc) {
+  DeviceOpGuard guard;
   if (!ctx->have_grid) return ctx->fail(RSG_ERR_STATE, "autotune_gram: grid not set");
   if (nb < 1 || reps < 1) return ctx->fail(RSG_ERR_ARG, "autotune_gram: nb and reps must be >= 1");
   const int64_t N = ctx->N;
@@ -1158,6 +1160,7 @@ int autotune_gram(rsg_ctx *ctx, int nb, int reps, float *best_ms, int64_t *best_
 }
 
 int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
+  DeviceOpGuard guard;
   int rc = check_store(ctx, "gram_schmidt");
   if (rc) return rc;
   if (n0 < 1 || n1 > ctx->mb || n0 > n1) return ctx->fail(RSG_ERR_ARG, "gram_schmidt: band range outside 1..mb");
@@ -1188,6 +1191,7 @@ int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
 }
 
 int subspace_diag(rsg_ctx *ctx, double *esp) {
+  DeviceOpGuard guard;
   int rc = check_store(ctx, "subspace_diag");
   if (rc) return rc;
   const int mb = ctx->mb;
@@ -1250,6 +1254,7 @@ namespace {
 // blocks go through an FP64 copy in tmp (N x nn doubles).
 template <typename W>
 int cggs_block(rsg_ctx *ctx, int swgs, int n0, int nn, W *pg, double *tmp) {
+  DeviceOpGuard guard;
   const int mb = ctx->mb;
   const int64_t N = ctx->N;
   if (ctx->cgcoef.ensure(size_t(mb) * nn * sizeof(double)) != hipSuccess)
@@ -1336,7 +1341,12 @@ template <typename F> int run_graphed(rsg_ctx *ctx, const std::vector<unsigned c
   if (!e) {
     constexpr size_t kMaxGraphs = 16;
     if (ctx->cg_graph_cache.size() >= kMaxGraphs) {
-      if (ctx->cg_graph_cache.front().exec) (void)hipGraphExecDestroy(ctx->cg_graph_cache.front().exec);
+      if (ctx->cg_graph_cache.front().exec) {
+        // the oldest graph may still run on the stream
+        RSG_HIP(hipStreamSynchronize(ctx->stream));
+        DeviceOpGuard guard;
+        (void)hipGraphExecDestroy(ctx->cg_graph_cache.front().exec);
+      }
       ctx->cg_graph_cache.erase(ctx->cg_graph_cache.begin());
     }
     ctx->cg_graph_cache.push_back(CgGraph());
@@ -1349,13 +1359,38 @@ template <typename F> int run_graphed(rsg_ctx *ctx, const std::vector<unsigned c
     return RSG_OK;
   }
   if (e->failed || e->seen++ == 0) return enqueue();
-  RSG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = enqueue();
+  int rc = RSG_OK;
+  hipError_t ce = hipErrorUnknown;
   hipGraph_t graph = nullptr;
-  const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-  if (rc == RSG_OK && ce == hipSuccess && graph &&
-      hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0) == hipSuccess) {
-    (void)hipGraphDestroy(graph);
+  {
+    // no allocation, free or library call of another thread may run while
+    // this stream records
+    CaptureGate gate;
+    if (!gate.usable()) {
+      e->failed = true;
+      return enqueue();
+    }
+    RSG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue();
+    ce = hipStreamEndCapture(ctx->stream, &graph);
+    // a capture the runtime invalidated can leave the stream recording:
+    // end it before anything else is queued there
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ctx->stream, &st) == hipSuccess && st != hipStreamCaptureStatusNone) {
+      hipGraph_t g2 = nullptr;
+      (void)hipStreamEndCapture(ctx->stream, &g2);
+      if (g2) (void)hipGraphDestroy(g2);
+      ce = hipErrorStreamCaptureInvalidated;
+    }
+    if (rc == RSG_OK && ce == hipSuccess && graph &&
+        hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      graph = nullptr;
+    } else {
+      e->exec = nullptr;
+    }
+  }
+  if (e->exec) {
     RSG_HIP(hipGraphLaunch(e->exec, ctx->stream));
     ++ctx->cg_graph_launches;
     return RSG_OK;
@@ -1363,10 +1398,13 @@ template <typename F> int run_graphed(rsg_ctx *ctx, const std::vector<unsigned c
   // the capture failed (or the work reported an error while recording):
   // nothing ran, so run the block eagerly and keep this key eager
   if (graph) (void)hipGraphDestroy(graph);
-  e->exec = nullptr;
   e->failed = true;
+  ++ctx->cg_graph_failures;
   (void)hipGetLastError();
   ctx->err.clear();
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(ctx->stream, &st) != hipSuccess || st != hipStreamCaptureStatusNone)
+    return ctx->fail(RSG_ERR_HIP, "conjugate_gradient: the stream is still recording after a failed graph capture");
   return enqueue();
 }
 
@@ -1657,6 +1695,7 @@ void solver_release(rsg_ctx *ctx) {
   ctx->dotpart.release();
   ctx->grampart.release();
   ctx->pcbuf.release();
+  DeviceOpGuard guard;
   for (auto &g : ctx->cg_graph_cache)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   ctx->cg_graph_cache.clear();

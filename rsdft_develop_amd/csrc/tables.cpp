@@ -12,6 +12,7 @@ namespace rsg {
 
 hipError_t DevBuf::ensure(size_t need) {
   if (need <= bytes && p) return hipSuccess;
+  DeviceOpGuard guard;
   release();
   if (need == 0) need = 256;
   hipError_t e = hipMalloc(&p, need);
@@ -25,7 +26,9 @@ hipError_t DevBuf::ensure(size_t need) {
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  DeviceOpGuard guard;
+  (void)hipFree(p);
   p = nullptr;
   bytes = 0;
 }

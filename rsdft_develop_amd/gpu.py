@@ -89,6 +89,7 @@ def load_library():
         "rsg_set_cg_swgs": (_c_int, [_vp, _c_int]),
         "rsg_set_cg_graphs": (_c_int, [_vp, _c_int]),
         "rsg_cg_graph_launches": (_c_int, [_vp, _P(_c_i64)]),
+        "rsg_cg_graph_failures": (_c_int, [_vp, _P(_c_i64)]),
         "rsg_autotune_gram": (_c_int, [_vp, _c_int, _c_int, _P(ctypes.c_float), _P(_c_i64)]),
         "rsg_axpby_f32": (_c_int, [_vp, _vp, _vp, ctypes.c_float, ctypes.c_float, _c_i64]),
         "rsg_precond_hprsdft_f32": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
@@ -300,6 +301,12 @@ class Context:
     def cg_graph_launches(self) -> int:
         n = _c_i64()
         self._check(self.L.rsg_cg_graph_launches(self.h, ctypes.byref(n)))
+        return n.value
+
+    def cg_graph_failures(self) -> int:
+        """Graph captures the runtime refused (those blocks ran directly)."""
+        n = _c_i64()
+        self._check(self.L.rsg_cg_graph_failures(self.h, ctypes.byref(n)))
         return n.value
 
     def set_cg_swgs(self, swgs: int):

@@ -16,6 +16,9 @@ def test_concurrent_cells_equal_sequential_cells():
     shapes = batch.ShapeCache(reps=1)
     par = batch.run_batch(cells, workers=3, params=params, shapes=shapes)
     seq = batch.run_batch(cells, workers=1, params=params, shapes=shapes)
+    # every worker captured its CG blocks while the others allocated, built
+    # tables and ran library calls: no capture may be refused
+    assert par.graph_launches > 0 and par.graph_failures == 0
     for a, b, c in zip(par.results, seq.results, cells):
         assert (a.converged, a.iterations) == (b.converged, b.iterations)
         assert max(abs(x - y) for x, y in zip(a.history, b.history)) / c.cell.natom < 1e-10
