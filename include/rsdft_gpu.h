@@ -150,6 +150,20 @@ int rsg_get_hpsi_order(rsg_ctx *ctx, int precision, int *order);
  * getter reports what an nb-band call would use (1 falls back to 0 where
  * the row tiles do not fit).  No reference counterpart. */
 int rsg_set_hpsi_proj(rsg_ctx *ctx, int precision, int proj);
+/* Where rsg_build_gth_projectors builds the analytic projector
+ * description (the set-up half of ps_nloc2_module.f90:42-1167 for
+ * l <= 1, n = 1 GTH sets): 0 (default) = with device kernels, 1 = on the
+ * host.  Both give the same arrays (factor tables up to the rounding of
+ * exp); takes effect at the next build. */
+int rsg_set_projector_setup(rsg_ctx *ctx, int on_host);
+/* Copy one array of the analytic description (FP64 form) to host memory
+ * and report its length: which = 0 sphere points, 1 segment begins,
+ * 2 segment ends, 3 segment atoms, 4 segment launch order, 5 segments per
+ * atom (offsets), 6 per-point hit words, 7 tile offsets, 8 tile atoms,
+ * 9 s factors, 10 p factors, 11 axis displacements, 12 amplitudes,
+ * 13 scales.  out may be NULL to query the length only.  For tests and
+ * inspection; no reference counterpart. */
+int rsg_get_analytic_array(rsg_ctx *ctx, int which, void *out, int64_t *count);
 int rsg_get_hpsi_proj(rsg_ctx *ctx, int precision, int *proj);
 /* Grid points along x per stencil thread: 1, or 2 for the pair kernel
  * (FP32 with md = 6, n1 even, tile rows dividing n2; elsewhere the call

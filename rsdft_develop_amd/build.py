@@ -28,11 +28,23 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", INCLUDE, "-I", CSRC, "-Wall", "-Wno-unused-result",
 ]
-SOURCES = ["capi.cpp", "tables.cpp", "hpsi_f64.hip", "hpsi_f32.hip", "solver.hip", "scf.hip", "lpc.hip"]
+SOURCES = ["capi.cpp", "tables.cpp", "setup.hip", "hpsi_f64.hip", "hpsi_f32.hip", "solver.hip", "scf.hip", "lpc.hip"]
 
 
-def _compile(src: str) -> str:
+def _stale(obj: str, src: str) -> bool:
+    """True when obj is older than its source or any header it may include."""
+    if not os.path.exists(obj):
+        return True
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    deps += [os.path.join(INCLUDE, f) for f in os.listdir(INCLUDE)]
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if incremental and not _stale(obj, src):
+        return obj
     lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
     cmd = [HIPCC, *COMMON, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -41,11 +53,13 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build_library(verbose: bool = False) -> str:
+def build_library(verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every source (or, with incremental, only those whose object is
+    older than the source or a header) and link librsdft_gpu.so."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda src: _compile(src, incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
            "-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]

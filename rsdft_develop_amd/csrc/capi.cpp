@@ -3,7 +3,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
-#include <shared_mutex>
+#include <condition_variable>
 
 #include "context.hpp"
 #include "lpc.hpp"
@@ -44,11 +44,42 @@ void *scratch_of(int device) {
 }
 
 // Process-wide gate between graph captures and the calls that may
-// invalidate them in other threads.  t_gate_depth counts this thread's
-// nested guards (and a capture in progress), so that only the outermost
-// one takes the lock: a shared lock taken twice by one thread can block
-// behind a waiting capture.
-std::shared_mutex g_capture_gate;
+// invalidate them in other threads: any number of guarded calls at once, or
+// one capture.  A waiting capture holds back new guarded calls, so that a
+// steady stream of them cannot starve it.  t_gate_depth counts this
+// thread's nested guards (and a capture in progress), so that only the
+// outermost one takes the gate.
+class CaptureGateLock {
+ public:
+  void lock_shared() {
+    std::unique_lock<std::mutex> l(m_);
+    cv_.wait(l, [&] { return !writer_ && writers_waiting_ == 0; });
+    ++readers_;
+  }
+  void unlock_shared() {
+    std::lock_guard<std::mutex> l(m_);
+    if (--readers_ == 0) cv_.notify_all();
+  }
+  void lock() {
+    std::unique_lock<std::mutex> l(m_);
+    ++writers_waiting_;
+    cv_.wait(l, [&] { return !writer_ && readers_ == 0; });
+    --writers_waiting_;
+    writer_ = true;
+  }
+  void unlock() {
+    std::lock_guard<std::mutex> l(m_);
+    writer_ = false;
+    cv_.notify_all();
+  }
+
+ private:
+  std::mutex m_;
+  std::condition_variable cv_;
+  int readers_ = 0, writers_waiting_ = 0;
+  bool writer_ = false;
+};
+CaptureGateLock g_capture_gate;
 thread_local int t_gate_depth = 0;
 
 }  // namespace
@@ -125,6 +156,8 @@ int rsg_destroy(rsg_ctx *ctx) {
   for (DevBuf *b : bufs) b->release();
   ctx->uvpart.release();
   ctx->cproj.release();
+  ctx->setup_tmp.release();
+  ctx->setup_scan.release();
   ctx->ana64.release_all();
   ctx->ana32.release_all();
   ctx->nl64.release_all();
@@ -322,12 +355,23 @@ int rsg_build_gth_projectors(rsg_ctx *ctx, int natom, const double *frac, const 
   ctx->have_nl = false;
   ctx->have_ana = false;
   ctx->tables_state = 1;
-  rsg::HostAnalytic h;
-  int rc = rsg::build_gth_analytic(ctx, natom, frac, kind, nkind, norb, lo, no, rps, hnl, h);
-  if (rc == RSG_ERR_UNSUPPORTED) return ensure_device_tables(ctx);  // the table form covers this set
-  if (rc) return ctx->fail(rc, "build_gth: invalid atom kinds");
-  if ((rc = rsg::upload_analytic(ctx, h))) return rc;
-  ctx->have_nl = h.ngroup > 0;
+  int rc;
+  if (ctx->ana_on_host) {
+    rsg::HostAnalytic h;
+    rc = rsg::build_gth_analytic(ctx, natom, frac, kind, nkind, norb, lo, no, rps, hnl, h);
+    if (rc == RSG_ERR_UNSUPPORTED) return ensure_device_tables(ctx);  // the table form covers this set
+    if (rc) return ctx->fail(rc, "build_gth: invalid atom kinds");
+    if ((rc = rsg::upload_analytic(ctx, h))) return rc;
+  } else {
+    rc = rsg::build_gth_analytic_device(ctx, natom, frac, kind, nkind, norb, lo, no, rps, hnl);
+    if (rc == RSG_ERR_UNSUPPORTED) {
+      ctx->have_ana = false;
+      return ensure_device_tables(ctx);  // the table form covers this set
+    }
+    if (rc == RSG_ERR_ARG) return ctx->fail(rc, "build_gth: invalid atom kinds");
+    if (rc) return rc;
+  }
+  ctx->have_nl = ctx->ana64.ngroup > 0;
   // the table form is built when a caller asks for it (tables, mode 1)
   if (ctx->nl_mode == 1) return ensure_device_tables(ctx);
   return RSG_OK;
@@ -457,7 +501,51 @@ int rsg_set_hpsi_proj(rsg_ctx *ctx, int precision, int proj) {
   if (!ctx) return RSG_ERR_ARG;
   if ((precision != 64 && precision != 32) || (proj != 0 && proj != 1))
     return ctx->fail(RSG_ERR_ARG, "set_hpsi_proj: precision 64/32 and proj 0/1");
+  if (proj == 1) {
+    RSG_HIP(hipSetDevice(ctx->device));
+    int rc = rsg::ensure_row_tiles(ctx);
+    if (rc) return rc;
+  }
   (precision == 64 ? ctx->shape64 : ctx->shape32).proj = proj;
+  return RSG_OK;
+}
+
+int rsg_set_projector_setup(rsg_ctx *ctx, int on_host) {
+  if (!ctx) return RSG_ERR_ARG;
+  ctx->ana_on_host = on_host != 0;
+  return RSG_OK;
+}
+
+int rsg_get_analytic_array(rsg_ctx *ctx, int which, void *out, int64_t *count) {
+  if (!ctx || !count) return RSG_ERR_ARG;
+  if (!ctx->have_ana) return ctx->fail(RSG_ERR_STATE, "get_analytic_array: no analytic projector description");
+  const rsg::AnalyticDev<double> &d = ctx->ana64;
+  const int ng = d.ngroup;
+  int64_t E = 0, ntg = 0;
+  RSG_HIP(hipSetDevice(ctx->device));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  if (d.nseg) RSG_HIP(hipMemcpy(&E, static_cast<const int *>(d.seg_end.p) + d.nseg - 1, 4, hipMemcpyDeviceToHost));
+  {
+    int t = 0;
+    RSG_HIP(hipMemcpy(&t, static_cast<const int *>(d.tile_off.p) + d.ntile, 4, hipMemcpyDeviceToHost));
+    ntg = t;
+  }
+  struct Item {
+    const rsg::DevBuf *b;
+    int64_t n;
+    size_t sz;
+  };
+  const Item items[] = {
+      {&d.pts, E, 4},          {&d.seg_beg, d.nseg, 4},     {&d.seg_end, d.nseg, 4},
+      {&d.seg_grp, d.nseg, 4}, {&d.seg_order, d.nseg, 4},   {&d.grp_seg_off, ng + 1, 4},
+      {&d.hits, ctx->N, 8},    {&d.tile_off, d.ntile + 1, 4}, {&d.tile_g, ntg, 4},
+      {&d.es, int64_t(ng) * d.S, 8}, {&d.ep, int64_t(ng) * d.S, 8}, {&d.dd, int64_t(ng) * d.S, 8},
+      {&d.amp, int64_t(ng) * 4, 8},  {&d.scale, int64_t(ng) * 4, 8}};
+  if (which < 0 || which >= int(sizeof(items) / sizeof(items[0])))
+    return ctx->fail(RSG_ERR_ARG, "get_analytic_array: which outside 0..13");
+  const Item &it = items[which];
+  *count = it.n;
+  if (out && it.n) RSG_HIP(hipMemcpy(out, it.b->p, size_t(it.n) * it.sz, hipMemcpyDeviceToHost));
   return RSG_OK;
 }
 

@@ -138,6 +138,20 @@ struct HostAnalytic {
   std::vector<int> rt_piece, rt_piece_off, rt_slot_off, rt_slot_gid, grp_slot_off;
 };
 
+// Per-atom parameters of the analytic form, shared by the host and the
+// device builders: atom position, nearest grid point, candidate box
+// half-widths, cutoffs, Gaussian exponents, amplitudes and signs.
+struct AnalyticGroup {
+  double R[3];
+  int ic[3], mm[3];
+  double rs = 0, rp = 0, c2s = 0, c2p = 0, amp_s = 0, amp_p = 0, sg_s = 1, sg_p = 1;
+  int has_s = 0, has_p = 0;
+};
+struct AnalyticGroups {
+  int S = 0;  // n1 + n2 + n3
+  std::vector<AnalyticGroup> g;  // atoms with projectors, in atom order
+};
+
 // Launch shape of the hpsi kernels for one grid and precision.
 struct HpsiShape {
   int ry = 2;     // rows per thread (y register blocking)
@@ -227,6 +241,9 @@ struct rsg_ctx {
   rsg::AnalyticDev<double> ana64;
   rsg::AnalyticDev<float> ana32;
   bool have_ana = false;  // analytic projector description available
+  bool ana_on_host = false;       // build the analytic description on the host (rsg_set_projector_setup)
+  bool ana_rows_pending = false;  // row tiles of a device-built description not built yet
+  rsg::DevBuf setup_tmp, setup_scan;  // device set-up scratch
   int nl_mode = 0;        // 0 auto, 1 tables, 2 analytic
   rsg::DevBuf cproj;      // analytic path: projections C[(g*4 + l) * chunk + b]
   rsg::DevBuf uvunk;    // nzlma x chunk projections uVunk, [lma][band]
@@ -278,6 +295,20 @@ struct rsg_ctx {
                                         rocblas_status_to_string(s_));     \
   } while (0)
 
+// A rocBLAS / rocSOLVER call that may allocate its workspace, held apart
+// from graph captures in other threads (DeviceOpGuard)
+#define RSG_BLAS_GUARDED(call)                                             \
+  do {                                                                     \
+    rocblas_status s_;                                                     \
+    {                                                                      \
+      DeviceOpGuard guard_;                                                \
+      s_ = (call);                                                         \
+    }                                                                      \
+    if (s_ != rocblas_status_success)                                      \
+      return ctx->fail(RSG_ERR_HIP, std::string(#call ": ") +              \
+                                        rocblas_status_to_string(s_));     \
+  } while (0)
+
 // tables.cpp
 namespace rsg {
 void fd_lap_closed_form(int md, double *lap);
@@ -291,6 +322,18 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac,
                        const int *lo, const int *no, const double *rps,
                        const double *hnl, HostAnalytic &out);
 int upload_analytic(rsg_ctx *ctx, const HostAnalytic &h);
+int analytic_groups(const rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind, const int *norb,
+                    const int *lo, const int *no, const double *rps, const double *hnl, AnalyticGroups &out);
+void build_row_tiles(const int *ng, HostAnalytic &out);
+// the row tiles of the projection for a description built on the device
+// (from its point list and segments), on first use
+int ensure_row_tiles(rsg_ctx *ctx);
+}  // namespace rsg
+
+// setup.hip
+namespace rsg {
+int build_gth_analytic_device(rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind,
+                              const int *norb, const int *lo, const int *no, const double *rps, const double *hnl);
 }  // namespace rsg
 
 // hpsi.hip

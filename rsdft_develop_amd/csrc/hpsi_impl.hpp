@@ -755,6 +755,8 @@ __global__ __launch_bounds__(256) void k_apply_m(
   using M = Mfma<T>;
   const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (tile >= ntile) return;  // whole wave
+  // accumulating onto a tile no sphere meets adds nothing: leave it alone
+  if (ACC && tile_off[tile] == tile_off[tile + 1]) return;  // whole wave
   const int l = threadIdx.x & 63;
   const int lo = l & 15, hi = l >> 4;
   const int b0 = blockIdx.y * 16 * NBT;
@@ -1356,9 +1358,12 @@ template <typename T> int autotune_t(rsg_ctx *ctx, int nb, int reps, float *best
       HpsiShape c = best;
       c.order = 1 - best.order;
       consider(c);
-      HpsiShape pj = best;
-      pj.proj = 1 - best.proj;
-      consider(pj);
+      // the row tiles of a description built on the device come on first use
+      if (ensure_row_tiles(ctx) == RSG_OK) {
+        HpsiShape pj = best;
+        pj.proj = 1 - best.proj;
+        consider(pj);
+      }
       for (int f : {4, 8}) {  // smaller chunks keep psi and H psi in the Infinity Cache
         HpsiShape d = best;
         d.chunk = std::max(1, nb / f);

@@ -1075,11 +1075,10 @@ int64_t gram_chunk(const rsg_ctx *ctx, int nb, int64_t N) {
 
 int gemm_tn_gram(rsg_ctx *ctx, int nb, int64_t N, double alpha, const double *A, const double *B, double *C,
                  int64_t kc_force = -1) {
-  DeviceOpGuard guard;
   const double zero = 0.0, one = 1.0;
   const int64_t kc = kc_force >= 0 ? kc_force : gram_chunk(ctx, nb, N);
   if (kc <= 0 || kc >= N) {
-    RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, int(N), &alpha, A,
+    RSG_BLAS_GUARDED(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, int(N), &alpha, A,
                            int(N), B, int(N), &zero, C, nb));
     return RSG_OK;
   }
@@ -1090,11 +1089,11 @@ int gemm_tn_gram(rsg_ctx *ctx, int nb, int64_t N, double alpha, const double *A,
   if (ctx->grampart.ensure(mat * nbat * sizeof(double)) != hipSuccess)
     return ctx->fail(RSG_ERR_HIP, "gram product: scratch allocation failed");
   double *part = ctx->grampart.as<double>();
-  RSG_BLAS(rocblas_dgemm_strided_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb,
+  RSG_BLAS_GUARDED(rocblas_dgemm_strided_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb,
                                          int(kc), &one, A, int(N), kc, B, int(N), kc, &zero, part, nb,
                                          rocblas_stride(mat), nfull));
   if (krem > 0)
-    RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, int(krem), &one,
+    RSG_BLAS_GUARDED(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, int(krem), &one,
                            A + int64_t(nfull) * kc, int(N), B + int64_t(nfull) * kc, int(N), &zero,
                            part + mat * nfull, nb));
   const int count = int(mat);
@@ -1160,7 +1159,6 @@ int autotune_gram(rsg_ctx *ctx, int nb, int reps, float *best_ms, int64_t *best_
 }
 
 int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
-  DeviceOpGuard guard;
   int rc = check_store(ctx, "gram_schmidt");
   if (rc) return rc;
   if (n0 < 1 || n1 > ctx->mb || n0 > n1) return ctx->fail(RSG_ERR_ARG, "gram_schmidt: band range outside 1..mb");
@@ -1176,7 +1174,7 @@ int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
   rc = gemm_tn_gram(ctx, nb, N, dv, U, U, S);
   if (rc) return rc;
   // S = R^T R (:286, PDPOTRF)
-  RSG_BLAS(rocsolver_dpotrf(ctx->blas, rocblas_fill_upper, nb, S, nb, ctx->info.as<rocblas_int>()));
+  RSG_BLAS_GUARDED(rocsolver_dpotrf(ctx->blas, rocblas_fill_upper, nb, S, nb, ctx->info.as<rocblas_int>()));
   int info = 0;
   rc = read_info(ctx, &info);
   if (rc) return rc;
@@ -1184,14 +1182,13 @@ int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
     return ctx->fail(RSG_ERR_NOTPD, "gram_schmidt: overlap not positive definite (leading minor " +
                                         std::to_string(info) + ")");
   // U <- U R^-1 (:290 PDTRTRI + :313 calc_dtrmm3, done as one triangular solve)
-  RSG_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+  RSG_BLAS_GUARDED(rocblas_dtrsm(ctx->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                          rocblas_diagonal_non_unit, int(N), nb, &one, S, nb, U, int(N)));
   RSG_HIP(hipStreamSynchronize(ctx->stream));
   return RSG_OK;
 }
 
 int subspace_diag(rsg_ctx *ctx, double *esp) {
-  DeviceOpGuard guard;
   int rc = check_store(ctx, "subspace_diag");
   if (rc) return rc;
   const int mb = ctx->mb;
@@ -1213,14 +1210,14 @@ int subspace_diag(rsg_ctx *ctx, double *esp) {
   hipLaunchKernelGGL(k_symmetrize, dim3((mb + 15) / 16, (mb + 15) / 16), dim3(16, 16), 0, ctx->stream, Hs, mb,
                      0.5 * ctx->dv);
   // eigen-decomposition, ascending (:155, DSYEV 'V','U')
-  RSG_BLAS(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, mb, Hs, mb, D, Ework,
+  RSG_BLAS_GUARDED(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, mb, Hs, mb, D, Ework,
                             ctx->info.as<rocblas_int>()));
   int info = 0;
   rc = read_info(ctx, &info);
   if (rc) return rc;
   if (info != 0) return ctx->fail(RSG_ERR_SOLVER, "subspace_diag: dsyevd did not converge");
   // rotation U <- U V (:190) into the work buffer, which becomes the store
-  RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, int(N), mb, mb, &one, U,
+  RSG_BLAS_GUARDED(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, int(N), mb, mb, &one, U,
                          int(N), Hs, mb, &zero, HU, int(N)));
   std::swap(ctx->unk, ctx->work);
   RSG_HIP(hipMemcpyAsync(esp, D, mb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1254,7 +1251,6 @@ namespace {
 // blocks go through an FP64 copy in tmp (N x nn doubles).
 template <typename W>
 int cggs_block(rsg_ctx *ctx, int swgs, int n0, int nn, W *pg, double *tmp) {
-  DeviceOpGuard guard;
   const int mb = ctx->mb;
   const int64_t N = ctx->N;
   if (ctx->cgcoef.ensure(size_t(mb) * nn * sizeof(double)) != hipSuccess)
@@ -1272,10 +1268,10 @@ int cggs_block(rsg_ctx *ctx, int swgs, int n0, int nn, W *pg, double *tmp) {
     hipLaunchKernelGGL((k_to_double<W>), g1, b1, 0, ctx->stream, pg, P, tot);
   }
   const double zero = 0.0, one = 1.0, mone = -1.0, dv = ctx->dv;
-  RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, mb, nn, int(N), &dv, U,
+  RSG_BLAS_GUARDED(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, mb, nn, int(N), &dv, U,
                          int(N), P, int(N), &zero, C, mb));
   hipLaunchKernelGGL(k_cggs_mask, dim3((mb + 255) / 256, nn), dim3(256), 0, ctx->stream, C, mb, nn, n0, swgs);
-  RSG_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, int(N), nn, mb, &mone, U,
+  RSG_BLAS_GUARDED(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, int(N), nn, mb, &mone, U,
                          int(N), C, mb, &one, P, int(N)));
   if constexpr (sizeof(W) != 8)
     hipLaunchKernelGGL((k_to_work<W>), g1, b1, 0, ctx->stream, P, pg, tot);
@@ -1295,9 +1291,10 @@ template <typename V> void key_put(std::vector<unsigned char> &k, const V &v) {
 
 template <typename T> void key_analytic(std::vector<unsigned char> &k, const AnalyticDev<T> &a) {
   for (const DevBuf *b : {&a.es, &a.ep, &a.dd, &a.amp, &a.scale, &a.pts, &a.seg_beg, &a.seg_end, &a.seg_grp,
-                          &a.seg_order, &a.grp_seg_off, &a.hits, &a.tile_off, &a.tile_g})
+                          &a.seg_order, &a.grp_seg_off, &a.hits, &a.tile_off, &a.tile_g, &a.rt_piece,
+                          &a.rt_piece_off, &a.rt_slot_off, &a.rt_slot_gid, &a.grp_slot_off})
     key_put(k, b->p);
-  for (int v : {a.ngroup, a.nseg, a.S, a.ntile}) key_put(k, v);
+  for (int v : {a.ngroup, a.nseg, a.S, a.ntile, a.rt_ty, a.rt_zr, a.rt_nslot, a.rt_nslot_max}) key_put(k, v);
 }
 
 template <typename T> void key_tables(std::vector<unsigned char> &k, const NonlocalDev<T> &t) {
@@ -1322,7 +1319,7 @@ std::vector<unsigned char> cg_graph_key(const rsg_ctx *ctx, int wsize, int ns, i
   key_tables(k, ctx->nl64);
   key_tables(k, ctx->nl32);
   for (const HpsiShape *h : {&ctx->shape64, &ctx->shape32})
-    for (int v : {h->ry, h->ty, h->zlen, h->chunk, h->bt, h->order, h->rx}) key_put(k, v);
+    for (int v : {h->ry, h->ty, h->zlen, h->chunk, h->bt, h->order, h->rx, h->proj}) key_put(k, v);
   for (double v : ctx->coef_lap) key_put(k, v);
   for (double v : {ctx->coef_lap0, ctx->dv, ctx->hgrid[0], ctx->hgrid[1], ctx->hgrid[2], ctx->ggg[0], ctx->ggg[1],
                    ctx->ggg[2]})

@@ -419,6 +419,8 @@ template <typename F> void parallel_range(int n, F &&f) {
 constexpr int kRowTilePoints = 288;  // rows per block ~ this many points / n1
 constexpr int kRowTileSlots = 32;    // at most this many atoms per block (LDS sums)
 
+}  // namespace
+
 void build_row_tiles(const int *ng, HostAnalytic &out) {
   const int ty = std::max(1, std::min(8, kRowTilePoints / ng[0]));
   const int ny = (ng[1] + ty - 1) / ty;
@@ -502,22 +504,20 @@ void build_row_tiles(const int *ng, HostAnalytic &out) {
   }
 }
 
-}  // namespace
-
-// Analytic description of GTH sets made only of (l = 0, n = 1) and
-// (l = 1, n = 1) projectors, at most one of each per species.  Returns
+// Per-atom parameters of the analytic description of GTH sets made only of
+// (l = 0, n = 1) and (l = 1, n = 1) projectors, at most one of each per
+// species (ps_nloc_gth_module.f90:19-180 for these forms).  Returns
 // RSG_ERR_UNSUPPORTED (not an error for the caller) when the set, the cell
 // or the grid does not fit that form; the table path then stays in use.
-int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind,
-                       const int *norb, const int *lo, const int *no, const double *rps, const double *hnl,
-                       HostAnalytic &out) {
-  out = HostAnalytic();
+int analytic_groups(const rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind, const int *norb,
+                    const int *lo, const int *no, const double *rps, const double *hnl, AnalyticGroups &out) {
+  out = AnalyticGroups();
   const double *aa = ctx->aa;
   if (aa[1] != 0.0 || aa[2] != 0.0 || aa[3] != 0.0 || aa[5] != 0.0 || aa[6] != 0.0 || aa[7] != 0.0)
     return RSG_ERR_UNSUPPORTED;
   const int *ng = ctx->n;
   if (ng[0] > 1023 || ng[1] > 1023 || ng[2] > 1023 || natom > 16382) return RSG_ERR_UNSUPPORTED;
-  // per species: orbital index of s and p (or -1), cutoffs, amplitudes
+  // per species: orbital index of s and p (or -1), cutoffs
   std::vector<int> qs(size_t(nkind), -1), qp(size_t(nkind), -1);
   std::vector<double> rcs(size_t(nkind), 0.0), rcp(size_t(nkind), 0.0);
   double rmax = 0.0;
@@ -535,17 +535,59 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
   }
   for (int t = 0; t < 3; ++t)
     if (!(aa[4 * t] > 2.0 * rmax + 2.0 * ctx->hgrid[t])) return RSG_ERR_UNSUPPORTED;
-  const int S = ng[0] + ng[1] + ng[2];
+  for (int a = 0; a < natom; ++a)
+    if (kind[a] < 0 || kind[a] >= nkind) return RSG_ERR_ARG;
+  out.S = ng[0] + ng[1] + ng[2];
+  const double pi4 = 4.0 * kPi;
+  // groups (atoms with projectors) in atom order
+  for (int a = 0; a < natom; ++a) {
+    const int ik = kind[a];
+    const int q0 = qs[size_t(ik)], q1 = qp[size_t(ik)];
+    if (q0 < 0 && q1 < 0) continue;
+    AnalyticGroup G;
+    const double *fa = frac + 3 * a;
+    // same arithmetic as build_gth_tables, so sphere membership agrees
+    G.R[0] = aa[0] * fa[0] + aa[3] * fa[1] + aa[6] * fa[2];
+    G.R[1] = aa[1] * fa[0] + aa[4] * fa[1] + aa[7] * fa[2];
+    G.R[2] = aa[2] * fa[0] + aa[5] * fa[1] + aa[8] * fa[2];
+    for (int t = 0; t < 3; ++t) G.ic[t] = nint_away(fa[t] * ng[t]);
+    G.has_s = q0 >= 0;
+    G.has_p = q1 >= 0;
+    G.rs = q0 >= 0 ? rcs[size_t(ik)] : 0.0;
+    G.rp = q1 >= 0 ? rcp[size_t(ik)] : 0.0;
+    const double rc = std::max(G.rs, G.rp);
+    for (int t = 0; t < 3; ++t) G.mm[t] = int(std::ceil(rc / ctx->hgrid[t])) + 2;
+    G.c2s = q0 >= 0 ? 0.5 / (rps[q0] * rps[q0]) : 0.0;
+    G.c2p = q1 >= 0 ? 0.5 / (rps[q1] * rps[q1]) : 0.0;
+    G.amp_s = q0 >= 0 ? std::sqrt(std::fabs(hnl[q0])) * std::sqrt(2.0) /
+                            (std::pow(rps[q0], 1.5) * std::sqrt(gamma_half(0, 1))) / std::sqrt(pi4)
+                      : 0.0;
+    G.amp_p = q1 >= 0 ? std::sqrt(std::fabs(hnl[q1])) * std::sqrt(2.0) /
+                            (std::pow(rps[q1], 2.5) * std::sqrt(gamma_half(1, 1))) * std::sqrt(3.0 / pi4)
+                      : 0.0;
+    G.sg_s = q0 >= 0 && hnl[q0] < 0.0 ? -1.0 : 1.0;
+    G.sg_p = q1 >= 0 && hnl[q1] < 0.0 ? -1.0 : 1.0;
+    out.g.push_back(G);
+  }
+  return RSG_OK;
+}
+
+// The analytic description built on the host from analytic_groups: factor
+// tables, sphere points, hit words, segments, tile lists and row tiles.
+int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind,
+                       const int *norb, const int *lo, const int *no, const double *rps, const double *hnl,
+                       HostAnalytic &out) {
+  out = HostAnalytic();
+  AnalyticGroups P;
+  const int prc = analytic_groups(ctx, natom, frac, kind, nkind, norb, lo, no, rps, hnl, P);
+  if (prc) return prc;
+  const double *aa = ctx->aa;
+  const int *ng = ctx->n;
+  const int S = P.S;
   const int axoff[3] = {0, ng[0], ng[0] + ng[1]};
   const double c[3] = {1.0 / ng[0], 1.0 / ng[1], 1.0 / ng[2]};
   const int64_t n12 = int64_t(ng[0]) * ng[1];
-  for (int a = 0; a < natom; ++a)
-    if (kind[a] < 0 || kind[a] >= nkind) return RSG_ERR_ARG;
-  // group (atom with projectors) numbering, in atom order
-  std::vector<int> atom_of;
-  for (int a = 0; a < natom; ++a)
-    if (qs[size_t(kind[a])] >= 0 || qp[size_t(kind[a])] >= 0) atom_of.push_back(a);
-  const int ngrp = int(atom_of.size());
+  const int ngrp = int(P.g.size());
   out.S = S;
   out.amp.assign(size_t(ngrp) * 4, 0.0);
   out.scale.assign(size_t(ngrp) * 4, 0.0);
@@ -558,33 +600,15 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
   // in group order, so the result does not depend on the thread count.
   std::vector<std::vector<std::pair<int64_t, uint32_t>>> gpts(static_cast<size_t>(ngrp));
   auto build_group = [&](int g) {
-    const int a = atom_of[size_t(g)];
-    const int ik = kind[a];
-    const int q0 = qs[size_t(ik)], q1 = qp[size_t(ik)];
-    const double *fa = frac + 3 * a;
-    // same arithmetic as build_gth_tables, so sphere membership agrees
-    const double R[3] = {aa[0] * fa[0] + aa[3] * fa[1] + aa[6] * fa[2],
-                         aa[1] * fa[0] + aa[4] * fa[1] + aa[7] * fa[2],
-                         aa[2] * fa[0] + aa[5] * fa[1] + aa[8] * fa[2]};
-    const int ic[3] = {nint_away(fa[0] * ng[0]), nint_away(fa[1] * ng[1]), nint_away(fa[2] * ng[2])};
-    const double rs = q0 >= 0 ? rcs[size_t(ik)] : 0.0, rp = q1 >= 0 ? rcp[size_t(ik)] : 0.0;
-    const double rc = std::max(rs, rp);
-    const double c2s = q0 >= 0 ? 0.5 / (rps[q0] * rps[q0]) : 0.0;
-    const double c2p = q1 >= 0 ? 0.5 / (rps[q1] * rps[q1]) : 0.0;
-    const double pi4 = 4.0 * kPi;
-    const double amp_s = q0 >= 0 ? std::sqrt(std::fabs(hnl[q0])) * std::sqrt(2.0) /
-                                       (std::pow(rps[q0], 1.5) * std::sqrt(gamma_half(0, 1))) / std::sqrt(pi4)
-                                 : 0.0;
-    const double amp_p = q1 >= 0 ? std::sqrt(std::fabs(hnl[q1])) * std::sqrt(2.0) /
-                                       (std::pow(rps[q1], 2.5) * std::sqrt(gamma_half(1, 1))) *
-                                       std::sqrt(3.0 / pi4)
-                                 : 0.0;
-    const double sg_s = q0 >= 0 && hnl[q0] < 0.0 ? -1.0 : 1.0, sg_p = q1 >= 0 && hnl[q1] < 0.0 ? -1.0 : 1.0;
+    const AnalyticGroup &G = P.g[size_t(g)];
+    const double *R = G.R;
+    const int *ic = G.ic;
+    const double rs = G.rs, rp = G.rp, c2s = G.c2s, c2p = G.c2p;
     const size_t g4 = size_t(g) * 4;
-    out.amp[g4] = amp_s;
-    out.amp[g4 + 1] = out.amp[g4 + 2] = out.amp[g4 + 3] = amp_p;
-    out.scale[g4] = q0 >= 0 ? sg_s * ctx->dv : 0.0;
-    out.scale[g4 + 1] = out.scale[g4 + 2] = out.scale[g4 + 3] = q1 >= 0 ? sg_p * ctx->dv : 0.0;
+    out.amp[g4] = G.amp_s;
+    out.amp[g4 + 1] = out.amp[g4 + 2] = out.amp[g4 + 3] = G.amp_p;
+    out.scale[g4] = G.has_s ? G.sg_s * ctx->dv : 0.0;
+    out.scale[g4 + 1] = out.scale[g4 + 2] = out.scale[g4 + 3] = G.has_p ? G.sg_p * ctx->dv : 0.0;
     // 1-D tables over each axis at the image nearest the atom
     const size_t base = size_t(g) * S;
     for (int t = 0; t < 3; ++t)
@@ -598,8 +622,7 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
         out.ep[base + size_t(axoff[t] + i)] = std::exp(-d * d * c2p);
       }
     // sphere points, in grid order
-    int mm[3];
-    for (int t = 0; t < 3; ++t) mm[t] = int(std::ceil(rc / ctx->hgrid[t])) + 2;
+    const int *mm = G.mm;
     std::vector<std::pair<int64_t, uint32_t>> &pts = gpts[size_t(g)];
     for (int o3 = -mm[2]; o3 <= mm[2]; ++o3)
       for (int o2 = -mm[1]; o2 <= mm[1]; ++o2)
@@ -610,8 +633,8 @@ int build_gth_analytic(const rsg_ctx *ctx, int natom, const double *frac, const 
           const double y = aa[1] * d1 + aa[4] * d2 + aa[7] * d3 - R[1];
           const double z = aa[2] * d1 + aa[5] * d2 + aa[8] * d3 - R[2];
           const double r2 = x * x + y * y + z * z;
-          const bool in_s = q0 >= 0 && r2 <= rs * rs + 1e-10;
-          const bool in_p = q1 >= 0 && r2 <= rp * rp + 1e-10;
+          const bool in_s = G.has_s && r2 <= rs * rs + 1e-10;
+          const bool in_p = G.has_p && r2 <= rp * rp + 1e-10;
           if (!in_s && !in_p) continue;
           int w[3];
           for (int t = 0; t < 3; ++t) w[t] = ((id[t] % ng[t]) + ng[t]) % ng[t];
@@ -752,7 +775,51 @@ int upload_analytic(rsg_ctx *ctx, const HostAnalytic &h) {
   int rc = upload_analytic_t<double>(ctx, ctx->ana64, h);
   if (!rc) rc = upload_analytic_t<float>(ctx, ctx->ana32, h);
   ctx->have_ana = rc == RSG_OK;
+  ctx->ana_rows_pending = false;
   return rc;
+}
+
+int ensure_row_tiles(rsg_ctx *ctx) {
+  if (!ctx->ana_rows_pending || !ctx->have_ana) return RSG_OK;
+  const AnalyticDev<double> &d = ctx->ana64;
+  HostAnalytic h;
+  h.ngroup = d.ngroup;
+  h.grp_seg_off.assign(size_t(d.ngroup) + 1, 0);
+  h.seg_beg.assign(size_t(d.nseg), 0);
+  h.seg_end.assign(size_t(d.nseg), 0);
+  hipStream_t st = ctx->stream;
+  RSG_HIP(hipMemcpyAsync(h.grp_seg_off.data(), d.grp_seg_off.p, h.grp_seg_off.size() * 4, hipMemcpyDeviceToHost, st));
+  RSG_HIP(hipMemcpyAsync(h.seg_beg.data(), d.seg_beg.p, h.seg_beg.size() * 4, hipMemcpyDeviceToHost, st));
+  RSG_HIP(hipMemcpyAsync(h.seg_end.data(), d.seg_end.p, h.seg_end.size() * 4, hipMemcpyDeviceToHost, st));
+  RSG_HIP(hipStreamSynchronize(st));
+  const size_t E = d.nseg ? size_t(h.seg_end.back()) : 0;
+  h.pts.assign(E, 0);
+  if (E) RSG_HIP(hipMemcpyAsync(h.pts.data(), d.pts.p, E * 4, hipMemcpyDeviceToHost, st));
+  RSG_HIP(hipStreamSynchronize(st));
+  build_row_tiles(ctx->n, h);
+  auto up_rows = [&](auto &dv) -> int {
+    auto up = [&](DevBuf &b, const std::vector<int> &v) -> int {
+      if (b.ensure(std::max<size_t>(v.size() * 4, 4)) != hipSuccess) return RSG_ERR_HIP;
+      if (!v.empty() && hipMemcpyAsync(b.p, v.data(), v.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+        return RSG_ERR_HIP;
+      return RSG_OK;
+    };
+    int rc = up(dv.rt_piece, h.rt_piece) | up(dv.rt_piece_off, h.rt_piece_off) | up(dv.rt_slot_off, h.rt_slot_off) |
+             up(dv.rt_slot_gid, h.rt_slot_gid) | up(dv.grp_slot_off, h.grp_slot_off);
+    dv.rt_ty = h.rt_ty;
+    dv.rt_zr = h.rt_zr;
+    dv.rt_ny = h.rt_ny;
+    dv.rt_nz = h.rt_nz;
+    dv.rt_nslot_max = h.rt_nslot_max;
+    dv.rt_nslot = h.grp_slot_off.empty() ? 0 : h.grp_slot_off.back();
+    return rc;
+  };
+  int rc = up_rows(ctx->ana64);
+  rc |= up_rows(ctx->ana32);
+  RSG_HIP(hipStreamSynchronize(st));
+  if (rc) return ctx->fail(RSG_ERR_HIP, "upload of the projection row tiles failed");
+  ctx->ana_rows_pending = false;
+  return RSG_OK;
 }
 
 }  // namespace rsg

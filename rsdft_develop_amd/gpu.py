@@ -82,6 +82,8 @@ def load_library():
         "rsg_set_hpsi_order": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_get_hpsi_order": (_c_int, [_vp, _c_int, _P(_c_int)]),
         "rsg_set_hpsi_proj": (_c_int, [_vp, _c_int, _c_int]),
+        "rsg_set_projector_setup": (_c_int, [_vp, _c_int]),
+        "rsg_get_analytic_array": (_c_int, [_vp, _c_int, _vp, _P(_c_i64)]),
         "rsg_get_hpsi_proj": (_c_int, [_vp, _c_int, _P(_c_int)]),
         "rsg_set_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_precond_hprsdft_f64": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_int]),
@@ -332,6 +334,25 @@ class Context:
     def set_hpsi_proj(self, precision=64, proj=0):
         """Analytic projections per sphere segment (0) or by row tiles (1)."""
         self._check(self.L.rsg_set_hpsi_proj(self.h, int(precision), int(proj)))
+
+    def set_projector_setup(self, on_host: bool = False):
+        """Build the analytic projector description on the host instead of
+        with device kernels (next build_gth_projectors)."""
+        self._check(self.L.rsg_set_projector_setup(self.h, int(bool(on_host))))
+
+    ANALYTIC_ARRAYS = ("pts", "seg_beg", "seg_end", "seg_grp", "seg_order", "grp_seg_off", "hits", "tile_off",
+                       "tile_g", "es", "ep", "dd", "amp", "scale")
+
+    def analytic_array(self, name: str) -> np.ndarray:
+        """One array of the analytic projector description (FP64 form)."""
+        which = self.ANALYTIC_ARRAYS.index(name)
+        n = _c_i64()
+        self._check(self.L.rsg_get_analytic_array(self.h, which, None, ctypes.byref(n)))
+        dt = {"pts": np.uint32, "hits": np.uint64}.get(name, np.float64 if which >= 9 else np.int32)
+        out = np.zeros(max(n.value, 0), dtype=dt)
+        if n.value:
+            self._check(self.L.rsg_get_analytic_array(self.h, which, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n)))
+        return out
 
     def hpsi_proj(self, precision=64) -> int:
         o = _c_int()

@@ -63,6 +63,7 @@ class ScfResult:
     eewald: float
     wall: float
     setup: float = 0.0      # set-up seconds before the first iteration (tables, density, start bands, GS)
+    setup_parts: dict = dataclasses.field(default_factory=dict)  # the same by part
 
     @property
     def niter(self) -> int:
@@ -112,15 +113,27 @@ class SCF:
     def __init__(self, cell: systems.Cell, params: ScfParams | None = None, ctx: gpu.Context | None = None,
                  bands: np.ndarray | None = None, rho: np.ndarray | None = None):
         t_setup = time.perf_counter()
+        parts = {}
+        mark = [t_setup]
+
+        def lap(name):  # wall seconds of each set-up part, after the device finished it
+            self.ctx.synchronize()
+            t = time.perf_counter()
+            parts[name] = t - mark[0]
+            mark[0] = t
+
         self.cell = cell
         self.p = params or ScfParams()
         self.ctx = ctx or gpu.Context(0)
         c = self.ctx
         c.cell = cell
         c.set_grid(cell.ngrid, cell.aa, cell.md)
+        lap("grid")
         c.build_gth_projectors(cell)
+        lap("projectors")
         c.set_local_gth(cell)
         self.eewald, self.nelectron = c.scf_constants()
+        lap("local_potential_ewald")
         if rho is None:
             c.init_density(self.p.init_width)
         else:  # restart density (IC with vrho.dat1)
@@ -129,6 +142,7 @@ class SCF:
         c.update_vloc()
         if self.p.pc_kind in (5, 6):
             c.lpc_set_potential()  # the network's potential input follows Vloc
+        lap("density_potential")
         c.alloc_bands(cell.nband)
         if bands is None and self.p.start == "uniform":
             c.random_bands(self.p.seed)
@@ -137,10 +151,12 @@ class SCF:
                 raise ValueError(f"start must be 'normal' or 'uniform', got {self.p.start!r}")
             u0 = bands if bands is not None else systems.random_bands(cell.npoints, cell.nband, seed=self.p.seed)
             c.upload_bands(u0)
+        lap("start_bands")
         c.gram_schmidt(1, cell.nband)   # main.f90:392
         c.mix_init(self.p.mmix, self.p.beta, self.p.imix)
-        c.synchronize()
+        lap("first_gram_schmidt")
         self.setup_s = time.perf_counter() - t_setup
+        self.setup_parts = parts
 
     def step(self, it: int, e_prev: float | None, precision: int = 64) -> ScfIteration:
         c, p = self.ctx, self.p
@@ -276,4 +292,5 @@ class SCF:
             if conv:
                 break
         wall = time.perf_counter() - w0
-        return ScfResult(conv, its, its[-1].etot, its[-1].fene, its[-1].esp, self.occ, self.eewald, wall, self.setup_s)
+        return ScfResult(conv, its, its[-1].etot, its[-1].fene, its[-1].esp, self.occ, self.eewald, wall, self.setup_s,
+                         dict(getattr(self, "setup_parts", {})))
