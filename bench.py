@@ -207,6 +207,10 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
                esp=np.sort(res.esp).tolist())
     if pc_kind in (5, 6):
         out["descent_check_fallbacks"] = ctx.lpc_fallbacks() - fb0
+    if len(its) > 1:  # how far the band energies still move per iteration at the stop
+        nocc = int(round(cell.nelectron / 2))
+        out["last_iteration_eig_change_occupied_Ha"] = float(
+            np.abs(np.sort(its[-1].esp)[:nocc] - np.sort(its[-2].esp)[:nocc]).max())
     run.p.pc_kind = 4  # the converged levels do not depend on the preconditioner
     levels, passes = run.refine_levels(tol=1e-9, max_passes=12)
     out["esp_levels"] = np.sort(levels).tolist()
@@ -384,6 +388,7 @@ def main() -> int:
     ap.add_argument("--si512", type=int, default=2, help="SCF steps of the Si512 per-step leg (0 skips)")
     ap.add_argument("--batch", type=int, default=64, help="cells of the batched-SCF leg (0 skips)")
     ap.add_argument("--batch-workers", type=int, default=8, help="concurrent contexts of the batched-SCF leg")
+    ap.add_argument("--lpc", type=int, default=1, help="learned-preconditioner leg on the held-out Si64 cell (0 skips)")
     ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx]")
     ap.add_argument("--shape32", default=None, help="pin the FP32 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx]")
     args = ap.parse_args()
@@ -405,7 +410,7 @@ def main() -> int:
     r32 = gpu_leg(cell, vloc, args.steps, args.warmup, np.float32, pin(args.shape32))
     batch_out = batch_leg(args.batch, args.batch_workers) if args.batch > 0 else None
     progress("learned preconditioner")
-    lpc_out = lpc_leg()
+    lpc_out = lpc_leg() if args.lpc else None
     progress("CPU baseline")
     cpu = None if args.no_cpu else cpu_leg(cell, vloc)
     progress("Si8 SCF on the GPU and on the host")

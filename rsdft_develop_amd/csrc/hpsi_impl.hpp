@@ -160,7 +160,7 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       v[sidx][r] = vloc[colofs[r] + zo];
-      if (ACC) a[sidx][r] = o[colofs[r] + zo];
+      if (ACC) a[sidx][r] = __builtin_nontemporal_load(o + colofs[r] + zo);
     }
   }
   T *dst_dead = dead + (tid & 63);
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
           v[sn][r] = vloc[colofs[r] + zo];
-          if (ACC) a[sn][r] = o[colofs[r] + zo];
+          if (ACC) a[sn][r] = __builtin_nontemporal_load(o + colofs[r] + zo);
         }
         const unsigned zf = unsigned(zq) * n12;
 #pragma unroll
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
         T acc = ((sx[0] + sx[1]) + (sy[0] + sy[1])) + (sz[0] + sz[1]);
         if (ACC) acc += a[sc][r];
         T *dst = own[r] ? o + colofs[r] + zb2 : dst_dead;
-        *dst = acc;
+        __builtin_nontemporal_store(acc, dst);
       }
     }
   }
@@ -234,6 +234,27 @@ __global__ __launch_bounds__(512) void k_stencil(const T *psi, T *hpsi, int64_t 
 template <typename T> struct PairOf;
 template <> struct PairOf<float> { typedef float2 V; };
 template <> struct PairOf<double> { typedef double2 V; };
+
+// H psi is streamed: its accumulated input is read once and the result is
+// not read again by this launch, so both go past the caches with
+// non-temporal hints, which leaves L2 to the psi rows that neighbouring
+// tiles share as halo.
+template <typename T> struct NativePair;
+template <> struct NativePair<float> { typedef float V __attribute__((ext_vector_type(2))); };
+template <> struct NativePair<double> { typedef double V __attribute__((ext_vector_type(2))); };
+template <typename T> __device__ inline typename PairOf<T>::V nt_load_pair(const T *p) {
+  const typename NativePair<T>::V n = __builtin_nontemporal_load(reinterpret_cast<const typename NativePair<T>::V *>(p));
+  typename PairOf<T>::V v;
+  v.x = n.x;
+  v.y = n.y;
+  return v;
+}
+template <typename T> __device__ inline void nt_store_pair(T *p, const typename PairOf<T>::V &v) {
+  typename NativePair<T>::V n;
+  n.x = v.x;
+  n.y = v.y;
+  __builtin_nontemporal_store(n, reinterpret_cast<typename NativePair<T>::V *>(p));
+}
 
 template <class F, int... U> __device__ inline void run_steps(F &f, int zb, std::integer_sequence<int, U...>) {
   (void)(f(zb, std::integral_constant<int, U>{}) && ...);
@@ -304,7 +325,7 @@ __global__ __launch_bounds__(512) void k_stencil2(const T *psi, T *hpsi, int64_t
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       v[sidx][r] = *reinterpret_cast<const V *>(vloc + zo + colofs[r]);
-      if (ACC) a[sidx][r] = *reinterpret_cast<const V *>(o + zo + colofs[r]);
+      if (ACC) a[sidx][r] = nt_load_pair(o + zo + colofs[r]);
     }
   }
   V *odead = reinterpret_cast<V *>(dead) + (tid & 63);
@@ -334,7 +355,7 @@ __global__ __launch_bounds__(512) void k_stencil2(const T *psi, T *hpsi, int64_t
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
         v[sn][r] = *reinterpret_cast<const V *>(vloc + zo + colofs[r]);
-        if (ACC) a[sn][r] = *reinterpret_cast<const V *>(o + zo + colofs[r]);
+        if (ACC) a[sn][r] = nt_load_pair(o + zo + colofs[r]);
       }
       const T *wq = w + unsigned(zq) * n12;
 #pragma unroll
@@ -381,7 +402,7 @@ __global__ __launch_bounds__(512) void k_stencil2(const T *psi, T *hpsi, int64_t
       V res;
       res.x = a0;
       res.y = a1;
-      *(active ? reinterpret_cast<V *>(oz + colofs[r]) : odead) = res;
+      nt_store_pair(active ? reinterpret_cast<T *>(oz + colofs[r]) : reinterpret_cast<T *>(odead), res);
     }
     return true;
   };
@@ -776,7 +797,7 @@ __global__ __launch_bounds__(256) void k_apply_m(
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int band = b0 + 16 * t + M::row(l, r);
-      old[t][r] = (ACC && inside && band < nbc) ? hpsi[j + int64_t(band) * ld] : T(0);
+      old[t][r] = (ACC && inside && band < nbc) ? __builtin_nontemporal_load(hpsi + j + int64_t(band) * ld) : T(0);
     }
   const int a0 = tile_off[tile], a1 = tile_off[tile + 1];
   for (int a = a0; a < a1; ++a) {
@@ -805,7 +826,7 @@ __global__ __launch_bounds__(256) void k_apply_m(
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int band = b0 + 16 * t + M::row(l, r);
-      if (band < nbc) hpsi[j + int64_t(band) * ld] = ACC ? old[t][r] + acc[t][r] : acc[t][r];
+      if (band < nbc) __builtin_nontemporal_store(ACC ? old[t][r] + acc[t][r] : acc[t][r], hpsi + j + int64_t(band) * ld);
     }
 }
 
