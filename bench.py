@@ -202,8 +202,7 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
                pc_kind=pc_kind, gram_chunk=gram[1], gram_ms=round(gram[0], 3),
                etot=res.etot,
                fp32_iterations=sum(1 for r in its if r.precision == 32),
-               per_phase_s={k: round(sum(r.times.get(k, 0.0) for r in its), 3)
-                            for k in ("sd", "cg", "gs", "esp", "eto", "pot")},
+               per_phase_s={k: round(sum(r.times.get(k, 0.0) for r in its), 3) for k in PHASES},
                esp=np.sort(res.esp).tolist())
     if pc_kind in (5, 6):
         out["descent_check_fallbacks"] = ctx.lpc_fallbacks() - fb0
@@ -219,17 +218,26 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     return out
 
 
+PHASES = ("sd", "cg", "gs", "esp", "eto", "pot")
+
+
+def mean_phases(times: list) -> dict:
+    """Mean seconds per SCF step in the reference log's categories."""
+    return {k: round(sum(t.get(k, 0.0) for t in times) / max(1, len(times)), 5) for k in PHASES}
+
+
 def scf_config0_leg():
     """The Si8 configuration (36^3, 20 bands, EKBT 0.01 Ha, IMIX 30, device
-    start bands) solved to dE < 1e-6 Ha/atom twice: by the library on the
+    start bands) solved to dE < 1e-8 Ha/atom twice: by the library on the
     GPU and by the oracle's FP64 CPU restatement on all host cores, with the
-    same start bands.  Wall times of whole SCF runs and their energy
-    difference."""
+    same start bands.  Wall times of whole SCF runs, the mean time per step
+    in the reference log's categories (sd/cg/gs/esp/eto/pot), and the two
+    energies' difference."""
     from oracle import pyoracle, scf_oracle
     from rsdft_develop_amd import gpu, scf, systems
 
     cell = systems.silicon_supercell(1, nband=20, displacement=0.0, seed=0)
-    params = dict(ekbt=0.01, etol_per_atom=1e-6, max_iter=80, imix=30, start="uniform")
+    params = dict(ekbt=0.01, etol_per_atom=1e-8, max_iter=80, imix=30, start="uniform")
     ctx = gpu.Context(0)
     scf.SCF(cell, scf.ScfParams(**params), ctx=ctx).run()  # first use of this size, untimed
     t0 = time.perf_counter()
@@ -241,11 +249,34 @@ def scf_config0_leg():
     t0 = time.perf_counter()
     c = scf_oracle.run_scf(cell, **params)
     cpu_s = time.perf_counter() - t0
-    return {"workload": "Si8 36^3, 20 bands, EKBT 0.01 Ha, IMIX 30, to dE < 1e-6 Ha/atom",
-            "gpu_seconds": round(gpu_s, 3), "gpu_iterations": g.niter,
+    return {"workload": "Si8 36^3, 20 bands, EKBT 0.01 Ha, IMIX 30, to dE < 1e-8 Ha/atom",
+            "gpu_seconds": round(gpu_s, 3), "gpu_iterations": g.niter, "gpu_converged": bool(g.converged),
+            "gpu_per_step_s": mean_phases([r.times for r in g.iterations]),
             "cpu_seconds": round(cpu_s, 3), "cpu_cores": cores, "cpu_kind": "port",
-            "cpu_iterations": len(c.get("history", [])),
+            "cpu_iterations": len(c.get("history", [])), "cpu_converged": bool(c.get("converged")),
+            "cpu_per_step_s": mean_phases([h["times"] for h in c["history"]]),
             "etot_diff_Ha_per_atom": abs(float(g.etot) - float(c["etot"])) / cell.natom}
+
+
+def cpu_si216_step_leg(steps: int):
+    """Per-SCF-step time of the oracle's FP64 restatement at Si216 (108^3,
+    480 bands, EKBT 0.01 Ha, IMIX 30) on all host cores, in the reference
+    log's categories: `steps` steps only, no extrapolation to convergence."""
+    from oracle import pyoracle, scf_oracle
+    from rsdft_develop_amd import systems
+
+    cell = systems.silicon_supercell(3, nband=NBANDS[3], displacement=0.05, seed=0)
+    cores = physical_cores()
+    pyoracle.set_threads(cores)
+    t0 = time.perf_counter()
+    progress("cpu Si216: set-up (tables, start bands, Gram-Schmidt)")
+    c = scf_oracle.run_scf(cell, ekbt=0.01, etol_per_atom=0.0, max_iter=steps, imix=30, start="uniform",
+                           phase_callback=lambda name, sec: progress(f"cpu Si216 {name} done ({sec:.1f} s)"),
+                           callback=lambda h: progress(f"cpu Si216 step {h['iter']}: "
+                                                       + " ".join(f"{k} {v:.1f}" for k, v in h["times"].items())))
+    return {"workload": f"Si216 108^3, 480 bands, {steps} SCF steps of the oracle (FP64) on {cores} cores",
+            "seconds_with_setup": round(time.perf_counter() - t0, 2), "cores": cores, "kind": "port",
+            "per_step_s": [{k: round(v, 3) for k, v in h["times"].items()} for h in c["history"]]}
 
 
 def scf_warmup(nrep: int):
@@ -389,6 +420,8 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=64, help="cells of the batched-SCF leg (0 skips)")
     ap.add_argument("--batch-workers", type=int, default=8, help="concurrent contexts of the batched-SCF leg")
     ap.add_argument("--lpc", type=int, default=1, help="learned-preconditioner leg on the held-out Si64 cell (0 skips)")
+    ap.add_argument("--cpu-si216-steps", type=int, default=0,
+                    help="SCF steps of the oracle at Si216 on the host cores (0 skips; about a minute per step)")
     ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx]")
     ap.add_argument("--shape32", default=None, help="pin the FP32 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx]")
     args = ap.parse_args()
@@ -415,6 +448,7 @@ def main() -> int:
     cpu = None if args.no_cpu else cpu_leg(cell, vloc)
     progress("Si8 SCF on the GPU and on the host")
     scf0 = None if args.no_cpu else scf_config0_leg()
+    cpu216 = cpu_si216_step_leg(args.cpu_si216_steps) if args.cpu_si216_steps > 0 else None
     scf_out = None
     if args.scf > 0:
         progress("scf warm-up")
@@ -479,6 +513,7 @@ def main() -> int:
                 "kernel_traffic": kernel_traffic("f32")},
         "cpu_baseline": cpu,
         "scf_si8_gpu_vs_cpu": scf0,
+        "cpu_si216_steps": cpu216,
         "scf": scf_out,
         "si512_step": step_out,
         "batch": batch_out,
