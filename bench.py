@@ -258,6 +258,52 @@ def scf_config0_leg():
             "etot_diff_Ha_per_atom": abs(float(g.etot) - float(c["etot"])) / cell.natom}
 
 
+def precision_si64_leg():
+    """Configuration 1's precision report: the Si64 cell (72^3, 160 bands,
+    displacements N(0, 0.05 bohr)) solved to dE < 1e-6 Ha/atom in plain FP64
+    and in mixed precision (FP32 hpsi and CG, FP64 Gram-Schmidt, subspace
+    eigensolve, density and energies), IMIX 30, EKBT 0.01 Ha.  Energy and
+    band-energy differences of the mixed run against the FP64 one, as the
+    runs leave them and after 12 fixed-potential passes, with the FP64
+    run's own last-iteration level change for scale."""
+    from rsdft_develop_amd import gpu, scf, systems
+
+    cell = systems.silicon_supercell(2, nband=NBANDS[2], displacement=0.05, seed=0)
+    nocc = int(round(cell.nelectron / 2))
+    out = {"workload": "Si64 72^3, 160 bands, EKBT 0.01 Ha, IMIX 30, to dE < 1e-6 Ha/atom"}
+    res = {}
+    for mixed in (False, True):
+        ctx = gpu.Context(0)
+        ctx.set_grid(cell.ngrid, cell.aa, cell.md)
+        ctx.build_gth_projectors(cell)
+        ctx.autotune_hpsi(64, cell.nband, reps=1)
+        ctx.autotune_hpsi(32, cell.nband, reps=1)
+        t0 = time.perf_counter()
+        run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=1e-6, max_iter=80, mixed=mixed, imix=30,
+                                          start="uniform"), ctx=ctx)
+        r = run.run()
+        sec = time.perf_counter() - t0
+        levels, _ = run.refine_levels(tol=1e-9, max_passes=12)
+        its = r.iterations
+        res[mixed] = dict(esp=np.sort(r.esp), levels=np.sort(levels), etot=r.etot)
+        key = "mixed" if mixed else "fp64"
+        out[key] = dict(seconds=round(sec, 3), iterations=r.niter, converged=bool(r.converged),
+                        fp32_iterations=sum(1 for x in its if x.precision == 32))
+        if len(its) > 1:
+            out[key]["last_iteration_eig_change_occupied_Ha"] = float(
+                np.abs(np.sort(its[-1].esp)[:nocc] - np.sort(its[-2].esp)[:nocc]).max())
+        ctx.close()
+    f, m = res[False], res[True]
+    out["mixed"].update(
+        etot_diff_vs_fp64_Ha_per_atom=abs(m["etot"] - f["etot"]) / cell.natom,
+        max_eig_diff_vs_fp64_occupied_Ha=float(np.abs(m["esp"] - f["esp"])[:nocc].max()),
+        max_eig_diff_vs_fp64_Ha=float(np.abs(m["esp"] - f["esp"]).max()),
+        max_level_diff_vs_fp64_occupied_Ha=float(np.abs(m["levels"] - f["levels"])[:nocc].max()),
+        max_level_diff_vs_fp64_Ha=float(np.abs(m["levels"] - f["levels"]).max()),
+        speedup_vs_fp64=round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3))
+    return out
+
+
 def cpu_si216_step_leg(steps: int):
     """Per-SCF-step time of the oracle's FP64 restatement at Si216 (108^3,
     480 bands, EKBT 0.01 Ha, IMIX 30) on all host cores, in the reference
@@ -420,6 +466,7 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=64, help="cells of the batched-SCF leg (0 skips)")
     ap.add_argument("--batch-workers", type=int, default=8, help="concurrent contexts of the batched-SCF leg")
     ap.add_argument("--lpc", type=int, default=1, help="learned-preconditioner leg on the held-out Si64 cell (0 skips)")
+    ap.add_argument("--precision64", type=int, default=1, help="Si64 FP64-vs-mixed SCF precision leg (0 skips)")
     ap.add_argument("--cpu-si216-steps", type=int, default=0,
                     help="SCF steps of the oracle at Si216 on the host cores (0 skips; about a minute per step)")
     ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx]")
@@ -449,6 +496,9 @@ def main() -> int:
     progress("Si8 SCF on the GPU and on the host")
     scf0 = None if args.no_cpu else scf_config0_leg()
     cpu216 = cpu_si216_step_leg(args.cpu_si216_steps) if args.cpu_si216_steps > 0 else None
+    if args.precision64:
+        progress("Si64 precision: FP64 and mixed SCF")
+    prec64 = precision_si64_leg() if args.precision64 else None
     scf_out = None
     if args.scf > 0:
         progress("scf warm-up")
@@ -514,6 +564,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "scf_si8_gpu_vs_cpu": scf0,
         "cpu_si216_steps": cpu216,
+        "precision_si64": prec64,
         "scf": scf_out,
         "si512_step": step_out,
         "batch": batch_out,

@@ -329,7 +329,8 @@ int rsg_mix_pulay(rsg_ctx *ctx);
  * convolution net on the periodic grid, P g = s * net(g/s, v), s =
  * rms(g), v the cell's normalised Vloc; bfloat16 weights and activations,
  * FP32 accumulation on MFMA.  A block whose <Pg, g> is not positive falls
- * back to precond_hprsdft.  Parameters (FP32, this order):
+ * back to precond_hprsdft, for that step and the rest of the CG call.
+ * Parameters (FP32, this order):
  * W1[16][2][27] b1[16] W2[16][16][27] b2[16] W3[1][16][27] b3[1], W[co][ci]
  * [tap], tap = d3*9 + d2*3 + d1 (offsets d - 1 along z, y, x). */
 int rsg_lpc_param_count(void);

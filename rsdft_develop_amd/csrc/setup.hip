@@ -32,7 +32,6 @@ struct GroupDev {
 
 // 1-D factor tables per group and axis at the image nearest the atom
 // (the host builder's loop, one thread per entry).
-template <typename T>
 __global__ void k_axis_tables(const GroupDev *__restrict__ grp, int ngrp, int n1, int n2, int n3, double a1, double a2,
                               double a3, double c1, double c2, double c3, double *__restrict__ es64,
                               double *__restrict__ ep64, double *__restrict__ dd64, float *__restrict__ es32,
@@ -304,7 +303,7 @@ int build_gth_analytic_device(rsg_ctx *ctx, int natom, const double *frac, const
     if (b->ensure(nS * 8) != hipSuccess) return ctx->fail(RSG_ERR_HIP, "build_gth: allocation failed");
   for (DevBuf *b : {&d32.es, &d32.ep, &d32.dd})
     if (b->ensure(nS * 4) != hipSuccess) return ctx->fail(RSG_ERR_HIP, "build_gth: allocation failed");
-  hipLaunchKernelGGL(k_axis_tables<double>, dim3(unsigned((nS + 255) / 256)), dim3(256), 0, st, dgrp, ngrp, ng[0],
+  hipLaunchKernelGGL(k_axis_tables, dim3(unsigned((nS + 255) / 256)), dim3(256), 0, st, dgrp, ngrp, ng[0],
                      ng[1], ng[2], a1, a2, a3, c1, c2, c3, d64.es.as<double>(), d64.ep.as<double>(),
                      d64.dd.as<double>(), d32.es.as<float>(), d32.ep.as<float>(), d32.dd.as<float>());
   hipLaunchKernelGGL(k_sphere_count, dim3(unsigned(nblk)), dim3(kSetupThreads), 0, st, dgrp, b3max, doff, dwrp, a1, a2,
@@ -417,16 +416,10 @@ int build_gth_analytic_device(rsg_ctx *ctx, int natom, const double *frac, const
   e |= dup(d32.grp_seg_off, d64.grp_seg_off, grp_seg_off.size() * 4);
   e |= dup(d32.seg_order, d64.seg_order, seg_order.size() * 4);
   if (e) return ctx->fail(RSG_ERR_HIP, "build_gth: upload of analytic projector data failed");
-  for (AnalyticDev<double> *d : {&d64}) {
-    d->ngroup = ngrp;
-    d->nseg = nseg;
-    d->S = S;
-    d->ntile = int(ntile);
-  }
-  d32.ngroup = ngrp;
-  d32.nseg = nseg;
-  d32.S = S;
-  d32.ntile = int(ntile);
+  d64.ngroup = d32.ngroup = ngrp;
+  d64.nseg = d32.nseg = nseg;
+  d64.S = d32.S = S;
+  d64.ntile = d32.ntile = int(ntile);
   // row tiles of the projection are built from these arrays on first use
   for (int *v : {&d64.rt_ty, &d64.rt_zr, &d64.rt_ny, &d64.rt_nz, &d64.rt_nslot_max, &d64.rt_nslot, &d32.rt_ty,
                  &d32.rt_zr, &d32.rt_ny, &d32.rt_nz, &d32.rt_nslot_max, &d32.rt_nslot})

@@ -1440,6 +1440,7 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
   // pc_kind 0 and 4 (the learned ones read back their descent check)
   auto enqueue_block = [&](int ns, int nn) -> int {
     int rc = RSG_OK;
+    bool net_on = true;  // learned preconditioner still in use for this block
     double *xs64 = ctx->unk.as<double>() + size_t(ns) * N;
     const dim3 gN = grid_nb(N, nn), blkd(kThreads), gb((nn + 63) / 64), b64(64);
     W *x;
@@ -1471,10 +1472,14 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
       // |E - E1| < 1e-15 the remaining steps leave it as it is (the
       // reference exits the loop), without a read-back per step
       hipLaunchKernelGGL(k_cg_conv, dim3(1), dim3(256), 0, ctx->stream, s.E, s.E1, nn, s.flags);
-      // pgk = P gk and rb = dV <pgk, gk>
-      if (pc_kind == 4 || pc_kind == 6) {
+      // pgk = P gk and rb = dV <pgk, gk>; once the network's output has
+      // failed the descent check in this call, the block keeps the hprsdft
+      // smoother for the rest of the call
+      const bool use_net = (pc_kind == 5 || pc_kind == 6) && net_on;
+      const int pc_step = use_net ? pc_kind : (pc_kind == 0 ? 0 : 4);
+      if (pc_step == 4 || pc_step == 6) {
         if ((rc = precond_hprsdft_fused<W>(ctx, mloop, gk, pgk, nn, pr, pp, pf, s, dv, s.rb))) return rc;
-        if (pc_kind == 6) {
+        if (pc_step == 6) {
           // hybrid: pgk = scale * P_hprsdft g + network(g); the
           // preconditioner's work vectors are free again, pf takes the net
           if ((rc = lpc_apply(ctx, gk, pf, N, nn, kMixed ? 32 : 64))) return rc;
@@ -1485,7 +1490,7 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
           if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
         }
       } else {
-        if (pc_kind == 5) {
+        if (pc_step == 5) {
           if ((rc = lpc_apply(ctx, gk, pgk, N, nn, kMixed ? 32 : 64))) return rc;
         } else {
           const int64_t tot = N * nn;
@@ -1495,7 +1500,7 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
         const W *xs[1] = {pgk}, *ys[1] = {gk};
         if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
       }
-      if (pc_kind == 5 || pc_kind == 6) {
+      if (use_net) {
         // descent check: <Pg, g> must be positive and finite for every band,
         // else the block falls back to the hprsdft smoother
         std::vector<double> rbh(static_cast<size_t>(nn));
@@ -1505,6 +1510,7 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
         for (double v : rbh) good = good && std::isfinite(v) && v > 0.0;
         if (!good) {
           ++ctx->lpc.fallbacks;
+          net_on = false;
           if ((rc = precond_hprsdft_fused<W>(ctx, mloop, gk, pgk, nn, pr, pp, pf, s, dv, s.rb))) return rc;
         }
       }
