@@ -272,6 +272,11 @@ def precision_si64_leg():
     nocc = int(round(cell.nelectron / 2))
     out = {"workload": "Si64 72^3, 160 bands, EKBT 0.01 Ha, IMIX 30, to dE < 1e-6 Ha/atom"}
     res = {}
+    # untimed: the first use of the kernels and library paths at this size
+    ctx = gpu.Context(0)
+    scf.SCF(cell, scf.ScfParams(ekbt=0.01, max_iter=2, mixed=True, fp64_below=1e9, imix=30, start="uniform"),
+            ctx=ctx).run()
+    ctx.close()
     for mixed in (False, True):
         ctx = gpu.Context(0)
         ctx.set_grid(cell.ngrid, cell.aa, cell.md)
@@ -407,21 +412,31 @@ def batch_leg(ncells: int = 64, workers: int = 8, seq_cells: int | None = None):
     # rocSOLVER, hipFFT code objects load on first use) before either timing
     firsts = list({c.cell.ngrid: c for c in cells}.values())
     batch.run_batch(firsts, workers=1, shapes=shapes)
-    progress(f"batch: {ncells} cells, {workers} workers, CG from captured graphs")
-    par = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=True)
-    progress(f"batch: {par.wall:.2f} s; the same without graphs")
+    progress(f"batch: {ncells} cells, {workers} workers, native SCF runs, CG from captured graphs")
+    par = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=True, native=True)
+    progress(f"batch: {par.wall:.2f} s; native without graphs")
+    par_native_direct = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=False, native=True)
+    progress(f"batch: {par_native_direct.wall:.2f} s; Python driver with graphs")
+    par_py = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=True)
+    progress(f"batch: {par_py.wall:.2f} s; Python driver without graphs")
     par_direct = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=False)
     progress(f"batch: {par_direct.wall:.2f} s; sequential plain solves of {seq_cells}")
     seq = batch.run_batch(cells[:seq_cells], workers=1, shapes=shapes, graphs=False)
     its = [r.iterations for r in par.results]
-    same = all(a.iterations == b.iterations and a.etot == b.etot for a, b in zip(par.results, par_direct.results))
+    same = all(a.iterations == b.iterations and a.etot == b.etot for a, b in zip(par_py.results, par_direct.results))
+    native_vs_python = max(max(abs(x - y) for x, y in zip(a.history, b.history)) / c.cell.natom
+                           for a, b, c in zip(par.results, par_direct.results, cells))
     return dict(workload=f"{ncells} Si8 cells, scale U(0.95,1.10), displacement N(0,0.1), grids {{24,32,40,48}}^3, "
                          f"U{{20..40}} bands, FP64 SCF to dE < 1e-6 Ha/atom",
                 gpu_max_hw_queues=os.environ.get("GPU_MAX_HW_QUEUES"),
                 workers=workers, batched_seconds=round(par.wall, 3), batched_cells_per_s=round(par.cells_per_s, 3),
-                batched_direct_launch_cells_per_s=round(par_direct.cells_per_s, 3),
+                batched_mode="native SCF runs (rsg_scf_run), CG from captured graphs",
+                native_direct_launch_cells_per_s=round(par_native_direct.cells_per_s, 3),
+                python_driver_graphs_cells_per_s=round(par_py.cells_per_s, 3),
+                python_driver_direct_launch_cells_per_s=round(par_direct.cells_per_s, 3),
                 graph_results_equal_direct=same, graph_launches=par.graph_launches,
-                graph_capture_failures=par.graph_failures,
+                graph_capture_failures=par.graph_failures + par_py.graph_failures,
+                native_vs_python_max_etot_diff_Ha_per_atom=native_vs_python,
                 sequential_cells=seq_cells, sequential_seconds=round(seq.wall, 3),
                 sequential_cells_per_s=round(seq.cells_per_s, 3),
                 speedup=round(par.cells_per_s / seq.cells_per_s, 3),

@@ -323,6 +323,45 @@ int rsg_mix_init(rsg_ctx *ctx, int mmix, double beta);
 int rsg_mix_init_imix(rsg_ctx *ctx, int mmix, double beta, int imix);
 int rsg_mix_pulay(rsg_ctx *ctx);
 
+/* ---- a whole SCF run in native code ----
+ * The SCF loop of calc_scf (scf_module.f90:183-481, Ndiag = 1) on the
+ * context's cell, with nothing crossing to the caller until it ends: start
+ * density (atomic Gaussians of init_width), uniform [0, 1) start bands from
+ * seed (random_initial_wf), Gram-Schmidt (main.f90:392), then per
+ * iteration subspace_diag, conjugate_gradient, gram_schmidt, esp_calc,
+ * Fermi-Dirac occupations at ekbt, the Harris and Kohn-Sham energies, and
+ * Pulay mixing (IMIX 10 or 30) with new Hartree/XC/Vloc, until
+ * |Etot_k - Etot_(k-1)| / natom < etol_per_atom between two FP64
+ * iterations or max_iter.  mixed = 1 runs FP32 CG/hpsi while that
+ * difference is at least fp64_below (0: twice etol_per_atom) and FP32
+ * keeps improving within fp32_stall iterations, then FP64.  The grid, the
+ * projectors (rsg_build_gth_projectors) and the local part
+ * (rsg_set_local_gth) must be set.  No reference counterpart as one call:
+ * it lets many cells run concurrently from host threads without returning
+ * to the caller per step (the Python and Fortran drivers make the same
+ * calls one by one). */
+typedef struct {
+  int nband;
+  int ncg, pc_kind, mloop; /* NCG, IPC (0/4/5/6), MLOOP */
+  int mmix, imix;          /* MMIX, IMIX (10 or 30) */
+  double beta, ekbt;       /* BETA, EKBT (Ha) */
+  double etol_per_atom;    /* stop test on |dEtot| / natom */
+  int max_iter;
+  double init_width;       /* bohr */
+  uint64_t seed;           /* start bands */
+  int mixed;
+  double fp64_below;
+  int fp32_stall;
+} rsg_scf_params;
+typedef struct {
+  int converged, iterations, fp32_iterations;
+  double etot, fene, efermi, eewald, wall_s;
+} rsg_scf_summary;
+/* etot_history (max_iter doubles) and esp (nband doubles, final band
+ * energies) may be NULL. */
+int rsg_scf_run(rsg_ctx *ctx, const rsg_scf_params *params, rsg_scf_summary *out, double *etot_history,
+                double *esp);
+
 /* ---- learned CG preconditioner (pc_kind = 5 in rsg_conjugate_gradient*).
  * No reference counterpart: it replaces the kinetic smoother
  * precond_hprsdft (cgpc_hprsdft_module.f90:133-361) with a 3-layer 3x3x3

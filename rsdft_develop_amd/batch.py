@@ -110,11 +110,13 @@ class ShapeCache:
 
 
 def run_batch(cells: list, workers: int = 8, params: scf.ScfParams | None = None, shapes: ShapeCache | None = None,
-              log=None, graphs: bool = True) -> BatchResult:
+              log=None, graphs: bool = True, native: bool = False) -> BatchResult:
     """SCF of every cell, `workers` at a time.  Results in input order.
     With `graphs` every context replays its CG band blocks from captured
     HIP graphs (rsg_set_cg_graphs), which removes the per-kernel launch cost
-    that dominates small cells."""
+    that dominates small cells.  With `native` each cell's whole SCF is one
+    library call (rsg_scf_run), so a worker holds the interpreter only
+    between cells; otherwise the Python driver (scf.SCF) runs it."""
     params = params or scf.ScfParams(ekbt=0.01, etol_per_atom=1e-6, max_iter=80, imix=30, start="uniform")
     shapes = shapes or ShapeCache()
     todo = queue.Queue()
@@ -145,12 +147,18 @@ def run_batch(cells: list, workers: int = 8, params: scf.ScfParams | None = None
                     ctx.build_gth_projectors(bc.cell)
                     shapes.apply(ctx, bc.cell, 64)
                     p = dataclasses.replace(params, seed=bc.seed)
-                    res = scf.SCF(bc.cell, p, ctx=ctx).run()
-                    results[i] = CellResult(i, bool(res.converged), res.niter, float(res.etot),
-                                            time.perf_counter() - t0, [float(r.etot) for r in res.iterations])
+                    if native:
+                        ctx.set_local_gth(bc.cell)
+                        r = ctx.scf_run(bc.cell.nband, p)
+                        results[i] = CellResult(i, r["converged"], r["iterations"], float(r["etot"]),
+                                                time.perf_counter() - t0, r["history"])
+                    else:
+                        res = scf.SCF(bc.cell, p, ctx=ctx).run()
+                        results[i] = CellResult(i, bool(res.converged), res.niter, float(res.etot),
+                                                time.perf_counter() - t0, [float(r.etot) for r in res.iterations])
                     if log:
-                        log(f"cell {i}: {bc.cell.ngrid[0]}^3, {bc.cell.nband} bands, {res.niter} iterations, "
-                            f"{time.perf_counter() - t0:.2f} s")
+                        log(f"cell {i}: {bc.cell.ngrid[0]}^3, {bc.cell.nband} bands, {results[i].iterations} "
+                            f"iterations, {time.perf_counter() - t0:.2f} s")
                 except Exception as e:  # noqa: BLE001 - reported to the caller after the pool drains
                     errors.append((i, e))
                     return

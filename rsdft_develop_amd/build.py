@@ -28,7 +28,8 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", INCLUDE, "-I", CSRC, "-Wall", "-Wno-unused-result",
 ]
-SOURCES = ["capi.cpp", "tables.cpp", "setup.hip", "hpsi_f64.hip", "hpsi_f32.hip", "solver.hip", "scf.hip", "lpc.hip"]
+SOURCES = ["capi.cpp", "tables.cpp", "setup.hip", "hpsi_f64.hip", "hpsi_f32.hip", "solver.hip", "scf.hip", "lpc.hip",
+           "scf_driver.cpp"]
 
 
 def _stale(obj: str, src: str) -> bool:

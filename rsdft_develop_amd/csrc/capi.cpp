@@ -522,6 +522,7 @@ int rsg_get_analytic_array(rsg_ctx *ctx, int which, void *out, int64_t *count) {
   const rsg::AnalyticDev<double> &d = ctx->ana64;
   const int ng = d.ngroup;
   int64_t E = 0, ntg = 0;
+  rsg::DeviceOpGuard guard;  // synchronous copies: kept apart from other threads' captures
   RSG_HIP(hipSetDevice(ctx->device));
   RSG_HIP(hipStreamSynchronize(ctx->stream));
   if (d.nseg) RSG_HIP(hipMemcpy(&E, static_cast<const int *>(d.seg_end.p) + d.nseg - 1, 4, hipMemcpyDeviceToHost));

@@ -596,6 +596,8 @@ int scf_set_local_gth(rsg_ctx *ctx, int natom, const double *frac, const int *ki
   return RSG_OK;
 }
 
+int scf_atom_count(const rsg_ctx *ctx) { return ctx->scf ? int(ctx->scf->kind.size()) : 0; }
+
 int scf_constants(rsg_ctx *ctx, double *eewald, double *nel) {
   if (!ctx->scf || !ctx->scf->have_local) return ctx->fail(RSG_ERR_STATE, "scf: local potential not set");
   if (eewald) *eewald = ctx->scf->eewald;

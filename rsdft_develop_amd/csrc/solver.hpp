@@ -28,6 +28,7 @@ int multidot(rsg_ctx *ctx, int npair, const float *const *x, const float *const 
 int scf_set_local_gth(rsg_ctx *ctx, int natom, const double *frac, const int *kind, int nkind, const double *zps,
                       const double *rloc, const double *cloc);
 int scf_constants(rsg_ctx *ctx, double *eewald, double *nel);
+int scf_atom_count(const rsg_ctx *ctx);
 int scf_field(rsg_ctx *ctx, int field, double *host, bool get);
 int scf_init_density(rsg_ctx *ctx, double width);
 int scf_normalize(rsg_ctx *ctx);

@@ -37,6 +37,22 @@ class RSDFTGPUError(RuntimeError):
 
 _lib = None
 _c_int, _c_i64, _c_dbl, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+
+class ScfRunParams(ctypes.Structure):
+    """rsg_scf_params of include/rsdft_gpu.h."""
+    _fields_ = [("nband", ctypes.c_int), ("ncg", ctypes.c_int), ("pc_kind", ctypes.c_int), ("mloop", ctypes.c_int),
+                ("mmix", ctypes.c_int), ("imix", ctypes.c_int), ("beta", ctypes.c_double), ("ekbt", ctypes.c_double),
+                ("etol_per_atom", ctypes.c_double), ("max_iter", ctypes.c_int), ("init_width", ctypes.c_double),
+                ("seed", ctypes.c_uint64), ("mixed", ctypes.c_int), ("fp64_below", ctypes.c_double),
+                ("fp32_stall", ctypes.c_int)]
+
+
+class ScfRunSummary(ctypes.Structure):
+    """rsg_scf_summary of include/rsdft_gpu.h."""
+    _fields_ = [("converged", ctypes.c_int), ("iterations", ctypes.c_int), ("fp32_iterations", ctypes.c_int),
+                ("etot", ctypes.c_double), ("fene", ctypes.c_double), ("efermi", ctypes.c_double),
+                ("eewald", ctypes.c_double), ("wall_s", ctypes.c_double)]
 _P = ctypes.POINTER
 
 
@@ -83,6 +99,7 @@ def load_library():
         "rsg_get_hpsi_order": (_c_int, [_vp, _c_int, _P(_c_int)]),
         "rsg_set_hpsi_proj": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_set_projector_setup": (_c_int, [_vp, _c_int]),
+        "rsg_scf_run": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
         "rsg_get_analytic_array": (_c_int, [_vp, _c_int, _vp, _P(_c_i64)]),
         "rsg_get_hpsi_proj": (_c_int, [_vp, _c_int, _P(_c_int)]),
         "rsg_set_hpsi_xpoints": (_c_int, [_vp, _c_int, _c_int]),
@@ -334,6 +351,26 @@ class Context:
     def set_hpsi_proj(self, precision=64, proj=0):
         """Analytic projections per sphere segment (0) or by row tiles (1)."""
         self._check(self.L.rsg_set_hpsi_proj(self.h, int(precision), int(proj)))
+
+    def scf_run(self, nband: int, p) -> dict:
+        """A whole SCF run in native code (rsg_scf_run) with the settings of
+        an scf.ScfParams (start bands drawn on the device); grid, projectors
+        and local part set before.  Returns the summary, the energy after
+        each iteration and the final band energies."""
+        if p.start != "uniform":
+            raise ValueError("the native SCF run draws its start bands on the device (start='uniform')")
+        prm = ScfRunParams(int(nband), int(p.ncg), int(p.pc_kind), int(p.mloop), int(p.mmix), int(p.imix),
+                           float(p.beta), float(p.ekbt), float(p.etol_per_atom), int(p.max_iter), float(p.init_width),
+                           int(p.seed) & 0xFFFFFFFFFFFFFFFF, int(bool(p.mixed)), float(p.fp64_below), int(p.fp32_stall))
+        out = ScfRunSummary()
+        hist = np.zeros(int(p.max_iter))
+        esp = np.zeros(int(nband))
+        self._check(self.L.rsg_scf_run(self.h, ctypes.addressof(prm), ctypes.addressof(out), _ptr(hist), _ptr(esp)))
+        self.mb = int(nband)
+        n = out.iterations
+        return dict(converged=bool(out.converged), iterations=n, fp32_iterations=out.fp32_iterations, etot=out.etot,
+                    fene=out.fene, efermi=out.efermi, eewald=out.eewald, wall_s=out.wall_s,
+                    history=hist[:n].tolist(), esp=esp)
 
     def set_projector_setup(self, on_host: bool = False):
         """Build the analytic projector description on the host instead of

@@ -60,3 +60,47 @@ def test_reused_context_forgets_the_previous_cell():
     fresh = run(b, fresh_ctx)
     fresh_ctx.close()
     assert reused == fresh
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_native_scf_run_equals_python_driver(mixed):
+    """rsg_scf_run makes the Python driver's calls in its order: the same
+    iterations and energies (occupations from the same bisection, summed in
+    another order) and the same final band energies."""
+    import numpy as np
+
+    from rsdft_develop_amd import gpu, scf, systems
+
+    cell = systems.silicon_supercell(1, nband=22, displacement=0.1, seed=3, scale=1.02, ngrid=32)
+    p = scf.ScfParams(ekbt=0.01, etol_per_atom=1e-6, max_iter=40, imix=30, start="uniform", seed=5, mixed=mixed)
+    print(f"case: Si8 32^3 native vs Python SCF, mixed={mixed}", flush=True)
+    ctx = gpu.Context()
+    ref = scf.SCF(cell, p, ctx=ctx).run()
+    ctx.close()
+    ctx = gpu.Context()
+    ctx.set_grid(cell.ngrid, cell.aa, cell.md)
+    ctx.build_gth_projectors(cell)
+    ctx.set_local_gth(cell)
+    got = ctx.scf_run(cell.nband, p)
+    ctx.close()
+    assert got["converged"] == ref.converged and got["iterations"] == ref.niter
+    assert got["fp32_iterations"] == sum(1 for r in ref.iterations if r.precision == 32)
+    hist = np.array([r.etot for r in ref.iterations])
+    assert np.abs(np.array(got["history"]) - hist).max() / cell.natom < 1e-10
+    assert abs(got["fene"] - ref.fene) / cell.natom < 1e-10
+    assert np.abs(np.sort(got["esp"]) - np.sort(ref.esp)).max() < 1e-9
+
+
+def test_native_batch_equals_python_batch():
+    from rsdft_develop_amd import batch, scf
+
+    cells = batch.config2_cells(4, seed=11)
+    print("case: 4 config-2 cells, native 3 workers vs Python 1 worker", flush=True)
+    params = scf.ScfParams(ekbt=0.01, etol_per_atom=1e-6, max_iter=40, imix=30, start="uniform")
+    shapes = batch.ShapeCache(reps=1)
+    nat = batch.run_batch(cells, workers=3, params=params, shapes=shapes, native=True)
+    py = batch.run_batch(cells, workers=1, params=params, shapes=shapes, graphs=False)
+    assert nat.graph_failures == 0
+    for a, b, c in zip(nat.results, py.results, cells):
+        assert (a.converged, a.iterations) == (b.converged, b.iterations)
+        assert max(abs(x - y) for x, y in zip(a.history, b.history)) / c.cell.natom < 1e-10
