@@ -66,7 +66,9 @@ def test_reused_context_forgets_the_previous_cell():
 def test_native_scf_run_equals_python_driver(mixed):
     """rsg_scf_run makes the Python driver's calls in its order: the same
     iterations and energies (occupations from the same bisection, summed in
-    another order) and the same final band energies."""
+    another order) and the same final band energies.  FP64 runs agree to
+    1e-10 Ha/atom; in FP32 iterations the last-bit differences of the
+    occupations move FP32 roundings, so mixed runs agree to FP32 noise."""
     import numpy as np
 
     from rsdft_develop_amd import gpu, scf, systems
@@ -86,9 +88,10 @@ def test_native_scf_run_equals_python_driver(mixed):
     assert got["converged"] == ref.converged and got["iterations"] == ref.niter
     assert got["fp32_iterations"] == sum(1 for r in ref.iterations if r.precision == 32)
     hist = np.array([r.etot for r in ref.iterations])
-    assert np.abs(np.array(got["history"]) - hist).max() / cell.natom < 1e-10
-    assert abs(got["fene"] - ref.fene) / cell.natom < 1e-10
-    assert np.abs(np.sort(got["esp"]) - np.sort(ref.esp)).max() < 1e-9
+    tol = 2e-9 if mixed else 1e-10
+    assert np.abs(np.array(got["history"]) - hist).max() / cell.natom < tol
+    assert abs(got["fene"] - ref.fene) / cell.natom < tol
+    assert np.abs(np.sort(got["esp"]) - np.sort(ref.esp)).max() < (1e-7 if mixed else 1e-9)
 
 
 def test_native_batch_equals_python_batch():
