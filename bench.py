@@ -412,11 +412,11 @@ def batch_leg(ncells: int = 64, workers: int = 8, seq_cells: int | None = None):
     # rocSOLVER, hipFFT code objects load on first use) before either timing
     firsts = list({c.cell.ngrid: c for c in cells}.values())
     batch.run_batch(firsts, workers=1, shapes=shapes)
-    progress(f"batch: {ncells} cells, {workers} workers, native SCF runs, CG from captured graphs")
-    par = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=True, native=True)
-    progress(f"batch: {par.wall:.2f} s; native without graphs")
-    par_native_direct = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=False, native=True)
-    progress(f"batch: {par_native_direct.wall:.2f} s; Python driver with graphs")
+    progress(f"batch: {ncells} cells, {workers} workers, native SCF runs")
+    par = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=False, native=True)
+    progress(f"batch: {par.wall:.2f} s; native with CG from captured graphs")
+    par_native_graphs = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=True, native=True)
+    progress(f"batch: {par_native_graphs.wall:.2f} s; Python driver with graphs")
     par_py = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=True)
     progress(f"batch: {par_py.wall:.2f} s; Python driver without graphs")
     par_direct = batch.run_batch(cells, workers=workers, shapes=shapes, graphs=False)
@@ -430,12 +430,12 @@ def batch_leg(ncells: int = 64, workers: int = 8, seq_cells: int | None = None):
                          f"U{{20..40}} bands, FP64 SCF to dE < 1e-6 Ha/atom",
                 gpu_max_hw_queues=os.environ.get("GPU_MAX_HW_QUEUES"),
                 workers=workers, batched_seconds=round(par.wall, 3), batched_cells_per_s=round(par.cells_per_s, 3),
-                batched_mode="native SCF runs (rsg_scf_run), CG from captured graphs",
-                native_direct_launch_cells_per_s=round(par_native_direct.cells_per_s, 3),
+                batched_mode="native SCF runs (rsg_scf_run), direct launches",
+                native_graphs_cells_per_s=round(par_native_graphs.cells_per_s, 3),
                 python_driver_graphs_cells_per_s=round(par_py.cells_per_s, 3),
                 python_driver_direct_launch_cells_per_s=round(par_direct.cells_per_s, 3),
-                graph_results_equal_direct=same, graph_launches=par.graph_launches,
-                graph_capture_failures=par.graph_failures + par_py.graph_failures,
+                graph_results_equal_direct=same, graph_launches=par_native_graphs.graph_launches,
+                graph_capture_failures=par_native_graphs.graph_failures + par_py.graph_failures,
                 native_vs_python_max_etot_diff_Ha_per_atom=native_vs_python,
                 sequential_cells=seq_cells, sequential_seconds=round(seq.wall, 3),
                 sequential_cells_per_s=round(seq.cells_per_s, 3),
