@@ -134,8 +134,10 @@ int rsg_hpsi_f64_host(rsg_ctx *ctx, const double *psi, double *hpsi,
 
 /* Launch shape of the hpsi kernels for the current grid, per precision
  * (64 or 32): rows per thread ry (1 or 2), tile rows ty, z-planes per
- * workgroup zlen, bands per Infinity-Cache chunk, scatter band tile bt
- * (4/8/16).  Zero fields are derived from the grid.  No reference
+ * workgroup zlen, bands per Infinity-Cache chunk, band tile bt of the
+ * nonlocal apply (4/8/16/20/32: 16, 32, 64, 80 or 128 bands per matrix-core
+ * wave of the analytic form; the table form's scatter takes 4/8/16, and 8
+ * for the others).  Zero fields are derived from the grid.  No reference
  * counterpart (launch tuning of this implementation). */
 int rsg_set_hpsi_shape(rsg_ctx *ctx, int precision, int ry, int ty, int zlen,
                        int chunk, int bt);

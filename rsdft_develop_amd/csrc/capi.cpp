@@ -471,8 +471,9 @@ int rsg_hpsi_f64_host(rsg_ctx *ctx, const double *psi, double *hpsi, int64_t ld,
 int rsg_set_hpsi_shape(rsg_ctx *ctx, int precision, int ry, int ty, int zlen, int chunk, int bt) {
   if (!ctx) return RSG_ERR_ARG;
   if (precision != 64 && precision != 32) return ctx->fail(RSG_ERR_ARG, "set_hpsi_shape: precision must be 64 or 32");
-  if (ry < 0 || ry > 2 || ty < 0 || zlen < 0 || chunk < 0 || (bt != 0 && bt != 4 && bt != 8 && bt != 16))
-    return ctx->fail(RSG_ERR_ARG, "set_hpsi_shape: ry in 0..2, ty/zlen/chunk >= 0, bt in {0,4,8,16}");
+  if (ry < 0 || ry > 2 || ty < 0 || zlen < 0 || chunk < 0 ||
+      (bt != 0 && bt != 4 && bt != 8 && bt != 16 && bt != 20 && bt != 32))
+    return ctx->fail(RSG_ERR_ARG, "set_hpsi_shape: ry in 0..2, ty/zlen/chunk >= 0, bt in {0,4,8,16,20,32}");
   rsg::HpsiShape &sh = precision == 64 ? ctx->shape64 : ctx->shape32;
   sh.ry = ry ? ry : 2;
   sh.ty = ty;

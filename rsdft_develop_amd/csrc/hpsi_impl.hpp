@@ -1057,7 +1057,7 @@ HpsiShape resolve_shape(const rsg_ctx *ctx, const HpsiShape &in, size_t s, int n
     sh.zlen = (n3 + zr - 1) / zr;
   }
   sh.zlen = std::max(1, std::min(sh.zlen, n3));
-  if (sh.bt != 4 && sh.bt != 8 && sh.bt != 16) sh.bt = 8;
+  if (sh.bt != 4 && sh.bt != 8 && sh.bt != 16 && sh.bt != 20 && sh.bt != 32) sh.bt = 8;
   if (sh.order != 0 && sh.order != 1) sh.order = 0;
   if (sh.proj != 0 && sh.proj != 1) sh.proj = 0;
   return sh;
@@ -1194,9 +1194,12 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const H
         int rc = launch_stencil<T>(ctx, pc, hc, ld, nbc, sh, 0);
         if (rc) return rc;
       }
+      // bands per apply wave: 16 NBT (bt 4, 8, 16, 20, 32 -> NBT 1, 2, 4, 5, 8)
       switch (sh.bt) {
         case 4: launch_apply_m<T, 1>(ctx, an, hc, ld, nbc, cp, nl_after); break;
         case 16: launch_apply_m<T, 4>(ctx, an, hc, ld, nbc, cp, nl_after); break;
+        case 20: launch_apply_m<T, 5>(ctx, an, hc, ld, nbc, cp, nl_after); break;
+        case 32: launch_apply_m<T, 8>(ctx, an, hc, ld, nbc, cp, nl_after); break;
         default: launch_apply_m<T, 2>(ctx, an, hc, ld, nbc, cp, nl_after); break;
       }
       if (!nl_after) {
@@ -1370,7 +1373,7 @@ template <typename T> int autotune_t(rsg_ctx *ctx, int nb, int reps, float *best
     ch.chunk = std::max(1, best.chunk / 2);
     ch.zlen = 0;
     consider(ch);
-    for (int bt : {4, 8, 16}) {
+    for (int bt : {4, 8, 16, 20, 32}) {
       HpsiShape c = best;
       c.bt = bt;
       consider(c);

@@ -247,6 +247,33 @@ def test_nonlocal_modes_agree_with_oracle(orc, gpu_mod, nrep, ng, disp, nb):
         d_out.free()
 
 
+@pytest.mark.parametrize("bt", [4, 8, 16, 20, 32])
+def test_apply_band_tiles_agree_with_oracle(orc, gpu_mod, bt):
+    """The analytic apply with 16, 32, 64, 80 and 128 bands per wave, writing
+    H psi (order 0) and accumulating onto the stencil's (order 1), on a band
+    count that leaves the last band tile ragged."""
+    from rsdft_develop_amd import hip
+
+    cell = systems.silicon_supercell(1, ngrid_per_cell=36, displacement=0.1, seed=bt)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    nb = 90
+    psi = systems.random_bands(cell.npoints, nb, seed=700 + bt)
+    want = ham.hpsi(psi)
+    for order in (0, 1):
+        for prec in (64, 32):
+            ctx.set_hpsi_shape(prec, bt=bt)
+            ctx.set_hpsi_order(prec, order)
+        print(f"case: apply band tile bt {bt} order {order}, Si8, {nb} bands", flush=True)
+        _close(ctx.hamiltonian(psi), want, 1e-12)
+        d_in = hip.DeviceArray.from_numpy(psi, dtype=np.float32)
+        d_out = hip.DeviceArray(psi.shape, dtype=np.float32)
+        ctx.hpsi_device(d_in.ptr, d_out.ptr, cell.npoints, nb, dtype=np.float32)
+        ctx.synchronize()
+        _close(d_out.to_numpy().astype(np.float64), want, 2e-5)
+        d_in.free()
+        d_out.free()
+
+
 def test_analytic_form_needs_a_cell_longer_than_two_cutoffs(orc, gpu_mod):
     # one Si atom in a 6.5 bohr cube: the p sphere (3.52 bohr) meets its own
     # periodic images, so only the tables (which fold images) apply
