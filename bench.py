@@ -167,11 +167,15 @@ def progress(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_kind: int = 4, weights=None):
+def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_kind: int = 4, weights=None,
+            reject_limit: int = 0):
     """Wall seconds of a whole SCF run to dE < etol Ha/atom (the run's own
     set-up on the device included: tables, initial density and GS).  Pulay
     mixing in G space with Kerker weights (IMIX 30), EKBT 0.01 Ha.  With
-    pc_kind 5 or 6 the learned preconditioner's weights come from `weights`.
+    pc_kind 5 or 6 the learned preconditioner's weights come from `weights`,
+    and after `reject_limit` CG calls in a row whose network output failed
+    the descent check the run keeps hprsdft (0: the network is tried in
+    every call).
     After the timed run, untimed, the band energies are converged at the
     run's final potential (FP64 eigensolver passes, SCF.refine_levels) and
     returned beside the run's own as `esp_levels`."""
@@ -193,7 +197,7 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     fb0 = ctx.lpc_fallbacks()
     t0 = time.perf_counter()
     run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=etol, max_iter=max_iter, mixed=mixed, imix=30,
-                                      pc_kind=pc_kind, start="uniform"), ctx=ctx)
+                                      pc_kind=pc_kind, start="uniform", lpc_reject_limit=reject_limit), ctx=ctx)
     progress(f"{tag}: set-up {run.setup_s:.2f} s")
     res = run.run(callback=lambda r: progress(f"{tag}: iter {r.iter} prec {r.precision} dE/atom {r.de_per_atom:.2e}"))
     wall = time.perf_counter() - t0
@@ -206,6 +210,8 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
                esp=np.sort(res.esp).tolist())
     if pc_kind in (5, 6):
         out["descent_check_fallbacks"] = ctx.lpc_fallbacks() - fb0
+        out["reject_limit"] = reject_limit
+        out["cg_calls_without_network"] = ctx.lpc_reject_state()[1]
     if len(its) > 1:  # how far the band energies still move per iteration at the stop
         nocc = int(round(cell.nelectron / 2))
         out["last_iteration_eig_change_occupied_Ha"] = float(
@@ -258,6 +264,9 @@ def scf_config0_leg():
             "etot_diff_Ha_per_atom": abs(float(g.etot) - float(c["etot"])) / cell.natom}
 
 
+CONTINUE_ITERS = 40
+
+
 def precision_si64_leg():
     """Configuration 1's precision report: the Si64 cell (72^3, 160 bands,
     displacements N(0, 0.05 bohr)) solved to dE < 1e-6 Ha/atom in plain FP64
@@ -265,7 +274,10 @@ def precision_si64_leg():
     eigensolve, density and energies), IMIX 30, EKBT 0.01 Ha.  Energy and
     band-energy differences of the mixed run against the FP64 one, as the
     runs leave them and after 12 fixed-potential passes, with the FP64
-    run's own last-iteration level change for scale."""
+    run's own last-iteration level change for scale.  Then, untimed, both
+    runs continue for CONTINUE_ITERS FP64 iterations and are compared again
+    (`continued`): the stop test bounds the energy change only, and the
+    levels of either run still move by about 1e-5 Ha after it."""
     from rsdft_develop_amd import gpu, scf, systems
 
     cell = systems.silicon_supercell(2, nband=NBANDS[2], displacement=0.05, seed=0)
@@ -290,15 +302,27 @@ def precision_si64_leg():
         sec = time.perf_counter() - t0
         levels, _ = run.refine_levels(tol=1e-9, max_passes=12)
         its = r.iterations
-        res[mixed] = dict(esp=np.sort(r.esp), levels=np.sort(levels), etot=r.etot)
+        # untimed: the same run continued by CONTINUE_ITERS FP64 iterations,
+        # towards the SCF fixed point both runs approach
+        run.p.mixed, run.p.etol_per_atom, run.p.max_iter = False, 0.0, CONTINUE_ITERS
+        rc = run.run()
+        res[mixed] = dict(esp=np.sort(r.esp), levels=np.sort(levels), etot=r.etot, esp_c=np.sort(rc.esp),
+                          etot_c=rc.etot)
         key = "mixed" if mixed else "fp64"
         out[key] = dict(seconds=round(sec, 3), iterations=r.niter, converged=bool(r.converged),
                         fp32_iterations=sum(1 for x in its if x.precision == 32))
         if len(its) > 1:
             out[key]["last_iteration_eig_change_occupied_Ha"] = float(
                 np.abs(np.sort(its[-1].esp)[:nocc] - np.sort(its[-2].esp)[:nocc]).max())
+        out[key]["continued_last_dE_per_atom"] = rc.iterations[-1].de_per_atom
         ctx.close()
     f, m = res[False], res[True]
+    out["continued"] = dict(
+        fp64_iterations_after_stop=CONTINUE_ITERS,
+        max_eig_diff_vs_fp64_occupied_Ha=float(np.abs(m["esp_c"] - f["esp_c"])[:nocc].max()),
+        max_eig_diff_vs_fp64_Ha=float(np.abs(m["esp_c"] - f["esp_c"]).max()),
+        etot_diff_vs_fp64_Ha_per_atom=abs(m["etot_c"] - f["etot_c"]) / cell.natom,
+        fp64_own_eig_change_from_stop_occupied_Ha=float(np.abs(f["esp_c"] - f["esp"])[:nocc].max()))
     out["mixed"].update(
         etot_diff_vs_fp64_Ha_per_atom=abs(m["etot"] - f["etot"]) / cell.natom,
         max_eig_diff_vs_fp64_occupied_Ha=float(np.abs(m["esp"] - f["esp"])[:nocc].max()),
@@ -482,6 +506,9 @@ def main() -> int:
     ap.add_argument("--batch-workers", type=int, default=8, help="concurrent contexts of the batched-SCF leg")
     ap.add_argument("--lpc", type=int, default=1, help="learned-preconditioner leg on the held-out Si64 cell (0 skips)")
     ap.add_argument("--precision64", type=int, default=1, help="Si64 FP64-vs-mixed SCF precision leg (0 skips)")
+    ap.add_argument("--lpc-reject-limit", type=int, default=2,
+                    help="Si216 learned/hybrid legs: CG calls in a row with a rejected network output after which "
+                         "the run keeps hprsdft (0: try the network in every call)")
     ap.add_argument("--cpu-si216-steps", type=int, default=0,
                     help="SCF steps of the oracle at Si216 on the host cores (0 skips; about a minute per step)")
     ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx]")
@@ -528,8 +555,10 @@ def main() -> int:
         plain = scf_leg(args.scf, mixed=False)
         runs = {"fp64": plain,
                 "mixed": scf_leg(args.scf, mixed=True),
-                "mixed_learned": scf_leg(args.scf, mixed=True, pc_kind=5, weights=LPC_WEIGHTS),
-                "mixed_hybrid": scf_leg(args.scf, mixed=True, pc_kind=6, weights=LPC_HYBRID)}
+                "mixed_learned": scf_leg(args.scf, mixed=True, pc_kind=5, weights=LPC_WEIGHTS,
+                                         reject_limit=args.lpc_reject_limit),
+                "mixed_hybrid": scf_leg(args.scf, mixed=True, pc_kind=6, weights=LPC_HYBRID,
+                                        reject_limit=args.lpc_reject_limit)}
         e_ref = np.array(plain["esp"])
         l_ref = np.array(plain["esp_levels"])
         for name, r in runs.items():

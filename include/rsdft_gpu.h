@@ -403,6 +403,16 @@ int rsg_lpc_reset_optimizer(rsg_ctx *ctx);
 int rsg_band_residuals_f32(rsg_ctx *ctx, float *d_g);
 /* CG band blocks that fell back to precond_hprsdft since creation */
 int rsg_lpc_fallbacks(rsg_ctx *ctx, int64_t *count);
+/* Rejection limit of the learned preconditioner (no reference
+ * counterpart; the reference has no learned preconditioner): once `limit`
+ * CG calls in a row had a band block whose network output failed the
+ * descent check, the following pc_kind 5/6 calls run precond_hprsdft
+ * without trying the network, until the limit is set again (which also
+ * clears the streak).  0 (the default) tries the network in every call. */
+int rsg_lpc_set_reject_limit(rsg_ctx *ctx, int limit);
+/* the current streak of rejected calls and the calls run without the
+ * network because of the limit, since creation */
+int rsg_lpc_reject_state(rsg_ctx *ctx, int *streak, int64_t *skipped_calls);
 /* pc_kind 6: the factor on the hprsdft output the network corrects
  * (default 1). */
 int rsg_lpc_set_hybrid_scale(rsg_ctx *ctx, double scale);

@@ -875,6 +875,21 @@ int rsg_lpc_fallbacks(rsg_ctx *ctx, int64_t *count) {
   return RSG_OK;
 }
 
+int rsg_lpc_set_reject_limit(rsg_ctx *ctx, int limit) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (limit < 0) return ctx->fail(RSG_ERR_ARG, "lpc_set_reject_limit: limit < 0");
+  ctx->lpc.reject_limit = limit;
+  ctx->lpc.reject_streak = 0;
+  return RSG_OK;
+}
+
+int rsg_lpc_reject_state(rsg_ctx *ctx, int *streak, int64_t *skipped_calls) {
+  if (!ctx || !streak || !skipped_calls) return RSG_ERR_ARG;
+  *streak = ctx->lpc.reject_streak;
+  *skipped_calls = ctx->lpc.skipped_calls;
+  return RSG_OK;
+}
+
 int rsg_lpc_set_hybrid_scale(rsg_ctx *ctx, double scale) {
   if (!ctx) return RSG_ERR_ARG;
   if (!std::isfinite(scale)) return ctx->fail(RSG_ERR_ARG, "lpc_set_hybrid_scale: scale not finite");

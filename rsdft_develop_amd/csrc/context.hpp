@@ -174,6 +174,11 @@ struct LpcState {
   int64_t vN = 0;
   int adam_t = 0;
   int64_t fallbacks = 0;  // CG blocks that fell back to hprsdft
+  // CG calls in a row whose network output failed the descent check; at
+  // reject_limit (> 0) the following calls run hprsdft without trying the
+  // network (counted in skipped_calls) until the limit is set again
+  int reject_streak = 0, reject_limit = 0;
+  int64_t skipped_calls = 0;
   double hybrid_scale = 1.0;  // pc_kind 6: P g = hybrid_scale * P_hprsdft g + network(g)
   LpcState() = default;
   LpcState(const LpcState &) = delete;

@@ -1418,6 +1418,14 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
   if (rc) return rc;
   if (ncg < 0 || (pc_kind != 0 && pc_kind != 4 && pc_kind != 5 && pc_kind != 6))
     return ctx->fail(RSG_ERR_ARG, "conjugate_gradient: ncg < 0 or pc_kind not 0/4/5/6");
+  const bool learned = pc_kind == 5 || pc_kind == 6;
+  if (learned && ctx->lpc.reject_limit > 0 && ctx->lpc.reject_streak >= ctx->lpc.reject_limit) {
+    // the network's output was rejected in the last reject_limit calls:
+    // this call runs the hprsdft smoother without trying it
+    ++ctx->lpc.skipped_calls;
+    pc_kind = 4;
+  }
+  const int64_t fallbacks0 = ctx->lpc.fallbacks;
   constexpr bool kMixed = sizeof(W) == 4;
   const int mb = ctx->mb;
   const int64_t N = ctx->N;
@@ -1555,6 +1563,7 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
     RSG_HIP(hipMemcpyAsync(res + ns, s.res, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
     RSG_HIP(hipStreamSynchronize(ctx->stream));
   }
+  if (pc_kind == 5 || pc_kind == 6) ctx->lpc.reject_streak = ctx->lpc.fallbacks > fallbacks0 ? ctx->lpc.reject_streak + 1 : 0;
   return RSG_OK;
 }
 

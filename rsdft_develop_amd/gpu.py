@@ -146,6 +146,8 @@ def load_library():
         "rsg_lpc_reset_optimizer": (_c_int, [_vp]),
         "rsg_band_residuals_f32": (_c_int, [_vp, _vp]),
         "rsg_lpc_fallbacks": (_c_int, [_vp, _P(_c_i64)]),
+        "rsg_lpc_set_reject_limit": (_c_int, [_vp, _c_int]),
+        "rsg_lpc_reject_state": (_c_int, [_vp, _P(_c_int), _P(_c_i64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -584,6 +586,18 @@ class Context:
         c = _c_i64()
         self._check(self.L.rsg_lpc_fallbacks(self.h, ctypes.byref(c)))
         return c.value
+
+    def lpc_set_reject_limit(self, limit: int):
+        """After `limit` CG calls in a row with a rejected network output,
+        pc_kind 5/6 calls run hprsdft without trying the network (0: try it
+        in every call).  Setting the limit clears the streak."""
+        self._check(self.L.rsg_lpc_set_reject_limit(self.h, int(limit)))
+
+    def lpc_reject_state(self) -> tuple:
+        """(current streak of rejected calls, calls run without the network)."""
+        st, sk = _c_int(), _c_i64()
+        self._check(self.L.rsg_lpc_reject_state(self.h, ctypes.byref(st), ctypes.byref(sk)))
+        return st.value, sk.value
 
     def mix_pulay(self):
         self._check(self.L.rsg_mix_pulay(self.h))

@@ -221,6 +221,49 @@ def test_hybrid_with_a_silent_network_is_hprsdft():
     assert ctx.lpc_fallbacks() == 0
 
 
+def test_reject_limit_skips_a_rejected_network():
+    """A silent network (P g = 0) fails the descent check in every block.
+    With a rejection limit of 1 the first CG call tries it and falls back,
+    the next call runs hprsdft without trying it, and setting the limit
+    again clears the streak; every call's result is pc_kind 4's."""
+    from rsdft_develop_amd import gpu
+
+    cell = systems.silicon_supercell(1, ngrid_per_cell=24, nband=12, displacement=0.05, seed=4)
+    v = systems.model_vloc(cell)
+    ctx = gpu.Context().setup(cell, v)
+    d = lpc.unpack(lpc.init_params(0))
+    d["W3"] = np.zeros_like(d["W3"])
+    d["b3"] = np.zeros_like(d["b3"])
+    ctx.lpc_set_params(lpc.pack(d))
+    ctx.lpc_set_potential(v)
+    ctx.alloc_bands(cell.nband)
+    u0 = systems.random_bands(cell.npoints, cell.nband, seed=1)
+
+    def cg(pc):
+        ctx.upload_bands(u0)
+        ctx.gram_schmidt()
+        ctx.subspace_diag()
+        return ctx.conjugate_gradient(ncg=4, pc_kind=pc, mloop=3)[0], ctx.download_bands()
+
+    ref = cg(4)
+    ctx.lpc_set_reject_limit(1)
+    seen = []
+    for expect_fb, expect_skip in ((1, 0), (1, 1), (1, 2)):
+        out = cg(5)
+        seen.append((ctx.lpc_fallbacks(), ctx.lpc_reject_state()))
+        np.testing.assert_allclose(out[0], ref[0], rtol=0, atol=1e-10)
+        assert np.abs(out[1] - ref[1]).max() < 1e-8 * np.abs(ref[1]).max()
+        assert ctx.lpc_fallbacks() == expect_fb and ctx.lpc_reject_state() == (1, expect_skip), seen
+    ctx.lpc_set_reject_limit(1)  # a new streak: the network is tried again
+    assert ctx.lpc_reject_state() == (0, 2)
+    cg(5)
+    assert ctx.lpc_fallbacks() == 2 and ctx.lpc_reject_state() == (1, 2)
+    ctx.lpc_set_reject_limit(0)  # no limit: every call tries it
+    for k in range(2):
+        cg(5)
+    assert ctx.lpc_fallbacks() == 4 and ctx.lpc_reject_state()[1] == 2
+
+
 def test_trained_network_gain_over_no_preconditioner():
     """The shipped weights on a held-out Si64 cell (72^3, 160 bands): CG
     rounds to the residual tolerance hprsdft reaches in 8 rounds.  The

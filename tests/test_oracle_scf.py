@@ -147,6 +147,15 @@ def test_si8_scf_converges(si8_scf):
     esp = np.sort(out["esp"])
     assert esp[1] - esp[0] > 0.05
     np.testing.assert_allclose(esp[1:7], esp[1], atol=1e-4)
+    # the 16th and 17th levels (the highest occupied at zero temperature and
+    # the one above it) lie in one six-fold shell, levels 15-20, which holds
+    # the last 4 of the 32 electrons: the cell is metallic at Gamma, each
+    # member partly filled (about 0.68 at EKBT 0.01 Ha)
+    occ = out["occ"][np.argsort(out["esp"])]
+    np.testing.assert_allclose(esp[14:20], esp[15], atol=1e-5)
+    assert esp[14] - esp[13] > 0.05 and esp.size == 20
+    assert np.all((occ[14:20] > 0.5) & (occ[14:20] < 0.9))
+    assert occ[14:20].sum() == pytest.approx(32.0 - occ[:14].sum(), abs=1e-9)
 
 
 def test_kerker_weights_closed_form():
