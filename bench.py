@@ -168,14 +168,15 @@ def progress(msg: str):
 
 
 def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_kind: int = 4, weights=None,
-            reject_limit: int = 0):
+            reject_limit: int = 0, precond_bits: int = 32):
     """Wall seconds of a whole SCF run to dE < etol Ha/atom (the run's own
     set-up on the device included: tables, initial density and GS).  Pulay
     mixing in G space with Kerker weights (IMIX 30), EKBT 0.01 Ha.  With
     pc_kind 5 or 6 the learned preconditioner's weights come from `weights`,
     and after `reject_limit` CG calls in a row whose network output failed
     the descent check the run keeps hprsdft (0: the network is tried in
-    every call).
+    every call).  precond_bits 16 keeps the internal vectors of the FP32
+    hprsdft calls in bfloat16.
     After the timed run, untimed, the band energies are converged at the
     run's final potential (FP64 eigensolver passes, SCF.refine_levels) and
     returned beside the run's own as `esp_levels`."""
@@ -197,13 +198,15 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     fb0 = ctx.lpc_fallbacks()
     t0 = time.perf_counter()
     run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=etol, max_iter=max_iter, mixed=mixed, imix=30,
-                                      pc_kind=pc_kind, start="uniform", lpc_reject_limit=reject_limit), ctx=ctx)
+                                      pc_kind=pc_kind, start="uniform", lpc_reject_limit=reject_limit,
+                                      precond_bits=precond_bits), ctx=ctx)
     progress(f"{tag}: set-up {run.setup_s:.2f} s")
     res = run.run(callback=lambda r: progress(f"{tag}: iter {r.iter} prec {r.precision} dE/atom {r.de_per_atom:.2e}"))
     wall = time.perf_counter() - t0
     its = res.iterations
     out = dict(seconds=round(wall, 3), setup_s=round(res.setup, 3), iterations=len(its), converged=bool(res.converged),
-               pc_kind=pc_kind, gram_chunk=gram[1], gram_ms=round(gram[0], 3),
+               pc_kind=pc_kind, fp32_phase_precond_storage_bits=precond_bits if mixed else None,
+               gram_chunk=gram[1], gram_ms=round(gram[0], 3),
                etot=res.etot,
                fp32_iterations=sum(1 for r in its if r.precision == 32),
                per_phase_s={k: round(sum(r.times.get(k, 0.0) for r in its), 3) for k in PHASES},
@@ -506,6 +509,8 @@ def main() -> int:
     ap.add_argument("--batch-workers", type=int, default=8, help="concurrent contexts of the batched-SCF leg")
     ap.add_argument("--lpc", type=int, default=1, help="learned-preconditioner leg on the held-out Si64 cell (0 skips)")
     ap.add_argument("--precision64", type=int, default=1, help="Si64 FP64-vs-mixed SCF precision leg (0 skips)")
+    ap.add_argument("--mixed-precond-bits", type=int, default=16, choices=(16, 32),
+                    help="Si216 mixed legs: storage of the FP32 hprsdft calls' internal vectors (16 = bfloat16)")
     ap.add_argument("--lpc-reject-limit", type=int, default=2,
                     help="Si216 learned/hybrid legs: CG calls in a row with a rejected network output after which "
                          "the run keeps hprsdft (0: try the network in every call)")
@@ -554,11 +559,11 @@ def main() -> int:
         # energies and from the levels converged at each run's final potential
         plain = scf_leg(args.scf, mixed=False)
         runs = {"fp64": plain,
-                "mixed": scf_leg(args.scf, mixed=True),
+                "mixed": scf_leg(args.scf, mixed=True, precond_bits=args.mixed_precond_bits),
                 "mixed_learned": scf_leg(args.scf, mixed=True, pc_kind=5, weights=LPC_WEIGHTS,
-                                         reject_limit=args.lpc_reject_limit),
+                                         reject_limit=args.lpc_reject_limit, precond_bits=args.mixed_precond_bits),
                 "mixed_hybrid": scf_leg(args.scf, mixed=True, pc_kind=6, weights=LPC_HYBRID,
-                                        reject_limit=args.lpc_reject_limit)}
+                                        reject_limit=args.lpc_reject_limit, precond_bits=args.mixed_precond_bits)}
         e_ref = np.array(plain["esp"])
         l_ref = np.array(plain["esp_levels"])
         for name, r in runs.items():

@@ -260,6 +260,15 @@ int rsg_precond_hprsdft_f64(rsg_ctx *ctx, int mloop, const double *g, double *pg
                             int64_t ld, int nb);
 int rsg_precond_hprsdft_f32(rsg_ctx *ctx, int mloop, const float *g, float *pg,
                             int64_t ld, int nb);
+/* Storage of the hprsdft preconditioner's internal vectors (its residual,
+ * direction and next direction) in FP32 calls: 32 (default) or 16
+ * (bfloat16, half their bytes; g, P g, the stencil sums and every update
+ * stay FP32, the dot products FP64).  A precision choice of this build with
+ * no reference counterpart: precond_hprsdft (cgpc_hprsdft_module.f90:
+ * 133-270) is FP64.  Applies to rsg_precond_hprsdft_f32 and to
+ * rsg_conjugate_gradient_f32 when every band column allows 2- or 4-wide
+ * vectors (else the FP32 kernels run). */
+int rsg_set_precond_storage(rsg_ctx *ctx, int bits);
 /* The same band update with every CG and preconditioner vector, hpsi
  * included, in FP32 storage with FP32 arithmetic; dot products accumulate
  * in FP64, the 2x2 Ritz step is FP64 and the accepted update lands in the

@@ -695,6 +695,13 @@ int rsg_set_cg_swgs(rsg_ctx *ctx, int swgs) {
   return RSG_OK;
 }
 
+int rsg_set_precond_storage(rsg_ctx *ctx, int bits) {
+  if (!ctx) return RSG_ERR_ARG;
+  if (bits != 16 && bits != 32) return ctx->fail(RSG_ERR_ARG, "set_precond_storage: bits must be 16 or 32");
+  ctx->pc_bf16 = bits == 16;
+  return RSG_OK;
+}
+
 int rsg_autotune_gram(rsg_ctx *ctx, int nb, int reps, float *best_ms, int64_t *chunk) {
   if (!ctx) return RSG_ERR_ARG;
   RSG_HIP(hipSetDevice(ctx->device));

@@ -268,6 +268,7 @@ struct rsg_ctx {
   rsg::DevBuf cgcoef;              // cggs coefficients dV U^T Pg (mb x block)
   int cg_swgs = 0;                 // SWGS keyword: 0 none, 1/2 cggs projections in CG
   bool cg_graphs = false;          // replay CG band blocks from captured HIP graphs
+  bool pc_bf16 = false;            // FP32 hprsdft calls keep r, p, the new direction in bfloat16
   int64_t cg_graph_launches = 0;   // band blocks run from a captured graph
   int64_t cg_graph_failures = 0;   // captures the runtime refused
   std::vector<rsg::CgGraph> cg_graph_cache;

@@ -454,13 +454,36 @@ __device__ inline void block_partial(double v, double *__restrict__ partial, int
 
 // Pointers of one pass: the stencil input is g (first), p (pap, steps) or
 // r + beta p (dir); x accumulates P g (it is g before the first step).
-template <typename W> struct PzArgs {
+// S is the storage type of the internal vectors r, p and the new direction
+// (W, or bfloat16 bits for W = float); g and x are always W.
+template <typename W, typename S = W> struct PzArgs {
   const W *g;
-  W *x, *r;
-  const W *p;
-  W *pw;  // first: p = r; dir: the new direction
+  W *x;
+  S *r;
+  const S *p;
+  S *pw;  // first: p = r; dir: the new direction
   const double *a, *rr1, *rr0;
   double *partial;
+};
+
+template <typename T> struct TypeTag {
+  typedef T type;
+};
+
+// Storage conversions of the internal preconditioner vectors: identity for
+// S = W; for bfloat16 bits (uint16_t) with W = float, the upper half of the
+// float on load and round-to-nearest-even on store.
+template <typename W, typename S> struct PzStore {
+  __device__ static W ld(S v) { return W(v); }
+  __device__ static S st(W v) { return S(v); }
+};
+template <> struct PzStore<float, uint16_t> {
+  __device__ static float ld(uint16_t v) { return __uint_as_float(uint32_t(v) << 16); }
+  __device__ static uint16_t st(float f) {
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0x7fffffu) ? 0x40u : 0u));
+    return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+  }
 };
 
 // Raw values of one plane the thread needs: the four in-plane neighbours
@@ -590,9 +613,11 @@ template <typename W, int VP> struct alignas(VP * sizeof(W)) PzVec {
   W v[VP];
 };
 
-template <typename W, int OP, int VP>
-__global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W> A, PzGeom q, W c0, W c1, W c2, W c3) {
+template <typename W, int OP, int VP, typename S = W>
+__global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W, S> A, PzGeom q, W c0, W c1, W c2, W c3) {
   typedef PzVec<W, VP> V;
+  typedef PzVec<S, VP> VS;
+  typedef PzStore<W, S> St;
   constexpr bool kStencil = OP != kPzStepL && OP != kPzStepFL;
   constexpr bool kStep = OP == kPzStepF || OP == kPzStepM;
   const unsigned nblk = gridDim.x, xc = blockIdx.x % 8;
@@ -625,52 +650,84 @@ __global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W> A, PzGeom q, W c0, 
   const int yu_src = yu_in ? yu_l : lane, yd_src = yd_in ? yd_l : lane;
   const int z0 = zr * q.zlen, z1 = min(z0 + q.zlen, q.n3);
   const int64_t off = int64_t(b) * q.N;
-  const W *gb = A.g + int64_t(b) * q.ld, *pb = A.p + off;
-  W *xb = A.x + int64_t(b) * q.ld, *rb = A.r + off;
+  const W *gb = A.g + int64_t(b) * q.ld;
+  const S *pb = A.p + off;
+  W *xb = A.x + int64_t(b) * q.ld;
+  S *rb = A.r + off;
   const W al = (kStep || OP == kPzStepL || OP == kPzStepFL) ? W(A.a[b]) : W(0);
   const W beta = OP == kPzDir ? W(pc_beta(A.rr1[b], A.rr0[b])) : W(0);
-  const W *src = OP == kPzFirst ? gb : (OP == kPzDir ? rb : pb);
-  auto ldv = [](const W *base, int64_t e) { return *reinterpret_cast<const V *>(base + e); };
-  auto stv = [](W *base, int64_t e, const W *w) {
+  // W-typed vector and scalar loads of a W or S stream
+  auto ldw = [](const W *base, int64_t e) { return *reinterpret_cast<const V *>(base + e); };
+  auto lds = [](const S *base, int64_t e) {
+    const VS t = *reinterpret_cast<const VS *>(base + e);
+    V v;
+#pragma unroll
+    for (int i = 0; i < VP; ++i) v.v[i] = St::ld(t.v[i]);
+    return v;
+  };
+  // stencil input: g (first), r (direction, with p beside it) or p
+  auto ld_src = [&](int64_t e) {
+    if constexpr (OP == kPzFirst)
+      return ldw(gb, e);
+    else
+      return lds(OP == kPzDir ? rb : pb, e);
+  };
+  auto src_at = [&](int64_t e) -> W {
+    if constexpr (OP == kPzFirst)
+      return gb[e];
+    else
+      return St::ld((OP == kPzDir ? rb : pb)[e]);
+  };
+  auto stw = [](W *base, int64_t e, const W *w) {
     V v;
 #pragma unroll
     for (int i = 0; i < VP; ++i) v.v[i] = w[i];
     *reinterpret_cast<V *>(base + e) = v;
+  };
+  // stores w rounded to S, and leaves the rounded values in w
+  auto sts = [](S *base, int64_t e, W *w) {
+    VS v;
+#pragma unroll
+    for (int i = 0; i < VP; ++i) {
+      v.v[i] = St::st(w[i]);
+      w[i] = St::ld(v.v[i]);
+    }
+    *reinterpret_cast<VS *>(base + e) = v;
   };
   auto comb = [&](W r, W m) -> W { return OP == kPzDir ? std::fma(beta, m, r) : r; };
   auto wrapz = [&](int z) { return ((z % q.n3) + q.n3) % q.n3; };
   double acc = 0.0;
   if constexpr (kStencil) {
     // ring of the group at z-1, z, z+1 (raw input; p alongside for the direction pass)
-    V cm = ldv(src, int64_t(wrapz(z0 - 1)) * q.n12 + e0), cz = ldv(src, int64_t(z0) * q.n12 + e0);
-    V cp = ldv(src, int64_t(wrapz(z0 + 1)) * q.n12 + e0);
+    V cm = ld_src(int64_t(wrapz(z0 - 1)) * q.n12 + e0), cz = ld_src(int64_t(z0) * q.n12 + e0);
+    V cp = ld_src(int64_t(wrapz(z0 + 1)) * q.n12 + e0);
     V dm{}, dz{}, dp{};
     if (OP == kPzDir) {
-      dm = ldv(pb, int64_t(wrapz(z0 - 1)) * q.n12 + e0);
-      dz = ldv(pb, int64_t(z0) * q.n12 + e0);
-      dp = ldv(pb, int64_t(wrapz(z0 + 1)) * q.n12 + e0);
+      dm = lds(pb, int64_t(wrapz(z0 - 1)) * q.n12 + e0);
+      dz = lds(pb, int64_t(z0) * q.n12 + e0);
+      dp = lds(pb, int64_t(wrapz(z0 + 1)) * q.n12 + e0);
     }
     int zn = wrapz(z0 + 1);
     for (int z = z0; z < z1; ++z) {
       const int64_t o = int64_t(z) * q.n12;
       zn = zn + 1 == q.n3 ? 0 : zn + 1;  // plane z + 2
-      const V cn = ldv(src, int64_t(zn) * q.n12 + e0);
+      const V cn = ld_src(int64_t(zn) * q.n12 + e0);
       V dn{};
-      if (OP == kPzDir) dn = ldv(pb, int64_t(zn) * q.n12 + e0);
+      if (OP == kPzDir) dn = lds(pb, int64_t(zn) * q.n12 + e0);
       W v[VP], wu[VP], wd[VP], av[VP];
 #pragma unroll
       for (int i = 0; i < VP; ++i) v[i] = comb(cz.v[i], dz.v[i]);
       auto ldc = [&](int64_t e, W *w) {  // combined input of the group at e
-        const V a = ldv(src, e);
+        const V a = ld_src(e);
         V m{};
-        if (OP == kPzDir) m = ldv(pb, e);
+        if (OP == kPzDir) m = lds(pb, e);
 #pragma unroll
         for (int i = 0; i < VP; ++i) w[i] = comb(a.v[i], m.v[i]);
       };
       // every lane of the wave is at plane z here (z0, z1 are block-uniform)
       W sl = __shfl(v[VP - 1], sl_src, 64), sr = __shfl(v[0], sr_src, 64);
-      if (!sl_in) sl = comb(src[o + exl], OP == kPzDir ? pb[o + exl] : W(0));
-      if (!sr_in) sr = comb(src[o + exr], OP == kPzDir ? pb[o + exr] : W(0));
+      if (!sl_in) sl = comb(src_at(o + exl), OP == kPzDir ? St::ld(pb[o + exl]) : W(0));
+      if (!sr_in) sr = comb(src_at(o + exr), OP == kPzDir ? St::ld(pb[o + exr]) : W(0));
       if (ysh) {  // uniform over the launch
 #pragma unroll
         for (int i = 0; i < VP; ++i) {
@@ -694,30 +751,30 @@ __global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W> A, PzGeom q, W c0, 
         if (OP == kPzFirst) {  // r = g - A g ; p = r
           W rv[VP];
 #pragma unroll
-          for (int i = 0; i < VP; ++i) {
-            rv[i] = v[i] - av[i];
-            acc += double(rv[i]) * double(rv[i]);
-          }
-          stv(rb, k, rv);
-          stv(A.pw + off, k, rv);
+          for (int i = 0; i < VP; ++i) rv[i] = v[i] - av[i];
+          sts(rb, k, rv);
+          sts(A.pw + off, k, rv);
+#pragma unroll
+          for (int i = 0; i < VP; ++i) acc += double(rv[i]) * double(rv[i]);
         } else if (OP == kPzPap) {
 #pragma unroll
           for (int i = 0; i < VP; ++i) acc += double(v[i]) * double(av[i]);
         } else if (OP == kPzDir) {  // p' = r + beta p (written apart)
-          stv(A.pw + off, k, v);
+          sts(A.pw + off, k, v);
 #pragma unroll
           for (int i = 0; i < VP; ++i) acc += double(v[i]) * double(av[i]);
         } else {  // steps before the last: x += a p ; r -= a A p
-          const V u = ldv(OP == kPzStepF ? gb : xb, k), rr = ldv(rb, k);
+          const V u = ldw(OP == kPzStepF ? gb : xb, k), rr = lds(rb, k);
           W xv[VP], rv[VP];
 #pragma unroll
           for (int i = 0; i < VP; ++i) {
             xv[i] = u.v[i] + al * v[i];
             rv[i] = rr.v[i] - al * av[i];
-            acc += double(rv[i]) * double(rv[i]);
           }
-          stv(xb, k, xv);
-          stv(rb, k, rv);
+          stw(xb, k, xv);
+          sts(rb, k, rv);
+#pragma unroll
+          for (int i = 0; i < VP; ++i) acc += double(rv[i]) * double(rv[i]);
         }
       }
       cm = cz;
@@ -731,15 +788,15 @@ __global__ __launch_bounds__(kThreads) void k_pczv(PzArgs<W> A, PzGeom q, W c0, 
     for (int z = z0; z < z1; ++z) {
       const int64_t k = int64_t(z) * q.n12 + e0;
       if (act) {
-        const V gv = ldv(gb, k), pv = ldv(pb, k);
-        const V xo = OP == kPzStepFL ? gv : ldv(xb, k);
+        const V gv = ldw(gb, k), pv = lds(pb, k);
+        const V xo = OP == kPzStepFL ? gv : ldw(xb, k);
         W xv[VP];
 #pragma unroll
         for (int i = 0; i < VP; ++i) {
           xv[i] = xo.v[i] + al * pv.v[i];
           acc += double(xv[i]) * double(gv.v[i]);
         }
-        stv(xb, k, xv);
+        stw(xb, k, xv);
       }
     }
   }
@@ -988,58 +1045,72 @@ int precond_hprsdft_fused(rsg_ctx *ctx, int mloop, const W *g, W *pg, int nn, W 
   if (int64_t(q.nseg) * nn > 0x7fffffff) return ctx->fail(RSG_ERR_UNSUPPORTED, "preconditioner: too many blocks");
   if (ctx->dotpart.ensure(size_t(nn) * q.nseg * sizeof(double)) != hipSuccess)
     return ctx->fail(RSG_ERR_HIP, "preconditioner: scratch allocation failed");
-  PzArgs<W> A;
-  A.g = g;
-  A.x = pg;
-  A.r = r;
-  A.p = p;
-  A.pw = p;
-  A.a = s.a;
-  A.rr1 = s.rr1;
-  A.rr0 = s.rr0;
-  A.partial = ctx->dotpart.as<double>();
-  const dim3 grid(unsigned(int64_t(q.nseg) * nn)), blk(kThreads), gb((nn + 63) / 64), b64(64);
-  const dim3 gr((nn + 255) / 256);
-  auto reduce = [&](double alpha, double *out) {
-    hipLaunchKernelGGL(k_reduce_partials, gr, dim3(256), 0, ctx->stream, A.partial, q.nseg, nn, alpha, out);
-  };
-  auto pass = [&](auto opc) {
-    constexpr int OP = decltype(opc)::value;
-    if (vp == 4)
-      hipLaunchKernelGGL((k_pczv<W, OP, 4>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
-    else if (vp == 2)
-      hipLaunchKernelGGL((k_pczv<W, OP, 2>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
-    else
-      hipLaunchKernelGGL((k_pcz<W, OP>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
-  };
-  pass(std::integral_constant<int, kPzFirst>{});
-  reduce(1.0, s.rr1);
-  // the early exit all(rr1 < 1e-24) decided on the device: every step then
-  // takes a zero step, leaving P g = g, with no read-back to the host
-  hipLaunchKernelGGL(k_all_below, dim3(1), dim3(256), 0, ctx->stream, s.rr1, nn, 1e-24, s.flags + 1);
-  W *pcur = p, *pnext = f;
-  for (int it = 1; it <= mloop; ++it) {
-    A.p = pcur;
-    if (it == 1) {
-      pass(std::integral_constant<int, kPzPap>{});
-    } else {
-      A.pw = pnext;
-      pass(std::integral_constant<int, kPzDir>{});
-      std::swap(pcur, pnext);
+  // S: storage of the internal vectors r, p and the new direction; with
+  // ctx->pc_bf16 an FP32 call keeps them in bfloat16 (half their bytes; g,
+  // P g and every product stay FP32, the dot products FP64)
+  auto run = [&](auto stag) {
+    using S = typename decltype(stag)::type;
+    PzArgs<W, S> A;
+    A.g = g;
+    A.x = pg;
+    A.r = reinterpret_cast<S *>(r);
+    A.p = reinterpret_cast<S *>(p);
+    A.pw = reinterpret_cast<S *>(p);
+    A.a = s.a;
+    A.rr1 = s.rr1;
+    A.rr0 = s.rr0;
+    A.partial = ctx->dotpart.as<double>();
+    const dim3 grid(unsigned(int64_t(q.nseg) * nn)), blk(kThreads), gb((nn + 63) / 64), b64(64);
+    const dim3 gr((nn + 255) / 256);
+    auto reduce = [&](double alpha, double *out) {
+      hipLaunchKernelGGL(k_reduce_partials, gr, dim3(256), 0, ctx->stream, A.partial, q.nseg, nn, alpha, out);
+    };
+    auto pass = [&](auto opc) {
+      constexpr int OP = decltype(opc)::value;
+      if (vp == 4)
+        hipLaunchKernelGGL((k_pczv<W, OP, 4, S>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
+      else if (vp == 2)
+        hipLaunchKernelGGL((k_pczv<W, OP, 2, S>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
+      else if constexpr (std::is_same<S, W>::value)
+        hipLaunchKernelGGL((k_pcz<W, OP>), grid, blk, 0, ctx->stream, A, q, C0, C1, C2, C3);
+    };
+    pass(std::integral_constant<int, kPzFirst>{});
+    reduce(1.0, s.rr1);
+    // the early exit all(rr1 < 1e-24) decided on the device: every step then
+    // takes a zero step, leaving P g = g, with no read-back to the host
+    hipLaunchKernelGGL(k_all_below, dim3(1), dim3(256), 0, ctx->stream, s.rr1, nn, 1e-24, s.flags + 1);
+    S *pcur = reinterpret_cast<S *>(p), *pnext = reinterpret_cast<S *>(f);
+    for (int it = 1; it <= mloop; ++it) {
       A.p = pcur;
+      if (it == 1) {
+        pass(std::integral_constant<int, kPzPap>{});
+      } else {
+        A.pw = pnext;
+        pass(std::integral_constant<int, kPzDir>{});
+        std::swap(pcur, pnext);
+        A.p = pcur;
+      }
+      reduce(1.0, s.pap);
+      hipLaunchKernelGGL(k_pc_alpha, gb, b64, 0, ctx->stream, s.rr1, s.pap, s.rr0, s.a, nn, s.flags + 1);
+      const bool first = it == 1, last = it == mloop;
+      if (first && last)
+        pass(std::integral_constant<int, kPzStepFL>{});
+      else if (first)
+        pass(std::integral_constant<int, kPzStepF>{});
+      else if (last)
+        pass(std::integral_constant<int, kPzStepL>{});
+      else
+        pass(std::integral_constant<int, kPzStepM>{});
+      reduce(last ? dv : 1.0, last ? rb : s.rr1);
     }
-    reduce(1.0, s.pap);
-    hipLaunchKernelGGL(k_pc_alpha, gb, b64, 0, ctx->stream, s.rr1, s.pap, s.rr0, s.a, nn, s.flags + 1);
-    const bool first = it == 1, last = it == mloop;
-    if (first && last)
-      pass(std::integral_constant<int, kPzStepFL>{});
-    else if (first)
-      pass(std::integral_constant<int, kPzStepF>{});
-    else if (last)
-      pass(std::integral_constant<int, kPzStepL>{});
+  };
+  if constexpr (std::is_same<W, float>::value) {
+    if (ctx->pc_bf16 && vp >= 2)
+      run(TypeTag<uint16_t>{});
     else
-      pass(std::integral_constant<int, kPzStepM>{});
-    reduce(last ? dv : 1.0, last ? rb : s.rr1);
+      run(TypeTag<W>{});
+  } else {
+    run(TypeTag<W>{});
   }
   RSG_HIP(hipGetLastError());
   return RSG_OK;
@@ -1308,7 +1379,7 @@ std::vector<unsigned char> cg_graph_key(const rsg_ctx *ctx, int wsize, int ns, i
                                         int mloop) {
   std::vector<unsigned char> k;
   for (int v : {wsize, ns, nn, mbd, ncg, pc_kind, mloop, ctx->n[0], ctx->n[1], ctx->n[2], ctx->md, ctx->nl_mode,
-                int(ctx->have_ana), int(ctx->have_nl), ctx->tables.nzlma})
+                int(ctx->have_ana), int(ctx->have_nl), ctx->tables.nzlma, int(ctx->pc_bf16)})
     key_put(k, v);
   key_put(k, ctx->N);
   for (const DevBuf *b : {&ctx->unk, &ctx->cgbuf, &ctx->scal, &ctx->dotpart, &ctx->cproj, &ctx->uvpart, &ctx->uvunk,

@@ -147,6 +147,7 @@ def load_library():
         "rsg_band_residuals_f32": (_c_int, [_vp, _vp]),
         "rsg_lpc_fallbacks": (_c_int, [_vp, _P(_c_i64)]),
         "rsg_lpc_set_reject_limit": (_c_int, [_vp, _c_int]),
+        "rsg_set_precond_storage": (_c_int, [_vp, _c_int]),
         "rsg_lpc_reject_state": (_c_int, [_vp, _P(_c_int), _P(_c_i64)]),
     }
     for name, (res, args) in sig.items():
@@ -586,6 +587,11 @@ class Context:
         c = _c_i64()
         self._check(self.L.rsg_lpc_fallbacks(self.h, ctypes.byref(c)))
         return c.value
+
+    def set_precond_storage(self, bits: int):
+        """Internal vectors of FP32 hprsdft calls in bfloat16 (16) or FP32
+        (32, the default)."""
+        self._check(self.L.rsg_set_precond_storage(self.h, int(bits)))
 
     def lpc_set_reject_limit(self, limit: int):
         """After `limit` CG calls in a row with a rejected network output,

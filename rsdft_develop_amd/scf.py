@@ -37,6 +37,7 @@ class ScfParams:
                             # needs dE/atom < etol between two FP64 iterations
     fp64_below: float = 0.0  # 0: twice etol_per_atom (FP32 energies follow FP64 to well below 1e-6 Ha/atom)
     fp32_stall: int = 3      # ... or after this many FP32 iterations without a new lowest dE/atom
+    precond_bits: int = 32  # storage of the FP32 hprsdft calls' internal vectors: 32, or 16 (bfloat16)
     lpc_reject_limit: int = 0  # pc_kind 5/6: after this many CG calls in a row with the network's output
                                # rejected, run hprsdft without trying it (0: try it in every call)
 
@@ -146,6 +147,7 @@ class SCF:
             c.lpc_set_potential()  # the network's potential input follows Vloc
             c.lpc_set_reject_limit(self.p.lpc_reject_limit)  # also starts a new streak
         lap("density_potential")
+        c.set_precond_storage(self.p.precond_bits)
         c.alloc_bands(cell.nband)
         if bands is None and self.p.start == "uniform":
             c.random_bands(self.p.seed)
