@@ -206,6 +206,19 @@ int rsg_upload_bands(rsg_ctx *ctx, const double *u, int64_t ld, int n0,
                      int nb);
 int rsg_download_bands(rsg_ctx *ctx, double *u, int64_t ld, int n0, int nb);
 int rsg_bands_device_ptr(rsg_ctx *ctx, double **d_u, int64_t *ld);
+/* Storage of the band store for the mixed-precision phase (no reference
+ * counterpart; unk is real(8) there): 32 moves the bands to an FP32 store,
+ * on which rsg_subspace_diag (FP32 hpsi and products summed in FP64, the
+ * FP64 eigensolve, an FP32 rotation) and rsg_conjugate_gradient_f32 (in
+ * place) then work.  rsg_gram_schmidt orthonormalises in FP64 through the
+ * FP64 store, which then holds the same bands in FP64: rsg_esp_calc and
+ * rsg_calc_density read it until the next change to the FP32 store (before
+ * that, FP32 hpsi with FP64 sums).  64 moves the bands back (no copy when
+ * the FP64 store is current).  Upload, download, the device pointer, random bands, the FP64
+ * CG and SWGS 1/2 return RSG_ERR_STATE / RSG_ERR_UNSUPPORTED meanwhile.
+ * rsg_alloc_bands starts a store in FP64. */
+int rsg_set_band_precision(rsg_ctx *ctx, int bits);
+int rsg_get_band_precision(rsg_ctx *ctx, int *bits);
 
 /* Orthonormalise bands n0..n1 (1-based, inclusive) of the store under
  * <u,v> = dV u^T v.  Replaces gram_schmidt(n0,n1,k,s)
@@ -363,6 +376,8 @@ typedef struct {
   int mixed;
   double fp64_below;
   int fp32_stall;
+  int fp32_store;          /* mixed: FP32 iterations on the FP32 band store */
+  int precond_bits;        /* FP32 hprsdft storage: 32, or 16 (bfloat16) */
 } rsg_scf_params;
 typedef struct {
   int converged, iterations, fp32_iterations;

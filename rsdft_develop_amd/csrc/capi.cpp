@@ -605,12 +605,26 @@ int rsg_alloc_bands(rsg_ctx *ctx, int mb) {
   RSG_HIP(hipMemsetAsync(ctx->unk.p, 0, bytes, ctx->stream));
   RSG_HIP(hipStreamSynchronize(ctx->stream));
   ctx->mb = mb;
+  ctx->band32 = false;  // a new store starts in FP64
+  return RSG_OK;
+}
+
+int rsg_set_band_precision(rsg_ctx *ctx, int bits) {
+  if (!ctx) return RSG_ERR_ARG;
+  RSG_HIP(hipSetDevice(ctx->device));
+  return rsg::set_band_precision(ctx, bits);
+}
+
+int rsg_get_band_precision(rsg_ctx *ctx, int *bits) {
+  if (!ctx || !bits) return RSG_ERR_ARG;
+  *bits = ctx->band32 ? 32 : 64;
   return RSG_OK;
 }
 
 int rsg_upload_bands(rsg_ctx *ctx, const double *u, int64_t ld, int n0, int nb) {
   if (!ctx || !u) return RSG_ERR_ARG;
   if (ctx->mb == 0) return ctx->fail(RSG_ERR_STATE, "upload_bands: no band store");
+  if (ctx->band32) return ctx->fail(RSG_ERR_STATE, "upload_bands: the bands are in the FP32 store");
   if (n0 < 0 || nb < 0 || n0 + nb > ctx->mb || ld < ctx->N)
     return ctx->fail(RSG_ERR_ARG, "upload_bands: band range or ld out of bounds");
   RSG_HIP(hipSetDevice(ctx->device));
@@ -629,6 +643,7 @@ int rsg_random_bands(rsg_ctx *ctx, int n0, int nb, uint64_t seed) {
 int rsg_download_bands(rsg_ctx *ctx, double *u, int64_t ld, int n0, int nb) {
   if (!ctx || !u) return RSG_ERR_ARG;
   if (ctx->mb == 0) return ctx->fail(RSG_ERR_STATE, "download_bands: no band store");
+  if (ctx->band32) return ctx->fail(RSG_ERR_STATE, "download_bands: the bands are in the FP32 store");
   if (n0 < 0 || nb < 0 || n0 + nb > ctx->mb || ld < ctx->N)
     return ctx->fail(RSG_ERR_ARG, "download_bands: band range or ld out of bounds");
   RSG_HIP(hipSetDevice(ctx->device));
@@ -641,6 +656,7 @@ int rsg_download_bands(rsg_ctx *ctx, double *u, int64_t ld, int n0, int nb) {
 int rsg_bands_device_ptr(rsg_ctx *ctx, double **d_u, int64_t *ld) {
   if (!ctx || !d_u) return RSG_ERR_ARG;
   if (ctx->mb == 0) return ctx->fail(RSG_ERR_STATE, "bands_device_ptr: no band store");
+  if (ctx->band32) return ctx->fail(RSG_ERR_STATE, "bands_device_ptr: the bands are in the FP32 store");
   *d_u = ctx->unk.as<double>();
   if (ld) *ld = ctx->N;
   return RSG_OK;

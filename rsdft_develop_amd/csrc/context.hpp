@@ -258,6 +258,14 @@ struct rsg_ctx {
   // wavefunction store and work arrays
   int mb = 0;
   rsg::DevBuf unk, work;           // N x mb each (work: rotation target)
+  // FP32 band store of the mixed-precision phase (rsg_set_band_precision):
+  // while band32 is set the bands live in unk32 (work32: rotation target
+  // and H psi), and unk is stale until the store returns to FP64
+  rsg::DevBuf unk32, work32, hsub32, grampart32;
+  bool band32 = false;
+  // with band32: unk holds the same bands in FP64 (Gram-Schmidt just wrote
+  // them there), so esp_calc and the density read unk
+  bool unk_fresh = false;
   rsg::DevBuf hsub, esp_dev, info; // mb x mb dense, mb, int
   rsg::DevBuf cgbuf;               // CG work vectors
   rsg::DevBuf scal;                // small per-band scalars

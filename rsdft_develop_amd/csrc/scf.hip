@@ -140,8 +140,10 @@ __global__ void k_hartree_g(GridG g, double2 *__restrict__ z) {
   z[j] = make_double2(z[j].x * f, z[j].y * f);
 }
 
-// rho = sum_n occ_n psi_n^2 (density_module.f90:120-151)
-__global__ void k_density(const double *__restrict__ u, int64_t N, int mb, const double *__restrict__ occ,
+// rho = sum_n occ_n psi_n^2 (density_module.f90:120-151), accumulated in
+// FP64 also from the FP32 band store
+template <typename U>
+__global__ void k_density(const U *__restrict__ u, int64_t N, int mb, const double *__restrict__ occ,
                           double *__restrict__ rho) {
   const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= N) return;
@@ -149,7 +151,7 @@ __global__ void k_density(const double *__restrict__ u, int64_t N, int mb, const
   for (int n = 0; n < mb; ++n) {
     const double o = occ[n];
     if (o != 0.0) {
-      const double v = u[j + int64_t(n) * N];
+      const double v = double(u[j + int64_t(n) * N]);
       s += o * v * v;
     }
   }
@@ -675,8 +677,12 @@ int scf_calc_density(rsg_ctx *ctx, const double *occ) {
     return ctx->fail(RSG_ERR_HIP, "calc_density: allocation failed");
   double *docc = st->occ.as<double>() + 64;  // slots 0..63 are dot scratch
   RSG_HIP(hipMemcpyAsync(docc, occ, size_t(ctx->mb) * 8, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_density, dim3(nblk(ctx->N)), dim3(kT), 0, ctx->stream, ctx->unk.as<double>(), ctx->N,
-                     ctx->mb, docc, st->rho_out.as<double>());
+  if (ctx->band32 && !ctx->unk_fresh)
+    hipLaunchKernelGGL(k_density<float>, dim3(nblk(ctx->N)), dim3(kT), 0, ctx->stream, ctx->unk32.as<float>(),
+                       ctx->N, ctx->mb, docc, st->rho_out.as<double>());
+  else
+    hipLaunchKernelGGL(k_density<double>, dim3(nblk(ctx->N)), dim3(kT), 0, ctx->stream, ctx->unk.as<double>(),
+                       ctx->N, ctx->mb, docc, st->rho_out.as<double>());
   RSG_HIP(hipStreamSynchronize(ctx->stream));
   return RSG_OK;
 }

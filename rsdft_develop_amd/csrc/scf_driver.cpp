@@ -82,6 +82,8 @@ extern "C" int rsg_scf_run(rsg_ctx *ctx, const rsg_scf_params *p, rsg_scf_summar
   if ((rc = rsg_random_bands(ctx, 0, nb, p->seed))) return rc;
   if ((rc = rsg_gram_schmidt(ctx, 1, nb))) return rc;
   if ((rc = rsg_mix_init_imix(ctx, p->mmix, p->beta, p->imix))) return rc;
+  if ((rc = rsg_set_precond_storage(ctx, p->precond_bits == 16 ? 16 : 32))) return rc;
+  const bool store32 = p->mixed && p->fp32_store;
   std::vector<double> esp(static_cast<size_t>(nb)), res(static_cast<size_t>(nb)), occ;
   int prec = p->mixed ? 32 : 64, prev_prec = 0, stall = 0, fp32_its = 0;
   const double switch_below = p->fp64_below > 0.0 ? p->fp64_below : 2.0 * p->etol_per_atom;
@@ -90,6 +92,8 @@ extern "C" int rsg_scf_run(rsg_ctx *ctx, const rsg_scf_params *p, rsg_scf_summar
   bool have_prev = false, conv = false;
   int it = 0;
   for (it = 1; it <= p->max_iter; ++it) {
+    // FP32 iterations on the FP32 band store, FP64 ones on the FP64 store
+    if (store32 && (rc = rsg_set_band_precision(ctx, prec))) return rc;
     // eigensolver half (scf_module.f90:231-268)
     if ((rc = rsg_subspace_diag(ctx, esp.data()))) return rc;
     rc = prec == 32 ? rsg_conjugate_gradient_f32(ctx, p->ncg, p->pc_kind, p->mloop, esp.data(), res.data())
@@ -136,6 +140,7 @@ extern "C" int rsg_scf_run(rsg_ctx *ctx, const rsg_scf_params *p, rsg_scf_summar
     have_prev = true;
     if (conv) break;
   }
+  if (store32 && (rc = rsg_set_band_precision(ctx, 64))) return rc;
   out->converged = conv ? 1 : 0;
   out->iterations = std::min(it, p->max_iter);
   out->fp32_iterations = fp32_its;

@@ -256,6 +256,18 @@ __global__ void k_cg_accept(double *__restrict__ x, W *__restrict__ x32, const W
   if (x32) x32[k] = W(v);
 }
 
+// The same update on bands held in the FP32 store only
+template <typename W>
+__global__ void k_cg_accept32(W *__restrict__ x, const W *__restrict__ p, const double *__restrict__ uc,
+                              const int *__restrict__ accept, int64_t N) {
+  const int b = blockIdx.y;
+  if (!accept[b]) return;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t k = int64_t(b) * N + i;
+  x[k] = W(uc[2 * b] * double(x[k]) + uc[2 * b + 1] * double(p[k]));
+}
+
 // y = a y + b x (n values)
 template <typename W>
 __global__ void k_axpby(W *__restrict__ y, const W *__restrict__ x, W a, W b, int64_t n) {
@@ -1172,6 +1184,44 @@ int gemm_tn_gram(rsg_ctx *ctx, int nb, int64_t N, double alpha, const double *A,
   RSG_HIP(hipGetLastError());
   return RSG_OK;
 }
+// out[t] = alpha * sum_c part[c * count + t], FP32 partials summed in FP64
+__global__ void k_sum_batches_f32(const float *__restrict__ part, int nbat, int count, double alpha,
+                                  double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  double v = 0.0;
+  for (int c = 0; c < nbat; ++c) v += double(part[int64_t(c) * count + t]);
+  out[t] = alpha * v;
+}
+
+// C (nb x nb, FP64) = alpha A^T B for FP32 bands: SGEMMs over K chunks of at
+// most 16384 points (FP32 sums stay short) into FP32 partial matrices, then
+// the chunk sum in FP64.
+int gemm_tn_gram_f32(rsg_ctx *ctx, int nb, int64_t N, double alpha, const float *A, const float *B, double *C) {
+  const float zero = 0.f, one = 1.f;
+  int64_t kc = gram_chunk(ctx, nb, N);
+  if (kc <= 0 || kc > 16384) kc = std::min<int64_t>(N, 16384);
+  const int nfull = int(N / kc);
+  const int64_t krem = N - int64_t(nfull) * kc;
+  const int nbat = nfull + (krem > 0 ? 1 : 0);
+  const size_t mat = size_t(nb) * nb;
+  if (ctx->grampart32.ensure(mat * nbat * sizeof(float)) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "gram product: scratch allocation failed");
+  float *part = ctx->grampart32.as<float>();
+  if (nfull > 0)
+    RSG_BLAS_GUARDED(rocblas_sgemm_strided_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb,
+                                                   nb, int(kc), &one, A, int(N), kc, B, int(N), kc, &zero, part, nb,
+                                                   rocblas_stride(mat), nfull));
+  if (krem > 0)
+    RSG_BLAS_GUARDED(rocblas_sgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, int(krem),
+                                   &one, A + int64_t(nfull) * kc, int(N), B + int64_t(nfull) * kc, int(N), &zero,
+                                   part + mat * nfull, nb));
+  const int count = int(mat);
+  hipLaunchKernelGGL(k_sum_batches_f32, dim3((count + 255) / 256), dim3(256), 0, ctx->stream, part, nbat, count, alpha,
+                     C);
+  RSG_HIP(hipGetLastError());
+  return RSG_OK;
+}
 }  // namespace
 
 // Time one GEMM over all of K against K-splits into 8..512 chunks of at
@@ -1229,10 +1279,9 @@ int autotune_gram(rsg_ctx *ctx, int nb, int reps, float *best_ms, int64_t *best_
   return RSG_OK;
 }
 
-int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
-  int rc = check_store(ctx, "gram_schmidt");
-  if (rc) return rc;
-  if (n0 < 1 || n1 > ctx->mb || n0 > n1) return ctx->fail(RSG_ERR_ARG, "gram_schmidt: band range outside 1..mb");
+namespace {
+int gram_schmidt64(rsg_ctx *ctx, int n0, int n1) {
+  int rc = RSG_OK;
   const int nb = n1 - n0 + 1;
   const int64_t N = ctx->N;
   double *U = ctx->unk.as<double>() + size_t(n0 - 1) * N;
@@ -1256,8 +1305,64 @@ int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
   RSG_BLAS_GUARDED(rocblas_dtrsm(ctx->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                          rocblas_diagonal_non_unit, int(N), nb, &one, S, nb, U, int(N)));
   RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return rc;
+}
+}  // namespace
+
+// With the bands in the FP32 store, the orthonormalisation still runs in
+// FP64 (the overlap and its Cholesky factor): the range goes to the FP64
+// store, through gram_schmidt64, and back.
+int gram_schmidt(rsg_ctx *ctx, int n0, int n1) {
+  int rc = check_store(ctx, "gram_schmidt");
+  if (rc) return rc;
+  if (n0 < 1 || n1 > ctx->mb || n0 > n1) return ctx->fail(RSG_ERR_ARG, "gram_schmidt: band range outside 1..mb");
+  if (!ctx->band32) return gram_schmidt64(ctx, n0, n1);
+  const int64_t tot = ctx->N * int64_t(n1 - n0 + 1), o = ctx->N * int64_t(n0 - 1);
+  const dim3 g(unsigned((tot + kThreads - 1) / kThreads)), b(kThreads);
+  hipLaunchKernelGGL((k_to_double<float>), g, b, 0, ctx->stream, ctx->unk32.as<float>() + o, ctx->unk.as<double>() + o,
+                     tot);
+  if ((rc = gram_schmidt64(ctx, n0, n1))) return rc;
+  hipLaunchKernelGGL((k_to_work<float>), g, b, 0, ctx->stream, ctx->unk.as<double>() + o, ctx->unk32.as<float>() + o,
+                     tot);
+  RSG_HIP(hipGetLastError());
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->unk_fresh = n0 == 1 && n1 == ctx->mb ? true : ctx->unk_fresh;
   return RSG_OK;
 }
+
+namespace {
+// subspace_diag on the FP32 store: FP32 hpsi, the subspace matrix from FP32
+// products summed in FP64, the FP64 eigensolve, and the rotation as an
+// FP32 GEMM with the eigenvectors rounded to FP32.
+int subspace_diag32(rsg_ctx *ctx, double *esp, double *Hs, double *D, double *Ework) {
+  const int mb = ctx->mb;
+  const int64_t N = ctx->N;
+  if (ctx->hsub32.ensure(size_t(mb) * mb * 4) != hipSuccess)
+    return ctx->fail(RSG_ERR_HIP, "subspace_diag: allocation failed");
+  float *U = ctx->unk32.as<float>(), *HU = ctx->work32.as<float>(), *V = ctx->hsub32.as<float>();
+  int rc = launch_hpsi<float>(ctx, U, HU, N, mb);
+  if (rc) return rc;
+  if ((rc = gemm_tn_gram_f32(ctx, mb, N, 1.0, U, HU, Hs))) return rc;
+  hipLaunchKernelGGL(k_symmetrize, dim3((mb + 15) / 16, (mb + 15) / 16), dim3(16, 16), 0, ctx->stream, Hs, mb,
+                     0.5 * ctx->dv);
+  RSG_BLAS_GUARDED(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, mb, Hs, mb, D, Ework,
+                                    ctx->info.as<rocblas_int>()));
+  int info = 0;
+  if ((rc = read_info(ctx, &info))) return rc;
+  if (info != 0) return ctx->fail(RSG_ERR_SOLVER, "subspace_diag: dsyevd did not converge");
+  const int64_t mm = int64_t(mb) * mb;
+  hipLaunchKernelGGL((k_to_work<float>), dim3(unsigned((mm + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     ctx->stream, Hs, V, mm);
+  const float zero = 0.f, one = 1.f;
+  RSG_BLAS_GUARDED(rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, int(N), mb, mb, &one, U,
+                                 int(N), V, mb, &zero, HU, int(N)));
+  std::swap(ctx->unk32, ctx->work32);
+  ctx->unk_fresh = false;
+  RSG_HIP(hipMemcpyAsync(esp, D, mb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  return RSG_OK;
+}
+}  // namespace
 
 int subspace_diag(rsg_ctx *ctx, double *esp) {
   int rc = check_store(ctx, "subspace_diag");
@@ -1269,6 +1374,7 @@ int subspace_diag(rsg_ctx *ctx, double *esp) {
     return ctx->fail(RSG_ERR_HIP, "subspace_diag: allocation failed");
   double *U = ctx->unk.as<double>(), *HU = ctx->work.as<double>(), *Hs = ctx->hsub.as<double>();
   double *D = ctx->esp_dev.as<double>(), *Ework = D + mb;
+  if (ctx->band32) return subspace_diag32(ctx, esp, Hs, D, Ework);
   // psit = H U for all bands in one band-blocked call (subspace_diag_la_module.f90:79)
   rc = launch_hpsi<double>(ctx, U, HU, N, mb);
   if (rc) return rc;
@@ -1302,12 +1408,19 @@ int esp_calc(rsg_ctx *ctx, double *esp) {
   const int mb = ctx->mb;
   const int64_t N = ctx->N;
   if (ctx->esp_dev.ensure(size_t(mb) * 16) != hipSuccess) return ctx->fail(RSG_ERR_HIP, "esp_calc: allocation failed");
-  double *U = ctx->unk.as<double>(), *HU = ctx->work.as<double>();
-  rc = launch_hpsi<double>(ctx, U, HU, N, mb);
-  if (rc) return rc;
-  const double *xs[1] = {U}, *ys[1] = {HU};
-  rc = multidot(ctx, 1, xs, ys, mb, ctx->dv, ctx->esp_dev.as<double>());
-  if (rc) return rc;
+  if (ctx->band32 && !ctx->unk_fresh) {  // FP32 store: FP32 hpsi, FP64 dot products
+    float *U = ctx->unk32.as<float>(), *HU = ctx->work32.as<float>();
+    if ((rc = launch_hpsi<float>(ctx, U, HU, N, mb))) return rc;
+    const float *xs[1] = {U}, *ys[1] = {HU};
+    if ((rc = multidot(ctx, 1, xs, ys, mb, ctx->dv, ctx->esp_dev.as<double>()))) return rc;
+  } else {
+    double *U = ctx->unk.as<double>(), *HU = ctx->work.as<double>();
+    rc = launch_hpsi<double>(ctx, U, HU, N, mb);
+    if (rc) return rc;
+    const double *xs[1] = {U}, *ys[1] = {HU};
+    rc = multidot(ctx, 1, xs, ys, mb, ctx->dv, ctx->esp_dev.as<double>());
+    if (rc) return rc;
+  }
   RSG_HIP(hipMemcpyAsync(esp, ctx->esp_dev.p, mb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   RSG_HIP(hipStreamSynchronize(ctx->stream));
   return RSG_OK;
@@ -1379,11 +1492,11 @@ std::vector<unsigned char> cg_graph_key(const rsg_ctx *ctx, int wsize, int ns, i
                                         int mloop) {
   std::vector<unsigned char> k;
   for (int v : {wsize, ns, nn, mbd, ncg, pc_kind, mloop, ctx->n[0], ctx->n[1], ctx->n[2], ctx->md, ctx->nl_mode,
-                int(ctx->have_ana), int(ctx->have_nl), ctx->tables.nzlma, int(ctx->pc_bf16)})
+                int(ctx->have_ana), int(ctx->have_nl), ctx->tables.nzlma, int(ctx->pc_bf16), int(ctx->band32)})
     key_put(k, v);
   key_put(k, ctx->N);
-  for (const DevBuf *b : {&ctx->unk, &ctx->cgbuf, &ctx->scal, &ctx->dotpart, &ctx->cproj, &ctx->uvpart, &ctx->uvunk,
-                          &ctx->vloc64, &ctx->vloc32, &ctx->dead})
+  for (const DevBuf *b : {&ctx->unk, &ctx->unk32, &ctx->cgbuf, &ctx->scal, &ctx->dotpart, &ctx->cproj, &ctx->uvpart,
+                          &ctx->uvunk, &ctx->vloc64, &ctx->vloc32, &ctx->dead})
     key_put(k, b->p);
   key_analytic(k, ctx->ana64);
   key_analytic(k, ctx->ana32);
@@ -1498,6 +1611,13 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
   }
   const int64_t fallbacks0 = ctx->lpc.fallbacks;
   constexpr bool kMixed = sizeof(W) == 4;
+  if (ctx->band32 && !kMixed)
+    return ctx->fail(RSG_ERR_STATE, "conjugate_gradient: the bands are in the FP32 store (FP32 CG only)");
+  if (ctx->band32 && ctx->cg_swgs != 0)
+    return ctx->fail(RSG_ERR_UNSUPPORTED, "conjugate_gradient: SWGS 1/2 needs the FP64 band store");
+  // FP32 store: the CG updates its bands in place
+  const bool in_place = kMixed && ctx->band32;
+  if (in_place) ctx->unk_fresh = false;
   const int mb = ctx->mb;
   const int64_t N = ctx->N;
   const double dv = ctx->dv;
@@ -1524,10 +1644,14 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
     const dim3 gN = grid_nb(N, nn), blkd(kThreads), gb((nn + 63) / 64), b64(64);
     W *x;
     if constexpr (kMixed) {
-      const int64_t tot = N * nn;
-      hipLaunchKernelGGL((k_to_work<W>), dim3(unsigned((tot + kThreads - 1) / kThreads)), blkd, 0, ctx->stream, xs64,
-                         x32, tot);
-      x = x32;
+      if (in_place) {
+        x = ctx->unk32.as<W>() + size_t(ns) * N;
+      } else {
+        const int64_t tot = N * nn;
+        hipLaunchKernelGGL((k_to_work<W>), dim3(unsigned((tot + kThreads - 1) / kThreads)), blkd, 0, ctx->stream,
+                           xs64, x32, tot);
+        x = x32;
+      }
     } else {
       x = reinterpret_cast<W *>(xs64);
     }
@@ -1615,7 +1739,10 @@ int cg_impl(rsg_ctx *ctx, int ncg, int pc_kind, int mloop, double *esp, double *
         if ((rc = multidot(ctx, 1, xs, ys, nn, dv, s.rb))) return rc;
       }
       hipLaunchKernelGGL(k_cg_guard, gb, b64, 0, ctx->stream, s.rb, s.res, s.E, s.E1, s.accept, nn);
-      hipLaunchKernelGGL((k_cg_accept<W>), gN, blkd, 0, ctx->stream, xs64, x32, pk, s.uc, s.accept, N);
+      if (in_place)
+        hipLaunchKernelGGL((k_cg_accept32<W>), gN, blkd, 0, ctx->stream, x, pk, s.uc, s.accept, N);
+      else
+        hipLaunchKernelGGL((k_cg_accept<W>), gN, blkd, 0, ctx->stream, xs64, x32, pk, s.uc, s.accept, N);
       RSG_HIP(hipGetLastError());
     }
     return RSG_OK;
@@ -1716,6 +1843,7 @@ __global__ __launch_bounds__(256) void k_random_bands(double *__restrict__ u, in
 // wf_module.f90:166-191: uniform random numbers per point and band).
 int random_bands(rsg_ctx *ctx, int n0, int nb, uint64_t seed) {
   if (ctx->mb == 0) return ctx->fail(RSG_ERR_STATE, "random_bands: no band store");
+  if (ctx->band32) return ctx->fail(RSG_ERR_STATE, "random_bands: the bands are in the FP32 store");
   if (n0 < 0 || nb < 0 || n0 + nb > ctx->mb) return ctx->fail(RSG_ERR_ARG, "random_bands: band range out of bounds");
   if (nb == 0) return RSG_OK;
   if (nb > 65535) return ctx->fail(RSG_ERR_ARG, "random_bands: nb above 65535");
@@ -1730,7 +1858,7 @@ int random_bands(rsg_ctx *ctx, int n0, int nb, uint64_t seed) {
 // whole store in FP32 (cg_module.f90:207, the first gradient of a CG call):
 // the residuals the learned preconditioner is trained on.
 int band_residuals_f32(rsg_ctx *ctx, float *g) {
-  int rc = check_store(ctx, "band_residuals");
+  int rc = check_store64(ctx, "band_residuals");
   if (rc) return rc;
   const int mb = ctx->mb;
   const int64_t N = ctx->N;
@@ -1774,10 +1902,45 @@ int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double
   return RSG_OK;
 }
 
+int check_store64(rsg_ctx *ctx, const char *who) {
+  int rc = check_store(ctx, who);
+  if (rc) return rc;
+  if (ctx->band32)
+    return ctx->fail(RSG_ERR_STATE, std::string(who) + ": the bands are in the FP32 store (set the band precision to 64)");
+  return RSG_OK;
+}
+
+int set_band_precision(rsg_ctx *ctx, int bits) {
+  int rc = check_store(ctx, "set_band_precision");
+  if (rc) return rc;
+  if (bits != 32 && bits != 64) return ctx->fail(RSG_ERR_ARG, "set_band_precision: bits must be 32 or 64");
+  if ((bits == 32) == ctx->band32) return RSG_OK;
+  const int64_t tot = ctx->N * int64_t(ctx->mb);
+  const dim3 g(unsigned((tot + kThreads - 1) / kThreads)), b(kThreads);
+  if (bits == 32) {
+    const size_t bytes = size_t(tot) * sizeof(float);
+    if (ctx->unk32.ensure(bytes) != hipSuccess || ctx->work32.ensure(bytes) != hipSuccess)
+      return ctx->fail(RSG_ERR_HIP, "set_band_precision: out of device memory");
+    hipLaunchKernelGGL((k_to_work<float>), g, b, 0, ctx->stream, ctx->unk.as<double>(), ctx->unk32.as<float>(), tot);
+  } else if (!ctx->unk_fresh) {  // else unk already holds the bands, in FP64
+    hipLaunchKernelGGL((k_to_double<float>), g, b, 0, ctx->stream, ctx->unk32.as<float>(), ctx->unk.as<double>(), tot);
+  }
+  RSG_HIP(hipGetLastError());
+  RSG_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->band32 = bits == 32;
+  ctx->unk_fresh = ctx->band32;  // the FP32 store starts as a copy of unk
+  return RSG_OK;
+}
+
 void solver_release(rsg_ctx *ctx) {
   ctx->dotpart.release();
   ctx->grampart.release();
+  ctx->grampart32.release();
   ctx->pcbuf.release();
+  ctx->unk32.release();
+  ctx->work32.release();
+  ctx->hsub32.release();
+  ctx->band32 = false;
   DeviceOpGuard guard;
   for (auto &g : ctx->cg_graph_cache)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);

@@ -15,6 +15,12 @@ int lpc_target(rsg_ctx *ctx, const float *g, float *t, int nb, int steps, double
 int band_residuals_f32(rsg_ctx *ctx, float *g);
 int esp_calc(rsg_ctx *ctx, double *esp);
 int random_bands(rsg_ctx *ctx, int n0, int nb, uint64_t seed);
+// 32: the bands move to the FP32 store (subspace_diag, CG (FP32 only),
+// esp_calc and the density then work on it; Gram-Schmidt goes through the
+// FP64 store); 64: back to the FP64 store
+int set_band_precision(rsg_ctx *ctx, int bits);
+// RSG_ERR_STATE while the bands are in the FP32 store
+int check_store64(rsg_ctx *ctx, const char *who);
 void solver_release(rsg_ctx *ctx);
 
 // Per-band dot products alpha * sum_i x_q(i,b) y_q(i,b) for up to 6 pairs

@@ -45,7 +45,7 @@ class ScfRunParams(ctypes.Structure):
                 ("mmix", ctypes.c_int), ("imix", ctypes.c_int), ("beta", ctypes.c_double), ("ekbt", ctypes.c_double),
                 ("etol_per_atom", ctypes.c_double), ("max_iter", ctypes.c_int), ("init_width", ctypes.c_double),
                 ("seed", ctypes.c_uint64), ("mixed", ctypes.c_int), ("fp64_below", ctypes.c_double),
-                ("fp32_stall", ctypes.c_int)]
+                ("fp32_stall", ctypes.c_int), ("fp32_store", ctypes.c_int), ("precond_bits", ctypes.c_int)]
 
 
 class ScfRunSummary(ctypes.Structure):
@@ -148,6 +148,8 @@ def load_library():
         "rsg_lpc_fallbacks": (_c_int, [_vp, _P(_c_i64)]),
         "rsg_lpc_set_reject_limit": (_c_int, [_vp, _c_int]),
         "rsg_set_precond_storage": (_c_int, [_vp, _c_int]),
+        "rsg_set_band_precision": (_c_int, [_vp, _c_int]),
+        "rsg_get_band_precision": (_c_int, [_vp, _P(_c_int)]),
         "rsg_lpc_reject_state": (_c_int, [_vp, _P(_c_int), _P(_c_i64)]),
     }
     for name, (res, args) in sig.items():
@@ -364,7 +366,8 @@ class Context:
             raise ValueError("the native SCF run draws its start bands on the device (start='uniform')")
         prm = ScfRunParams(int(nband), int(p.ncg), int(p.pc_kind), int(p.mloop), int(p.mmix), int(p.imix),
                            float(p.beta), float(p.ekbt), float(p.etol_per_atom), int(p.max_iter), float(p.init_width),
-                           int(p.seed) & 0xFFFFFFFFFFFFFFFF, int(bool(p.mixed)), float(p.fp64_below), int(p.fp32_stall))
+                           int(p.seed) & 0xFFFFFFFFFFFFFFFF, int(bool(p.mixed)), float(p.fp64_below), int(p.fp32_stall),
+                           int(bool(p.fp32_store)), int(p.precond_bits))
         out = ScfRunSummary()
         hist = np.zeros(int(p.max_iter))
         esp = np.zeros(int(nband))
@@ -587,6 +590,16 @@ class Context:
         c = _c_i64()
         self._check(self.L.rsg_lpc_fallbacks(self.h, ctypes.byref(c)))
         return c.value
+
+    def set_band_precision(self, bits: int):
+        """Move the band store to FP32 (32) for the mixed-precision phase, or
+        back to FP64 (64)."""
+        self._check(self.L.rsg_set_band_precision(self.h, int(bits)))
+
+    def band_precision(self) -> int:
+        b = _c_int()
+        self._check(self.L.rsg_get_band_precision(self.h, ctypes.byref(b)))
+        return b.value
 
     def set_precond_storage(self, bits: int):
         """Internal vectors of FP32 hprsdft calls in bfloat16 (16) or FP32

@@ -38,6 +38,7 @@ class ScfParams:
     fp64_below: float = 0.0  # 0: twice etol_per_atom (FP32 energies follow FP64 to well below 1e-6 Ha/atom)
     fp32_stall: int = 3      # ... or after this many FP32 iterations without a new lowest dE/atom
     precond_bits: int = 32  # storage of the FP32 hprsdft calls' internal vectors: 32, or 16 (bfloat16)
+    fp32_store: bool = True  # mixed: FP32 iterations keep the bands in the FP32 store (GS through FP64)
     lpc_reject_limit: int = 0  # pc_kind 5/6: after this many CG calls in a row with the network's output
                                # rejected, run hprsdft without trying it (0: try it in every call)
 
@@ -163,8 +164,16 @@ class SCF:
         self.setup_s = time.perf_counter() - t_setup
         self.setup_parts = parts
 
+    def _band_store(self, bits: int):
+        """FP32 iterations of a mixed run on the FP32 band store, FP64 ones
+        (and everything after the run) on the FP64 store."""
+        if self.p.mixed and self.p.fp32_store and getattr(self, "_store_bits", 64) != bits:
+            self.ctx.set_band_precision(bits)
+            self._store_bits = bits
+
     def step(self, it: int, e_prev: float | None, precision: int = 64) -> ScfIteration:
         c, p = self.ctx, self.p
+        self._band_store(precision)
         t = {}
         t0 = time.perf_counter()
         c.subspace_diag()
@@ -296,6 +305,8 @@ class SCF:
             e_prev = rec.etot
             if conv:
                 break
+        if getattr(self, "_store_bits", 64) != 64:
+            self._band_store(64)
         wall = time.perf_counter() - w0
         return ScfResult(conv, its, its[-1].etot, its[-1].fene, its[-1].esp, self.occ, self.eewald, wall, self.setup_s,
                          dict(getattr(self, "setup_parts", {})))

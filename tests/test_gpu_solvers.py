@@ -151,6 +151,61 @@ def test_fp32_conjugate_gradient_tracks_fp64(setup):
     np.testing.assert_allclose(e_mixed[:12], e_ref[:12], atol=1e-8)
 
 
+def test_fp32_band_store_tracks_fp64_and_returns(setup):
+    """The FP32 band store of the mixed phase against the FP64 store on the
+    same start bands, one eigensolver round each: subspace_diag's
+    eigenvalues to 2e-5 of the largest |e| (FP32 hpsi, products summed in
+    FP64, FP32 rotation), the in-place FP32 CG's band energies to 5e-5 (as
+    the copy-based FP32 CG), and esp_calc after Gram-Schmidt (FP64 through
+    the FP64 store) to 5e-5.  Meanwhile the FP64 CG, download and random
+    bands refuse; back in FP64 the store holds orthonormal bands (1e-6), and
+    FP64 rounds reach the FP64 fixed point (lowest 12 states within
+    1e-8 Ha)."""
+    from rsdft_develop_amd import gpu
+
+    cell, ham, ctx, u0 = setup
+    q = ham.gram_schmidt(u0, "lusl")
+    _load(ctx, q)
+    e64 = ctx.subspace_diag()
+    c64, _ = ctx.conjugate_gradient(ncg=4, pc_kind=4, mloop=3)
+    ctx.gram_schmidt()
+    s64 = ctx.esp_calc()
+    _load(ctx, q)
+    ctx.set_band_precision(32)
+    try:
+        assert ctx.band_precision() == 32
+        e32 = ctx.subspace_diag()
+        scale = np.abs(e64).max()
+        assert np.abs(e32 - e64).max() < 2e-5 * scale
+        for bad in (lambda: ctx.conjugate_gradient(ncg=4), lambda: ctx.download_bands(),
+                    lambda: ctx.random_bands(3)):
+            with pytest.raises(gpu.RSDFTGPUError):
+                bad()
+        c32, _ = ctx.conjugate_gradient(ncg=4, pc_kind=4, mloop=3, precision=32)
+        assert np.abs(c32 - c64).max() < 5e-5 * scale
+        ctx.gram_schmidt()
+        s32 = ctx.esp_calc()
+        assert np.abs(s32 - s64).max() < 5e-5 * scale
+    finally:
+        ctx.set_band_precision(64)
+    assert ctx.band_precision() == 64
+    u = ctx.download_bands()
+    ov = cell.dv * u.T @ u
+    assert np.abs(ov - np.eye(u.shape[1])).max() < 1e-6
+    for _ in range(14):
+        ctx.subspace_diag()
+        ctx.conjugate_gradient(ncg=4)
+        ctx.gram_schmidt()
+    e_mixed = np.sort(ctx.esp_calc())
+    w = q
+    for _ in range(20):
+        w, _ = ham.subspace_diag(w)
+        w, _, _ = ham.conjugate_gradient(w, ncg=4)
+        w = ham.gram_schmidt(w, "lusl")
+    e_ref = np.sort(ham.esp_calc(w))
+    np.testing.assert_allclose(e_mixed[:12], e_ref[:12], atol=1e-8)
+
+
 def _precond_gpu(ctx, g, ld, mloop, dtype):
     from rsdft_develop_amd import hip
 
