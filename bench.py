@@ -168,7 +168,7 @@ def progress(msg: str):
 
 
 def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_kind: int = 4, weights=None,
-            reject_limit: int = 0, precond_bits: int = 32):
+            reject_limit: int = 0, precond_bits: int = 32, continue_iters: int = 0):
     """Wall seconds of a whole SCF run to dE < etol Ha/atom (the run's own
     set-up on the device included: tables, initial density and GS).  Pulay
     mixing in G space with Kerker weights (IMIX 30), EKBT 0.01 Ha.  With
@@ -176,7 +176,9 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     and after `reject_limit` CG calls in a row whose network output failed
     the descent check the run keeps hprsdft (0: the network is tried in
     every call).  precond_bits 16 keeps the internal vectors of the FP32
-    hprsdft calls in bfloat16.
+    hprsdft calls in bfloat16.  With continue_iters > 0 the run then goes
+    on, untimed, for that many FP64 iterations (`esp_continued`,
+    `etot_continued`).
     After the timed run, untimed, the band energies are converged at the
     run's final potential (FP64 eigensolver passes, SCF.refine_levels) and
     returned beside the run's own as `esp_levels`."""
@@ -223,6 +225,13 @@ def scf_leg(nrep: int, mixed: bool, etol: float = 1e-6, max_iter: int = 80, pc_k
     levels, passes = run.refine_levels(tol=1e-9, max_passes=12)
     out["esp_levels"] = np.sort(levels).tolist()
     out["level_passes"] = passes
+    if continue_iters > 0:
+        progress(f"{tag}: {continue_iters} more FP64 iterations (untimed)")
+        run.p.mixed, run.p.etol_per_atom, run.p.max_iter = False, 0.0, continue_iters
+        rc = run.run()
+        out["esp_continued"] = np.sort(rc.esp).tolist()
+        out["etot_continued"] = rc.etot
+        out["continued_last_dE_per_atom"] = rc.iterations[-1].de_per_atom
     ctx.close()
     return out
 
@@ -511,6 +520,9 @@ def main() -> int:
     ap.add_argument("--precision64", type=int, default=1, help="Si64 FP64-vs-mixed SCF precision leg (0 skips)")
     ap.add_argument("--mixed-precond-bits", type=int, default=16, choices=(16, 32),
                     help="Si216 mixed legs: storage of the FP32 hprsdft calls' internal vectors (16 = bfloat16)")
+    ap.add_argument("--scf-continue", type=int, default=40,
+                    help="Si216 FP64 and mixed legs: FP64 iterations after the stop, untimed, before a second "
+                         "comparison (0 skips)")
     ap.add_argument("--lpc-reject-limit", type=int, default=2,
                     help="Si216 learned/hybrid legs: CG calls in a row with a rejected network output after which "
                          "the run keeps hprsdft (0: try the network in every call)")
@@ -557,15 +569,26 @@ def main() -> int:
         # against the plain FP64 run (the FP64 path equals the oracle's SCF
         # at Si8 and its operators at this size), from the runs' own band
         # energies and from the levels converged at each run's final potential
-        plain = scf_leg(args.scf, mixed=False)
+        plain = scf_leg(args.scf, mixed=False, continue_iters=args.scf_continue)
         runs = {"fp64": plain,
-                "mixed": scf_leg(args.scf, mixed=True, precond_bits=args.mixed_precond_bits),
+                "mixed": scf_leg(args.scf, mixed=True, precond_bits=args.mixed_precond_bits,
+                                 continue_iters=args.scf_continue),
                 "mixed_learned": scf_leg(args.scf, mixed=True, pc_kind=5, weights=LPC_WEIGHTS,
                                          reject_limit=args.lpc_reject_limit, precond_bits=args.mixed_precond_bits),
                 "mixed_hybrid": scf_leg(args.scf, mixed=True, pc_kind=6, weights=LPC_HYBRID,
                                         reject_limit=args.lpc_reject_limit, precond_bits=args.mixed_precond_bits)}
         e_ref = np.array(plain["esp"])
         l_ref = np.array(plain["esp_levels"])
+        continued = None
+        if args.scf_continue > 0:
+            f, m = runs["fp64"], runs["mixed"]
+            ef, em = np.array(f.pop("esp_continued")), np.array(m.pop("esp_continued"))
+            de = abs(m.pop("etot_continued") - f.pop("etot_continued")) / natom
+            continued = dict(fp64_iterations_after_stop=args.scf_continue,
+                             max_eig_diff_vs_fp64_occupied_Ha=float(np.abs(em - ef)[:nocc].max()),
+                             max_eig_diff_vs_fp64_Ha=float(np.abs(em - ef).max()),
+                             etot_diff_vs_fp64_Ha_per_atom=de,
+                             fp64_own_eig_change_from_stop_occupied_Ha=float(np.abs(ef - e_ref)[:nocc].max()))
         for name, r in runs.items():
             d = np.abs(np.array(r.pop("esp")) - e_ref)
             dl = np.abs(np.array(r.pop("esp_levels")) - l_ref)
@@ -579,7 +602,7 @@ def main() -> int:
             r["speedup_vs_fp64"] = round(plain["seconds"] / r["seconds"], 3)
         scf_out = {"workload": f"Si{natom} {36 * args.scf}^3 grid, {NBANDS[args.scf]} bands, EKBT 0.01 Ha, "
                                f"IMIX 30, uniform random start bands drawn on the device, to dE < 1e-6 Ha/atom",
-                   "occupied_bands": nocc, **runs,
+                   "occupied_bands": nocc, **runs, "mixed_vs_fp64_continued": continued,
                    "speedup_mixed_learned_vs_fp64": runs["mixed_learned"]["speedup_vs_fp64"],
                    "speedup_mixed_vs_fp64": runs["mixed"]["speedup_vs_fp64"]}
     step_out = step_leg(4, args.si512) if args.si512 > 0 else None

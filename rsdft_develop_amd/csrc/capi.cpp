@@ -238,6 +238,8 @@ int rsg_set_grid(rsg_ctx *ctx, int n1, int n2, int n3, const double *aa, int md)
   ctx->tables = rsg::HostTables();
   ctx->tables_state = 0;
   ctx->mb = 0;
+  ctx->band32 = false;
+  ctx->unk_fresh = false;
   return RSG_OK;
 }
 
