@@ -53,3 +53,71 @@ def test_stalled_fp32_switches():
     # lowest dE move the loop to FP64
     precs, conv = _run([1.0, 1e-3, 3e-5, 4e-5, 3.5e-5, 3.2e-5, 1e-5, 8e-7])
     assert precs == [32, 32, 32, 32, 32, 32, 64, 64] and conv
+
+
+class _FakeCtx:
+    """Records the band-store switches of SCF.step; every other call returns
+    fixed values, so the energy change is 0 after the first iteration."""
+
+    def __init__(self):
+        self.switches = []
+
+    def set_band_precision(self, bits):
+        self.switches.append(bits)
+
+    def subspace_diag(self):
+        return np.array([-1.0, 0.5])
+
+    def conjugate_gradient(self, *a, **k):
+        return np.array([-1.0, 0.5]), np.zeros(2)
+
+    def gram_schmidt(self):
+        pass
+
+    def esp_calc(self):
+        return np.array([-1.0, 0.5])
+
+    def density_integrals(self, field):
+        return 0.0, 0.0
+
+    def calc_density(self, occ):
+        pass
+
+    def hartree_xc(self, field):
+        return 0.0, 0.0
+
+
+def _run_with_store(**kw):
+    run = object.__new__(scf.SCF)
+    run.p = scf.ScfParams(mixed=True, etol_per_atom=1e-6, max_iter=10, **kw)
+    run.cell = type("C", (), {"natom": 1})()
+    run.ctx = _FakeCtx()
+    run.nelectron, run.eewald, run.eh_in, run.exc_in, run.setup_s = 2.0, 0.0, 0.0, 0.0, 0.0
+    run.update_potential = lambda: 0.0
+    res = run.run()
+    return [r.precision for r in res.iterations], run.ctx.switches, res.converged
+
+
+def test_fp32_iterations_run_on_the_fp32_band_store():
+    # FP32 until dE < 2 etol (the second iteration), two FP64 iterations to
+    # stop; the store goes to FP32 for the first and back for the third
+    precs, switches, conv = _run_with_store()
+    assert precs == [32, 32, 64, 64] and conv
+    assert switches == [32, 64]
+
+
+def test_band_store_stays_fp64_when_disabled():
+    precs, switches, conv = _run_with_store(fp32_store=False)
+    assert precs == [32, 32, 64, 64] and conv and switches == []
+
+
+def test_band_store_returns_to_fp64_when_the_loop_ends_in_fp32():
+    run = object.__new__(scf.SCF)
+    run.p = scf.ScfParams(mixed=True, etol_per_atom=1e-6, max_iter=1)
+    run.cell = type("C", (), {"natom": 1})()
+    run.ctx = _FakeCtx()
+    run.nelectron, run.eewald, run.eh_in, run.exc_in, run.setup_s = 2.0, 0.0, 0.0, 0.0, 0.0
+    run.update_potential = lambda: 0.0
+    res = run.run()
+    assert [r.precision for r in res.iterations] == [32] and not res.converged
+    assert run.ctx.switches == [32, 64]
