@@ -132,12 +132,16 @@ __global__ __launch_bounds__(64 * kWaves) void k_conv(Grid3 g, int64_t N, const 
     if (p0 >= N) break;  // wave-uniform
     const int64_t p = min(p0 + lr, N - 1);
     const int i1 = int(p % g.n1), i2 = int((p / g.n1) % g.n2), i3 = int(p / (int64_t(g.n1) * g.n2));
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    // three independent accumulator chains (taps c mod 3): the matrix-core
+    // latency of one chain of NC dependent products, not its issue rate,
+    // would otherwise bound the tile
+    f32x4 acc3[3] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const bf16x4 b = load_b<CIN>(src, g, i1, i2, i3, c, lq);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[c], b, acc, 0, 0, 0);
+      acc3[c % 3] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[c], b, acc3[c % 3], 0, 0, 0);
     }
+    const f32x4 acc = (acc3[0] + acc3[1]) + acc3[2];
     const bool ok = p0 + lr < N;
     if constexpr (MODE == kReluOut || MODE == kMaskedGrad) {
       uint16_t h[4];
