@@ -386,47 +386,72 @@ def scf_warmup(nrep: int):
     ctx.close()
 
 
-def step_leg(nrep: int, steps: int):
-    """Time per SCF step of Si(8 nrep^3) in mixed precision and in FP64 for
-    the same number of steps (IMIX 30, EKBT 0.01 Ha, device start bands), the
-    energy difference between the two after each step, and the FP64 GEMM
-    rates of Gram-Schmidt (S = dV U^T U: 2 MB^2 N FLOP) and subspace_diag
-    (U^T HU and the rotation: 4 MB^2 N FLOP) as lower bounds: whole phase
-    time, including its hpsi, Cholesky and eigensolve."""
-    from rsdft_develop_amd import gpu, scf, systems
+def step_leg(nrep: int, steps: int, precond_bits: int = 16, reject_limit: int = 2):
+    """Time per SCF step of Si(8 nrep^3) in FP64 and in mixed precision (FP32
+    band store): with hprsdft in FP32 (`mixed`, the same algorithm as FP64
+    up to rounding), with its internal vectors stored with `precond_bits`
+    (`mixed_bf16pc`), and with the learned preconditioner (`mixed_learned`,
+    configuration 4: tried until `reject_limit` CG calls in a row rejected
+    it), for the same number of steps (IMIX 30, EKBT 0.01 Ha, device start
+    bands); the energy difference to FP64 after each step (from random start
+    bands the first steps' energies follow the CG's progress, so only
+    `mixed` measures rounding; a different preconditioner moves them by its
+    different progress); and the FP64 GEMM rates of Gram-Schmidt
+    (S = dV U^T U: 2 MB^2 N FLOP) and subspace_diag (U^T HU and the
+    rotation: 4 MB^2 N FLOP) as lower bounds: whole phase time, including
+    its hpsi, Cholesky and eigensolve."""
+    from rsdft_develop_amd import gpu, lpc, scf, systems
 
     cell = systems.silicon_supercell(nrep, nband=NBANDS[nrep], displacement=0.05, seed=0)
     N, mb = cell.npoints, cell.nband
     out = {"workload": f"Si{cell.natom} {cell.ngrid[0]}^3 grid, {mb} bands, {steps} SCF steps"}
     etots = {}
-    for mixed in (True, False):
+    variants = [("mixed", True, 4, 32), ("mixed_bf16pc", True, 4, precond_bits),
+                ("mixed_learned", True, 5, precond_bits), ("fp64", False, 4, 32)]
+    for key, mixed, pc, bits in variants:
+        if pc == 5 and not os.path.exists(LPC_WEIGHTS):
+            continue
         ctx = gpu.Context(0)
         ctx.set_grid(cell.ngrid, cell.aa, cell.md)
         ctx.build_gth_projectors(cell)
         ctx.autotune_hpsi(64, mb, reps=1)
         ctx.autotune_hpsi(32, mb, reps=1)
         ctx.autotune_gram(mb, reps=1)
-        tag = f"step Si{cell.natom} {'mixed' if mixed else 'fp64'}"
+        if pc == 5:
+            ctx.lpc_set_params(lpc.load(LPC_WEIGHTS))
+        tag = f"step Si{cell.natom} {key}"
         progress(f"{tag}: start")
+        fb0 = ctx.lpc_fallbacks()
         run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, etol_per_atom=0.0, max_iter=steps, mixed=mixed, imix=30,
-                                          start="uniform"), ctx=ctx)
+                                          start="uniform", pc_kind=pc, precond_bits=bits,
+                                          lpc_reject_limit=reject_limit), ctx=ctx)
         res = run.run(callback=lambda r: progress(f"{tag}: iter {r.iter} prec {r.precision} "
                                                   + " ".join(f"{k} {v:.2f}" for k, v in r.times.items())))
         its = res.iterations
         per = {k: round(sum(r.times.get(k, 0.0) for r in its) / len(its), 3)
                for k in ("sd", "cg", "gs", "esp", "eto", "pot")}
-        key = "mixed" if mixed else "fp64"
         out[key] = dict(seconds_per_step=round(sum(sum(r.times.values()) for r in its) / len(its), 3),
                         setup_s=round(res.setup, 3), setup_parts_s={k: round(v, 4) for k, v in res.setup_parts.items()},
-                        per_phase_s=per,
+                        per_phase_s=per, pc_kind=pc,
                         precisions=[r.precision for r in its])
+        if mixed:
+            out[key]["fp32_phase_precond_storage_bits"] = bits
+        if pc == 5:
+            out[key]["descent_check_fallbacks"] = ctx.lpc_fallbacks() - fb0
+            out[key]["cg_calls_without_network"] = ctx.lpc_reject_state()[1]
         etots[key] = [r.etot for r in its]
         if not mixed:
             out["fp64"]["gs_gemm_tflops_lower_bound"] = round(2.0 * mb * mb * N / per["gs"] / 1e12, 2)
             out["fp64"]["sd_gemm_tflops_lower_bound"] = round(4.0 * mb * mb * N / per["sd"] / 1e12, 2)
         ctx.close()
-    out["mixed_vs_fp64_etot_diff_Ha_per_atom"] = [abs(a - b) / cell.natom for a, b in zip(etots["mixed"], etots["fp64"])]
-    out["speedup_mixed_vs_fp64_per_step"] = round(out["fp64"]["seconds_per_step"] / out["mixed"]["seconds_per_step"], 3)
+    for key in ("mixed", "mixed_bf16pc", "mixed_learned"):
+        if key in out:
+            out[key]["etot_diff_vs_fp64_Ha_per_atom"] = [abs(a - b) / cell.natom
+                                                         for a, b in zip(etots[key], etots["fp64"])]
+            out[key]["speedup_vs_fp64_per_step"] = round(out["fp64"]["seconds_per_step"]
+                                                         / out[key]["seconds_per_step"], 3)
+    out["mixed_vs_fp64_etot_diff_Ha_per_atom"] = out["mixed"]["etot_diff_vs_fp64_Ha_per_atom"]
+    out["speedup_mixed_vs_fp64_per_step"] = out["mixed"]["speedup_vs_fp64_per_step"]
     return out
 
 
@@ -605,7 +630,8 @@ def main() -> int:
                    "occupied_bands": nocc, **runs, "mixed_vs_fp64_continued": continued,
                    "speedup_mixed_learned_vs_fp64": runs["mixed_learned"]["speedup_vs_fp64"],
                    "speedup_mixed_vs_fp64": runs["mixed"]["speedup_vs_fp64"]}
-    step_out = step_leg(4, args.si512) if args.si512 > 0 else None
+    step_out = (step_leg(4, args.si512, precond_bits=args.mixed_precond_bits, reject_limit=args.lpc_reject_limit)
+                if args.si512 > 0 else None)
     out = {
         "metric": METRIC,
         "value": round(r64["gbs"], 1),

@@ -150,3 +150,39 @@ def test_si216_cg_with_learned_preconditioner_lowers_energies(pc_kind):
         del run
     finally:
         ctx.close()
+
+
+def test_si512_fp32_store_cg_with_learned_preconditioner_lowers_energies():
+    """Configuration 4's combination at full size (Si512, 144^3, 1100
+    bands): on the FP32 band store, with the bfloat16 preconditioner
+    vectors, one FP32 CG call with the learned preconditioner (pc_kind 5,
+    hprsdft for a block whose network output fails the descent check)
+    lowers every band energy to FP32 rounding, and Gram-Schmidt through the
+    FP64 store leaves the bands orthonormal."""
+    from rsdft_develop_amd import gpu, lpc, scf
+
+    cell = _cell(4)
+    print("case: Si512 FP32-store CG pc_kind 5", flush=True)
+    ctx = gpu.Context()
+    try:
+        run = scf.SCF(cell, scf.ScfParams(ekbt=0.01, imix=30, pc_kind=5, start="uniform", precond_bits=16), ctx=ctx)
+        ctx.lpc_set_params(lpc.load(os.path.join(ROOT, "rsdft_develop_amd", "data", "lpc_si64.npz")))
+        ctx.lpc_set_potential()
+        ctx.set_band_precision(32)
+        try:
+            e0 = ctx.subspace_diag()
+            before = ctx.lpc_fallbacks()
+            e1, res = ctx.conjugate_gradient(ncg=4, pc_kind=5, mloop=3, precision=32)
+            assert np.all(np.isfinite(e1)) and np.all(np.isfinite(res))
+            assert np.all(e1 <= e0 + 2e-5), np.max(e1 - e0)
+            print(f"mean energy drop {np.mean(e0 - e1):.4e} Ha, descent-check fallbacks "
+                  f"{ctx.lpc_fallbacks() - before}", flush=True)
+            ctx.gram_schmidt()
+        finally:
+            ctx.set_band_precision(64)
+        u = ctx.download_bands(0, 8)
+        ov = cell.dv * u.T @ u
+        assert np.abs(ov - np.eye(8)).max() < 1e-6
+        del run
+    finally:
+        ctx.close()
