@@ -147,10 +147,11 @@ int rsg_set_hpsi_shape(rsg_ctx *ctx, int precision, int ry, int ty, int zlen,
 int rsg_set_hpsi_order(rsg_ctx *ctx, int precision, int order);
 int rsg_get_hpsi_order(rsg_ctx *ctx, int precision, int *order);
 /* Form of the analytic projections (d_ps_nloc2_op_module.f90:182-191):
- * 0 = per sphere segment (one atom's points on one plane), 1 = by row
- * tiles (grid rows staged once per plane, every atom meeting them).  The
- * getter reports what an nb-band call would use (1 falls back to 0 where
- * the row tiles do not fit).  No reference counterpart. */
+ * 0 = per sphere segment (one atom's points on one plane; 32 bands per
+ * wave), 2 = the same with 64 bands per wave, 1 = by row tiles (grid rows
+ * staged once per plane, every atom meeting them).  The getter reports
+ * what an nb-band call would use (1 falls back to 0 where the row tiles do
+ * not fit).  No reference counterpart. */
 int rsg_set_hpsi_proj(rsg_ctx *ctx, int precision, int proj);
 /* Where rsg_build_gth_projectors builds the analytic projector
  * description (the set-up half of ps_nloc2_module.f90:42-1167 for

@@ -502,8 +502,8 @@ int rsg_get_hpsi_order(rsg_ctx *ctx, int precision, int *order) {
 
 int rsg_set_hpsi_proj(rsg_ctx *ctx, int precision, int proj) {
   if (!ctx) return RSG_ERR_ARG;
-  if ((precision != 64 && precision != 32) || (proj != 0 && proj != 1))
-    return ctx->fail(RSG_ERR_ARG, "set_hpsi_proj: precision 64/32 and proj 0/1");
+  if ((precision != 64 && precision != 32) || proj < 0 || proj > 2)
+    return ctx->fail(RSG_ERR_ARG, "set_hpsi_proj: precision 64/32 and proj 0/1/2");
   if (proj == 1) {
     RSG_HIP(hipSetDevice(ctx->device));
     int rc = rsg::ensure_row_tiles(ctx);

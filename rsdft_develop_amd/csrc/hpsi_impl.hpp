@@ -1059,7 +1059,7 @@ HpsiShape resolve_shape(const rsg_ctx *ctx, const HpsiShape &in, size_t s, int n
   sh.zlen = std::max(1, std::min(sh.zlen, n3));
   if (sh.bt != 4 && sh.bt != 8 && sh.bt != 16 && sh.bt != 20 && sh.bt != 32) sh.bt = 8;
   if (sh.order != 0 && sh.order != 1) sh.order = 0;
-  if (sh.proj != 0 && sh.proj != 1) sh.proj = 0;
+  if (sh.proj < 0 || sh.proj > 2) sh.proj = 0;
   return sh;
 }
 
@@ -1178,6 +1178,12 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const H
                            an.rt_slot_off.template as<int>(), an.rt_slot_gid.template as<int>(),
                            an.pts.template as<uint32_t>(), an.es.template as<T>(), an.ep.template as<T>(),
                            an.dd.template as<T>(), an.amp.template as<T>(), an.S, part);
+      } else if (sh.proj == 2) {  // 64 bands per wave: the segment's points and values once per 64 bands
+        hipLaunchKernelGGL((k_proj_m<T, 4>), dim3(unsigned(an.nseg * ((nbc + 63) / 64))), dim3(64), 0, ctx->stream,
+                           pc, ld, nbc, ctx->n[0], ctx->n[1], an.nseg, an.seg_order.template as<int>(),
+                           an.seg_beg.template as<int>(), an.seg_end.template as<int>(),
+                           an.seg_grp.template as<int>(), an.pts.template as<uint32_t>(), an.es.template as<T>(),
+                           an.ep.template as<T>(), an.dd.template as<T>(), an.amp.template as<T>(), an.S, part);
       } else {
         hipLaunchKernelGGL((k_proj_m<T, 2>), dim3(unsigned(an.nseg * ((nbc + 31) / 32))), dim3(64), 0, ctx->stream,
                            pc, ld, nbc, ctx->n[0], ctx->n[1], an.nseg, an.seg_order.template as<int>(),
@@ -1252,8 +1258,10 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const H
 #if RSG_HPSI_TYPE_IS_DOUBLE
 int effective_proj(rsg_ctx *ctx, int precision) {
   if (!ctx->have_ana || ctx->nl_mode == 1) return 0;
-  if (precision == 64) return ctx->shape64.proj == 1 && row_tiles_fit<double>(ctx, ctx->ana64) ? 1 : 0;
-  return ctx->shape32.proj == 1 && row_tiles_fit<float>(ctx, ctx->ana32) ? 1 : 0;
+  const int p = (precision == 64 ? ctx->shape64 : ctx->shape32).proj;
+  if (p != 1) return p;
+  if (precision == 64) return row_tiles_fit<double>(ctx, ctx->ana64) ? 1 : 0;
+  return row_tiles_fit<float>(ctx, ctx->ana32) ? 1 : 0;
 }
 
 int resolved_shape(rsg_ctx *ctx, int precision, int nb, HpsiShape *out) {
@@ -1385,7 +1393,12 @@ template <typename T> int autotune_t(rsg_ctx *ctx, int nb, int reps, float *best
       // the row tiles of a description built on the device come on first use
       if (ensure_row_tiles(ctx) == RSG_OK) {
         HpsiShape pj = best;
-        pj.proj = 1 - best.proj;
+        pj.proj = best.proj == 1 ? 0 : 1;
+        consider(pj);
+      }
+      {  // segment projections with 64 bands per wave
+        HpsiShape pj = best;
+        pj.proj = best.proj == 2 ? 0 : 2;
         consider(pj);
       }
       for (int f : {4, 8}) {  // smaller chunks keep psi and H psi in the Infinity Cache

@@ -354,7 +354,8 @@ class Context:
         return o.value
 
     def set_hpsi_proj(self, precision=64, proj=0):
-        """Analytic projections per sphere segment (0) or by row tiles (1)."""
+        """Analytic projections per sphere segment, 32 (0) or 64 (2) bands per
+        wave, or by row tiles (1)."""
         self._check(self.L.rsg_set_hpsi_proj(self.h, int(precision), int(proj)))
 
     def scf_run(self, nband: int, p) -> dict:

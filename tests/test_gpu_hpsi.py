@@ -218,7 +218,8 @@ def test_hpsi_errors(gpu_mod):
 @pytest.mark.parametrize("nrep,ng,disp,nb", [(1, 36, 0.1, 13), (2, 36, 0.05, 19), (1, 30, 0.1, 21)])
 def test_nonlocal_modes_agree_with_oracle(orc, gpu_mod, nrep, ng, disp, nb):
     """Tables (mode 1) and on-the-fly GTH projectors (mode 2), FP64 and FP32;
-    the analytic projections per sphere segment (proj 0) and by row tiles
+    the analytic projections per sphere segment with 32 or 64 bands per wave
+    (proj 0, 2; the band counts leave partial waves) and by row tiles
     (proj 1, whose last tile of rows is partial on these grids)."""
     from rsdft_develop_amd import hip
 
@@ -228,7 +229,7 @@ def test_nonlocal_modes_agree_with_oracle(orc, gpu_mod, nrep, ng, disp, nb):
     psi = systems.random_bands(cell.npoints, nb, seed=300 + nb)
     want = ham.hpsi(psi)
     for mode, order, chunk, proj in ((1, 0, 0, 0), (2, 0, 0, 0), (2, 1, 0, 0), (2, 0, 4, 0), (2, 1, 4, 0),
-                                     (2, 0, 0, 1), (2, 1, 4, 1), (2, 0, 17, 1)):
+                                     (2, 0, 0, 1), (2, 1, 4, 1), (2, 0, 17, 1), (2, 0, 0, 2), (2, 1, 4, 2)):
         ctx.set_nonlocal_mode(mode)
         for prec in (64, 32):
             ctx.set_hpsi_shape(prec, chunk=chunk)
