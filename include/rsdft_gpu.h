@@ -153,6 +153,12 @@ int rsg_get_hpsi_order(rsg_ctx *ctx, int precision, int *order);
  * what an nb-band call would use (1 falls back to 0 where the row tiles do
  * not fit).  No reference counterpart. */
 int rsg_set_hpsi_proj(rsg_ctx *ctx, int precision, int proj);
+/* LDS (KiB, 0..64) reserved by each wave of the segment projections (forms
+ * 0 and 2), unused by the kernel: fewer waves in flight per CU, so the
+ * sphere chords they share stay in L2 (the autotuner tries 4-32).  No
+ * reference counterpart. */
+int rsg_set_hpsi_proj_lds(rsg_ctx *ctx, int precision, int kib);
+int rsg_get_hpsi_proj_lds(rsg_ctx *ctx, int precision, int *kib);
 /* Where rsg_build_gth_projectors builds the analytic projector
  * description (the set-up half of ps_nloc2_module.f90:42-1167 for
  * l <= 1, n = 1 GTH sets): 0 (default) = with device kernels, 1 = on the

@@ -553,6 +553,21 @@ int rsg_get_analytic_array(rsg_ctx *ctx, int which, void *out, int64_t *count) {
   return RSG_OK;
 }
 
+int rsg_set_hpsi_proj_lds(rsg_ctx *ctx, int precision, int kib) {
+  if (!ctx) return RSG_ERR_ARG;
+  if ((precision != 64 && precision != 32) || kib < 0 || kib > 64)
+    return ctx->fail(RSG_ERR_ARG, "set_hpsi_proj_lds: precision 64/32 and 0 <= KiB <= 64");
+  (precision == 64 ? ctx->shape64 : ctx->shape32).plds = kib;
+  return RSG_OK;
+}
+
+int rsg_get_hpsi_proj_lds(rsg_ctx *ctx, int precision, int *kib) {
+  if (!ctx || !kib) return RSG_ERR_ARG;
+  if (precision != 64 && precision != 32) return ctx->fail(RSG_ERR_ARG, "get_hpsi_proj_lds: precision 64/32");
+  *kib = (precision == 64 ? ctx->shape64 : ctx->shape32).plds;
+  return RSG_OK;
+}
+
 int rsg_get_hpsi_proj(rsg_ctx *ctx, int precision, int *proj) {
   if (!ctx || !proj) return RSG_ERR_ARG;
   if (precision != 64 && precision != 32) return ctx->fail(RSG_ERR_ARG, "get_hpsi_proj: precision 64/32");

@@ -162,6 +162,7 @@ struct HpsiShape {
   int order = 0;  // analytic projectors: 0 = nonlocal writes, stencil adds; 1 = stencil writes, nonlocal adds
   int rx = 1;     // x-points per stencil thread: 1, or 2 (pair kernel: FP32, Md 6, even n1, ty | n2)
   int proj = 0;   // analytic projections: 0 = per sphere segment (k_proj_m), 1 = by row tiles (k_proj_rows)
+  int plds = 0;   // KiB of LDS reserved per segment-projection wave (fewer waves in flight: a smaller L2 set)
 };
 
 // Learned preconditioner state: bf16 weight matrices for the kernels, the

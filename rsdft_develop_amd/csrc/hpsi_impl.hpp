@@ -1060,6 +1060,7 @@ HpsiShape resolve_shape(const rsg_ctx *ctx, const HpsiShape &in, size_t s, int n
   if (sh.bt != 4 && sh.bt != 8 && sh.bt != 16 && sh.bt != 20 && sh.bt != 32) sh.bt = 8;
   if (sh.order != 0 && sh.order != 1) sh.order = 0;
   if (sh.proj < 0 || sh.proj > 2) sh.proj = 0;
+  if (sh.plds < 0 || sh.plds > 64) sh.plds = 0;
   return sh;
 }
 
@@ -1179,13 +1180,15 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const H
                            an.pts.template as<uint32_t>(), an.es.template as<T>(), an.ep.template as<T>(),
                            an.dd.template as<T>(), an.amp.template as<T>(), an.S, part);
       } else if (sh.proj == 2) {  // 64 bands per wave: the segment's points and values once per 64 bands
-        hipLaunchKernelGGL((k_proj_m<T, 4>), dim3(unsigned(an.nseg * ((nbc + 63) / 64))), dim3(64), 0, ctx->stream,
+        hipLaunchKernelGGL((k_proj_m<T, 4>), dim3(unsigned(an.nseg * ((nbc + 63) / 64))), dim3(64),
+                           size_t(sh.plds) << 10, ctx->stream,
                            pc, ld, nbc, ctx->n[0], ctx->n[1], an.nseg, an.seg_order.template as<int>(),
                            an.seg_beg.template as<int>(), an.seg_end.template as<int>(),
                            an.seg_grp.template as<int>(), an.pts.template as<uint32_t>(), an.es.template as<T>(),
                            an.ep.template as<T>(), an.dd.template as<T>(), an.amp.template as<T>(), an.S, part);
       } else {
-        hipLaunchKernelGGL((k_proj_m<T, 2>), dim3(unsigned(an.nseg * ((nbc + 31) / 32))), dim3(64), 0, ctx->stream,
+        hipLaunchKernelGGL((k_proj_m<T, 2>), dim3(unsigned(an.nseg * ((nbc + 31) / 32))), dim3(64),
+                           size_t(sh.plds) << 10, ctx->stream,
                            pc, ld, nbc, ctx->n[0], ctx->n[1], an.nseg, an.seg_order.template as<int>(),
                            an.seg_beg.template as<int>(), an.seg_end.template as<int>(),
                            an.seg_grp.template as<int>(), an.pts.template as<uint32_t>(), an.es.template as<T>(),
@@ -1401,6 +1404,12 @@ template <typename T> int autotune_t(rsg_ctx *ctx, int nb, int reps, float *best
         pj.proj = best.proj == 2 ? 0 : 2;
         consider(pj);
       }
+      if (best.proj != 1)  // fewer segment waves in flight per CU
+        for (int kib : {4, 8, 16, 32}) {
+          HpsiShape pl = best;
+          pl.plds = kib;
+          consider(pl);
+        }
       for (int f : {4, 8}) {  // smaller chunks keep psi and H psi in the Infinity Cache
         HpsiShape d = best;
         d.chunk = std::max(1, nb / f);

@@ -148,6 +148,8 @@ def load_library():
         "rsg_lpc_fallbacks": (_c_int, [_vp, _P(_c_i64)]),
         "rsg_lpc_set_reject_limit": (_c_int, [_vp, _c_int]),
         "rsg_set_precond_storage": (_c_int, [_vp, _c_int]),
+        "rsg_set_hpsi_proj_lds": (_c_int, [_vp, _c_int, _c_int]),
+        "rsg_get_hpsi_proj_lds": (_c_int, [_vp, _c_int, _P(_c_int)]),
         "rsg_set_band_precision": (_c_int, [_vp, _c_int]),
         "rsg_get_band_precision": (_c_int, [_vp, _P(_c_int)]),
         "rsg_lpc_reject_state": (_c_int, [_vp, _P(_c_int), _P(_c_i64)]),
@@ -398,6 +400,10 @@ class Context:
             self._check(self.L.rsg_get_analytic_array(self.h, which, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n)))
         return out
 
+    def set_hpsi_proj_lds(self, precision=64, kib=0):
+        """LDS (KiB) each segment-projection wave reserves: fewer waves per CU."""
+        self._check(self.L.rsg_set_hpsi_proj_lds(self.h, int(precision), int(kib)))
+
     def hpsi_proj(self, precision=64) -> int:
         o = _c_int()
         self._check(self.L.rsg_get_hpsi_proj(self.h, int(precision), ctypes.byref(o)))
@@ -416,6 +422,9 @@ class Context:
         self._check(self.L.rsg_get_hpsi_xpoints(self.h, int(precision), int(nb), ctypes.byref(rx)))
         d["rx"] = rx.value
         d["proj"] = self.hpsi_proj(precision)
+        k = _c_int()
+        self._check(self.L.rsg_get_hpsi_proj_lds(self.h, int(precision), ctypes.byref(k)))
+        d["plds"] = k.value
         return d
 
     def apply_hpsi_shape(self, precision, sh: dict, chunk=None):
@@ -425,6 +434,7 @@ class Context:
         self.set_hpsi_order(precision, sh.get("order", 0))
         self.set_hpsi_xpoints(precision, sh.get("rx", 1))
         self.set_hpsi_proj(precision, sh.get("proj", 0))
+        self.set_hpsi_proj_lds(precision, sh.get("plds", 0))
 
     def autotune_hpsi(self, precision=64, nb=1, reps=5) -> float:
         ms = ctypes.c_float()

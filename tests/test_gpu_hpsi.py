@@ -248,6 +248,30 @@ def test_nonlocal_modes_agree_with_oracle(orc, gpu_mod, nrep, ng, disp, nb):
         d_out.free()
 
 
+def test_projection_lds_reservation_leaves_results_unchanged(orc, gpu_mod):
+    """The LDS a segment-projection wave reserves only limits how many waves
+    share a CU: H psi is bit for bit the same for 0 and 32 KiB, in both
+    projection forms and precisions, and equals the oracle."""
+    cell = systems.silicon_supercell(1, ngrid_per_cell=36, displacement=0.1, seed=5)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    psi = systems.random_bands(cell.npoints, 40, seed=77)
+    want = ham.hpsi(psi)
+    for proj in (0, 2):
+        got = {}
+        for kib in (0, 32):
+            for prec in (64, 32):
+                ctx.set_hpsi_proj(prec, proj)
+                ctx.set_hpsi_proj_lds(prec, kib)
+            assert ctx.hpsi_shape(64, 40)["plds"] == kib
+            print(f"case: projection form {proj}, {kib} KiB reserved", flush=True)
+            got[kib] = ctx.hamiltonian(psi)
+        assert np.array_equal(got[0], got[32])
+        _close(got[32], want, 1e-12)
+    for prec in (64, 32):
+        ctx.set_hpsi_proj_lds(prec, 0)
+        ctx.set_hpsi_proj(prec, 0)
+
+
 @pytest.mark.parametrize("bt", [4, 8, 16, 20, 32])
 def test_apply_band_tiles_agree_with_oracle(orc, gpu_mod, bt):
     """The analytic apply with 16, 32, 64, 80 and 128 bands per wave, writing
