@@ -159,6 +159,15 @@ int rsg_set_hpsi_proj(rsg_ctx *ctx, int precision, int proj);
  * reference counterpart. */
 int rsg_set_hpsi_proj_lds(rsg_ctx *ctx, int precision, int kib);
 int rsg_get_hpsi_proj_lds(rsg_ctx *ctx, int precision, int *kib);
+/* Segment projections (forms 0 and 2) in 1 (default), 2, 4 or 8 pieces
+ * per sphere segment, each piece a wave of its own with its own partial
+ * sums: shorter waves read the chords that neighbouring spheres share in a
+ * plane closer together in time, so the second read finds them in L2 (the
+ * autotuner tries 2-8).  The partial sums are added in a fixed order; the
+ * result differs from one piece only by rounding.  No reference
+ * counterpart. */
+int rsg_set_hpsi_proj_split(rsg_ctx *ctx, int precision, int pieces);
+int rsg_get_hpsi_proj_split(rsg_ctx *ctx, int precision, int *pieces);
 /* Where rsg_build_gth_projectors builds the analytic projector
  * description (the set-up half of ps_nloc2_module.f90:42-1167 for
  * l <= 1, n = 1 GTH sets): 0 (default) = with device kernels, 1 = on the

@@ -568,6 +568,21 @@ int rsg_get_hpsi_proj_lds(rsg_ctx *ctx, int precision, int *kib) {
   return RSG_OK;
 }
 
+int rsg_set_hpsi_proj_split(rsg_ctx *ctx, int precision, int pieces) {
+  if (!ctx) return RSG_ERR_ARG;
+  if ((precision != 64 && precision != 32) || (pieces != 1 && pieces != 2 && pieces != 4 && pieces != 8))
+    return ctx->fail(RSG_ERR_ARG, "set_hpsi_proj_split: precision 64/32 and pieces 1, 2, 4 or 8");
+  (precision == 64 ? ctx->shape64 : ctx->shape32).psplit = pieces;
+  return RSG_OK;
+}
+
+int rsg_get_hpsi_proj_split(rsg_ctx *ctx, int precision, int *pieces) {
+  if (!ctx || !pieces) return RSG_ERR_ARG;
+  if (precision != 64 && precision != 32) return ctx->fail(RSG_ERR_ARG, "get_hpsi_proj_split: precision 64/32");
+  *pieces = (precision == 64 ? ctx->shape64 : ctx->shape32).psplit;
+  return RSG_OK;
+}
+
 int rsg_get_hpsi_proj(rsg_ctx *ctx, int precision, int *proj) {
   if (!ctx || !proj) return RSG_ERR_ARG;
   if (precision != 64 && precision != 32) return ctx->fail(RSG_ERR_ARG, "get_hpsi_proj: precision 64/32");

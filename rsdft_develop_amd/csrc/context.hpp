@@ -163,6 +163,7 @@ struct HpsiShape {
   int rx = 1;     // x-points per stencil thread: 1, or 2 (pair kernel: FP32, Md 6, even n1, ty | n2)
   int proj = 0;   // analytic projections: 0 = per sphere segment (k_proj_m), 1 = by row tiles (k_proj_rows)
   int plds = 0;   // KiB of LDS reserved per segment-projection wave (fewer waves in flight: a smaller L2 set)
+  int psplit = 1; // segment projections: pieces per segment (1, 2, 4, 8), each its own wave and partial slot
 };
 
 // Learned preconditioner state: bf16 weight matrices for the kernels, the

@@ -517,16 +517,17 @@ __global__ __launch_bounds__(256) void k_nl_scatter(T *__restrict__ hpsi, int64_
   }
 }
 
-// C[(g*4 + l)*nbc + b] = iuV dV * sum over the group's segments
+// C[(g*4 + l)*nbc + b] = iuV dV * sum over the group's segments (npc
+// partial slots per segment, contiguous)
 template <typename T>
 __global__ void k_proj_reduce(const T *__restrict__ part, int nbc, int ngroup, const int *__restrict__ grp_seg_off,
-                              const T *__restrict__ scale, T *__restrict__ cproj) {
+                              int npc, const T *__restrict__ scale, T *__restrict__ cproj) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= int64_t(ngroup) * 4 * nbc) return;
   const int gl = int(t / nbc), b = int(t - int64_t(gl) * nbc);
   const int g = gl >> 2, l = gl & 3;
   T v = T(0);
-  for (int s = grp_seg_off[g]; s < grp_seg_off[g + 1]; ++s) v += part[(int64_t(s) * 4 + l) * nbc + b];
+  for (int s = grp_seg_off[g] * npc; s < grp_seg_off[g + 1] * npc; ++s) v += part[(int64_t(s) * 4 + l) * nbc + b];
   cproj[t] = scale[gl] * v;
 }
 
@@ -566,38 +567,44 @@ __device__ inline T ana_value_k(const T *__restrict__ es, const T *__restrict__ 
   return fp ? amp_p * ep[ix] * ep[iy] * ep[iz] * dd[ia] : T(0);
 }
 
-// Partial projections of one segment (one atom, one plane run):
-//   part[(s*4 + k)*nbc + b] = sum_{j in segment} beta_k(j) psi(j, b)
+// Partial projections of one piece of a segment (one atom, one plane run;
+// npc pieces of whole 32-point passes per segment, slot s*npc + q):
+//   part[((s*npc + q)*4 + k)*nbc + b] = sum_{j in piece} beta_k(j) psi(j, b)
 // D[band][k] += A[band][point] B[point][k]: one wave, NBT tiles of 16 bands.
+// Short pieces shorten the time between the reads of two spheres that
+// share a chord of the plane, so the second read finds it in L2.
 // The A operand puts 16 bands of one point on neighbouring lanes, so psi is
 // first staged through LDS in passes of 32 points, loaded with neighbouring
 // lanes on neighbouring points (rows padded by one element against bank
 // conflicts).
 template <typename T, int NBT>
 __global__ __launch_bounds__(64) void k_proj_m(
-    const T *__restrict__ psi, int64_t ld, int nbc, int n1, int n2, int nseg, const int *__restrict__ seg_order,
-    const int *__restrict__ seg_beg, const int *__restrict__ seg_end, const int *__restrict__ seg_grp,
-    const uint32_t *__restrict__ pts, const T *__restrict__ es, const T *__restrict__ ep,
-    const T *__restrict__ dd, const T *__restrict__ amp, int S, T *__restrict__ part) {
+    const T *__restrict__ psi, int64_t ld, int nbc, int n1, int n2, int nseg, int npc,
+    const int *__restrict__ seg_order, const int *__restrict__ seg_beg, const int *__restrict__ seg_end,
+    const int *__restrict__ seg_grp, const uint32_t *__restrict__ pts, const T *__restrict__ es,
+    const T *__restrict__ ep, const T *__restrict__ dd, const T *__restrict__ amp, int S, T *__restrict__ part) {
   using M = Mfma<T>;
   constexpr int P = 32;  // points per pass
   __shared__ T tile[NBT][16][P + 1];
   __shared__ T bsh[4][P + 1];  // projector values of the pass
-  // logical block: segment (in plane order) fastest, then band group
+  // logical block: piece fastest, then segment (in plane order), then band group
   const unsigned lblk = xcd_block(blockIdx.x, gridDim.x);
-  const int s = seg_order[lblk % unsigned(nseg)];
+  const unsigned nsp = unsigned(nseg) * unsigned(npc), sp = lblk % nsp;
+  const int s = seg_order[sp / unsigned(npc)], q = int(sp % unsigned(npc));
   const int l = threadIdx.x;
   const int lo = l & 15, hi = l >> 4;
   const int lp = l & (P - 1), lb = l / P;  // staging: point, band parity
-  const int b0 = int(lblk / unsigned(nseg)) * 16 * NBT;
-  const int beg = seg_beg[s], end = seg_end[s], g = seg_grp[s];
+  const int b0 = int(lblk / nsp) * 16 * NBT;
+  const int sbeg = seg_beg[s], send = seg_end[s], g = seg_grp[s];
+  const int npass = (send - sbeg + P - 1) / P;
+  const int beg = sbeg + P * (npass * q / npc), end = min(send, sbeg + P * (npass * (q + 1) / npc));
   const int64_t gS = int64_t(g) * S;
   const T amp_s = amp[4 * g], amp_p = amp[4 * g + 1];
   const int64_t n12 = int64_t(n1) * n2;
   typename M::V acc[NBT];
 #pragma unroll
   for (int t = 0; t < NBT; ++t) acc[t] = typename M::V{};
-  uint32_t wd_next = pts[beg + lp < end ? beg + lp : beg];
+  uint32_t wd_next = beg < end ? pts[beg + lp < end ? beg + lp : beg] : 0u;  // a piece may be empty
   for (int i = beg; i < end; i += P) {
     {
       const int ip = i + lp;
@@ -641,7 +648,7 @@ __global__ __launch_bounds__(64) void k_proj_m(
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int band = b0 + 16 * t + M::row(l, r);
-      if (lo < 4 && band < nbc) part[(int64_t(s) * 4 + lo) * nbc + band] = acc[t][r];
+      if (lo < 4 && band < nbc) part[((int64_t(s) * npc + q) * 4 + lo) * nbc + band] = acc[t][r];
     }
 }
 
@@ -1061,6 +1068,7 @@ HpsiShape resolve_shape(const rsg_ctx *ctx, const HpsiShape &in, size_t s, int n
   if (sh.order != 0 && sh.order != 1) sh.order = 0;
   if (sh.proj < 0 || sh.proj > 2) sh.proj = 0;
   if (sh.plds < 0 || sh.plds > 64) sh.plds = 0;
+  if (sh.psplit != 1 && sh.psplit != 2 && sh.psplit != 4 && sh.psplit != 8) sh.psplit = 1;
   return sh;
 }
 
@@ -1164,7 +1172,8 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const H
     AnalyticDev<T> &an = ana_of<T>(ctx);
     T *cp = nullptr, *part = nullptr;
     const bool rows = sh.proj == 1 && row_tiles_fit<T>(ctx, an);
-    const int npart = rows ? an.rt_nslot : an.nseg;
+    const int npc = rows ? 1 : sh.psplit;  // partial slots per segment
+    const int npart = rows ? an.rt_nslot : an.nseg * npc;
     int rc0 = projection_buffers<T>(ctx, scr, size_t(an.ngroup) * 4, size_t(npart) * 4, sh, &cp, &part);
     if (rc0) return rc0;
     for (int c0 = 0; c0 < nb; c0 += sh.chunk) {
@@ -1180,23 +1189,23 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const H
                            an.pts.template as<uint32_t>(), an.es.template as<T>(), an.ep.template as<T>(),
                            an.dd.template as<T>(), an.amp.template as<T>(), an.S, part);
       } else if (sh.proj == 2) {  // 64 bands per wave: the segment's points and values once per 64 bands
-        hipLaunchKernelGGL((k_proj_m<T, 4>), dim3(unsigned(an.nseg * ((nbc + 63) / 64))), dim3(64),
+        hipLaunchKernelGGL((k_proj_m<T, 4>), dim3(unsigned(an.nseg * npc * ((nbc + 63) / 64))), dim3(64),
                            size_t(sh.plds) << 10, ctx->stream,
-                           pc, ld, nbc, ctx->n[0], ctx->n[1], an.nseg, an.seg_order.template as<int>(),
+                           pc, ld, nbc, ctx->n[0], ctx->n[1], an.nseg, npc, an.seg_order.template as<int>(),
                            an.seg_beg.template as<int>(), an.seg_end.template as<int>(),
                            an.seg_grp.template as<int>(), an.pts.template as<uint32_t>(), an.es.template as<T>(),
                            an.ep.template as<T>(), an.dd.template as<T>(), an.amp.template as<T>(), an.S, part);
       } else {
-        hipLaunchKernelGGL((k_proj_m<T, 2>), dim3(unsigned(an.nseg * ((nbc + 31) / 32))), dim3(64),
+        hipLaunchKernelGGL((k_proj_m<T, 2>), dim3(unsigned(an.nseg * npc * ((nbc + 31) / 32))), dim3(64),
                            size_t(sh.plds) << 10, ctx->stream,
-                           pc, ld, nbc, ctx->n[0], ctx->n[1], an.nseg, an.seg_order.template as<int>(),
+                           pc, ld, nbc, ctx->n[0], ctx->n[1], an.nseg, npc, an.seg_order.template as<int>(),
                            an.seg_beg.template as<int>(), an.seg_end.template as<int>(),
                            an.seg_grp.template as<int>(), an.pts.template as<uint32_t>(), an.es.template as<T>(),
                            an.ep.template as<T>(), an.dd.template as<T>(), an.amp.template as<T>(), an.S, part);
       }
       const int64_t nred = int64_t(an.ngroup) * 4 * nbc;
       hipLaunchKernelGGL((k_proj_reduce<T>), dim3(unsigned((nred + 255) / 256)), dim3(256), 0, ctx->stream, part,
-                         nbc, an.ngroup, (rows ? an.grp_slot_off : an.grp_seg_off).template as<int>(),
+                         nbc, an.ngroup, (rows ? an.grp_slot_off : an.grp_seg_off).template as<int>(), npc,
                          an.scale.template as<T>(), cp);
       const bool nl_after = sh.order == 1;
       if (nl_after) {
@@ -1409,6 +1418,12 @@ template <typename T> int autotune_t(rsg_ctx *ctx, int nb, int reps, float *best
           HpsiShape pl = best;
           pl.plds = kib;
           consider(pl);
+        }
+      if (best.proj != 1)  // segments in pieces: chords shared by two spheres read close in time
+        for (int np : {2, 4, 8}) {
+          HpsiShape ps = best;
+          ps.psplit = np;
+          consider(ps);
         }
       for (int f : {4, 8}) {  // smaller chunks keep psi and H psi in the Infinity Cache
         HpsiShape d = best;

@@ -1503,7 +1503,7 @@ std::vector<unsigned char> cg_graph_key(const rsg_ctx *ctx, int wsize, int ns, i
   key_tables(k, ctx->nl64);
   key_tables(k, ctx->nl32);
   for (const HpsiShape *h : {&ctx->shape64, &ctx->shape32})
-    for (int v : {h->ry, h->ty, h->zlen, h->chunk, h->bt, h->order, h->rx, h->proj, h->plds}) key_put(k, v);
+    for (int v : {h->ry, h->ty, h->zlen, h->chunk, h->bt, h->order, h->rx, h->proj, h->plds, h->psplit}) key_put(k, v);
   for (double v : ctx->coef_lap) key_put(k, v);
   for (double v : {ctx->coef_lap0, ctx->dv, ctx->hgrid[0], ctx->hgrid[1], ctx->hgrid[2], ctx->ggg[0], ctx->ggg[1],
                    ctx->ggg[2]})

@@ -150,6 +150,8 @@ def load_library():
         "rsg_set_precond_storage": (_c_int, [_vp, _c_int]),
         "rsg_set_hpsi_proj_lds": (_c_int, [_vp, _c_int, _c_int]),
         "rsg_get_hpsi_proj_lds": (_c_int, [_vp, _c_int, _P(_c_int)]),
+        "rsg_set_hpsi_proj_split": (_c_int, [_vp, _c_int, _c_int]),
+        "rsg_get_hpsi_proj_split": (_c_int, [_vp, _c_int, _P(_c_int)]),
         "rsg_set_band_precision": (_c_int, [_vp, _c_int]),
         "rsg_get_band_precision": (_c_int, [_vp, _P(_c_int)]),
         "rsg_lpc_reject_state": (_c_int, [_vp, _P(_c_int), _P(_c_i64)]),
@@ -404,6 +406,10 @@ class Context:
         """LDS (KiB) each segment-projection wave reserves: fewer waves per CU."""
         self._check(self.L.rsg_set_hpsi_proj_lds(self.h, int(precision), int(kib)))
 
+    def set_hpsi_proj_split(self, precision=64, pieces=1):
+        """Sphere segments projected in 1, 2, 4 or 8 pieces (one wave each)."""
+        self._check(self.L.rsg_set_hpsi_proj_split(self.h, int(precision), int(pieces)))
+
     def hpsi_proj(self, precision=64) -> int:
         o = _c_int()
         self._check(self.L.rsg_get_hpsi_proj(self.h, int(precision), ctypes.byref(o)))
@@ -425,6 +431,8 @@ class Context:
         k = _c_int()
         self._check(self.L.rsg_get_hpsi_proj_lds(self.h, int(precision), ctypes.byref(k)))
         d["plds"] = k.value
+        self._check(self.L.rsg_get_hpsi_proj_split(self.h, int(precision), ctypes.byref(k)))
+        d["psplit"] = k.value
         return d
 
     def apply_hpsi_shape(self, precision, sh: dict, chunk=None):
@@ -435,6 +443,7 @@ class Context:
         self.set_hpsi_xpoints(precision, sh.get("rx", 1))
         self.set_hpsi_proj(precision, sh.get("proj", 0))
         self.set_hpsi_proj_lds(precision, sh.get("plds", 0))
+        self.set_hpsi_proj_split(precision, sh.get("psplit", 1))
 
     def autotune_hpsi(self, precision=64, nb=1, reps=5) -> float:
         ms = ctypes.c_float()

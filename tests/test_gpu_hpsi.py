@@ -272,6 +272,39 @@ def test_projection_lds_reservation_leaves_results_unchanged(orc, gpu_mod):
         ctx.set_hpsi_proj(prec, 0)
 
 
+def test_projection_segment_pieces_agree_with_oracle(orc, gpu_mod):
+    """Sphere segments projected in 2, 4 or 8 pieces (the segments at the
+    poles of a sphere span one 32-point pass, so some pieces are empty) give
+    the oracle's H psi in both segment forms and precisions, and the same
+    result on a second call."""
+    cell = systems.silicon_supercell(1, ngrid_per_cell=36, displacement=0.1, seed=5)
+    ham, ctx = _pair(orc, gpu_mod, cell)
+    from rsdft_develop_amd import hip
+
+    psi = systems.random_bands(cell.npoints, 40, seed=78)
+    want = ham.hpsi(psi)
+    d_in = hip.DeviceArray.from_numpy(psi, dtype=np.float32)
+    d_out = hip.DeviceArray(psi.shape, dtype=np.float32)
+    for proj in (0, 2):
+        for pieces in (1, 2, 4, 8):
+            for prec in (64, 32):
+                ctx.set_hpsi_proj(prec, proj)
+                ctx.set_hpsi_proj_split(prec, pieces)
+            assert ctx.hpsi_shape(64, 40)["psplit"] == pieces
+            print(f"case: projection form {proj}, {pieces} pieces per segment", flush=True)
+            got = ctx.hamiltonian(psi)
+            _close(got, want, 1e-12)
+            assert np.array_equal(ctx.hamiltonian(psi), got)
+            ctx.hpsi_device(d_in.ptr, d_out.ptr, cell.npoints, 40, dtype=np.float32)
+            ctx.synchronize()
+            _close(d_out.to_numpy().astype(np.float64), want, 2e-5)
+    d_in.free()
+    d_out.free()
+    for prec in (64, 32):
+        ctx.set_hpsi_proj_split(prec, 1)
+        ctx.set_hpsi_proj(prec, 0)
+
+
 @pytest.mark.parametrize("bt", [4, 8, 16, 20, 32])
 def test_apply_band_tiles_agree_with_oracle(orc, gpu_mod, bt):
     """The analytic apply with 16, 32, 64, 80 and 128 bands per wave, writing
