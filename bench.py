@@ -52,11 +52,12 @@ def gpu_leg(cell, vloc, steps: int, warmup: int, dtype, pin=None):
 
     ctx = gpu.Context(0).setup(cell, vloc)
     prec = 64 if dtype == np.float64 else 32
-    if pin:  # launch shape given (ry, ty, zlen, chunk, bt, order[, rx]): a counter run repeats a timed run's shape
-        keys = ("ry", "ty", "zlen", "chunk", "bt", "order", "rx")
+    if pin:  # launch shape given (ry, ty, zlen, chunk, bt, order[, rx, proj, plds, psplit]): a counter run
+        # repeats a timed run's shape
+        keys = ("ry", "ty", "zlen", "chunk", "bt", "order", "rx", "proj", "plds", "psplit")
         ctx.apply_hpsi_shape(prec, dict(zip(keys, pin)))
     else:
-        ctx.autotune_hpsi(prec, cell.nband, reps=3)  # launch shape from the library's autotuner
+        ctx.autotune_hpsi(prec, cell.nband, reps=6)  # launch shape from the library's autotuner
     shape = ctx.hpsi_shape(prec, cell.nband)
     tabs = ctx.projector_tables()
     E = int(tabs["nentries"])
@@ -553,8 +554,8 @@ def main() -> int:
                          "the run keeps hprsdft (0: try the network in every call)")
     ap.add_argument("--cpu-si216-steps", type=int, default=0,
                     help="SCF steps of the oracle at Si216 on the host cores (0 skips; about a minute per step)")
-    ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx]")
-    ap.add_argument("--shape32", default=None, help="pin the FP32 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx]")
+    ap.add_argument("--shape64", default=None, help="pin the FP64 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx,proj,plds,psplit]")
+    ap.add_argument("--shape32", default=None, help="pin the FP32 hpsi launch shape: ry,ty,zlen,chunk,bt,order[,rx,proj,plds,psplit]")
     args = ap.parse_args()
     # more hardware queues for the batched leg's concurrent streams; read by
     # HIP when it starts, which has not happened yet in this process

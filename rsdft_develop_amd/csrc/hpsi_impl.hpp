@@ -518,17 +518,27 @@ __global__ __launch_bounds__(256) void k_nl_scatter(T *__restrict__ hpsi, int64_
 }
 
 // C[(g*4 + l)*nbc + b] = iuV dV * sum over the group's segments (npc
-// partial slots per segment, contiguous)
+// partial slots per segment, contiguous).  A block owns one (g, l) row and
+// 64 bands; each of its four waves sums every fourth slot (four short load
+// chains instead of one long one), and the four sums are added in a fixed
+// order.
 template <typename T>
-__global__ void k_proj_reduce(const T *__restrict__ part, int nbc, int ngroup, const int *__restrict__ grp_seg_off,
-                              int npc, const T *__restrict__ scale, T *__restrict__ cproj) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= int64_t(ngroup) * 4 * nbc) return;
-  const int gl = int(t / nbc), b = int(t - int64_t(gl) * nbc);
+__global__ __launch_bounds__(256) void k_proj_reduce(const T *__restrict__ part, int nbc, int ngroup,
+                                                     const int *__restrict__ grp_seg_off, int npc,
+                                                     const T *__restrict__ scale, T *__restrict__ cproj) {
+  __shared__ T red[4][64];
+  const int nbb = (nbc + 63) / 64;
+  const int gl = int(blockIdx.x) / nbb, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int b = (int(blockIdx.x) % nbb) * 64 + lane;
   const int g = gl >> 2, l = gl & 3;
   T v = T(0);
-  for (int s = grp_seg_off[g] * npc; s < grp_seg_off[g + 1] * npc; ++s) v += part[(int64_t(s) * 4 + l) * nbc + b];
-  cproj[t] = scale[gl] * v;
+  if (b < nbc)
+    for (int s = grp_seg_off[g] * npc + w; s < grp_seg_off[g + 1] * npc; s += 4)
+      v += part[(int64_t(s) * 4 + l) * nbc + b];
+  red[w][lane] = v;
+  __syncthreads();
+  if (w == 0 && b < nbc)
+    cproj[int64_t(gl) * nbc + b] = scale[gl] * ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
 }
 
 // ---- analytic projectors on the matrix cores ----
@@ -1203,8 +1213,8 @@ int launch_hpsi(rsg_ctx *ctx, const T *psi, T *hpsi, int64_t ld, int nb, const H
                            an.seg_grp.template as<int>(), an.pts.template as<uint32_t>(), an.es.template as<T>(),
                            an.ep.template as<T>(), an.dd.template as<T>(), an.amp.template as<T>(), an.S, part);
       }
-      const int64_t nred = int64_t(an.ngroup) * 4 * nbc;
-      hipLaunchKernelGGL((k_proj_reduce<T>), dim3(unsigned((nred + 255) / 256)), dim3(256), 0, ctx->stream, part,
+      const int64_t nrb = int64_t(an.ngroup) * 4 * ((nbc + 63) / 64);  // blocks: (g, l) rows x 64 bands
+      hipLaunchKernelGGL((k_proj_reduce<T>), dim3(unsigned(nrb)), dim3(256), 0, ctx->stream, part,
                          nbc, an.ngroup, (rows ? an.grp_slot_off : an.grp_seg_off).template as<int>(), npc,
                          an.scale.template as<T>(), cp);
       const bool nl_after = sh.order == 1;
